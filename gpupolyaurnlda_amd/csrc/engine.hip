@@ -1,0 +1,1090 @@
+// engine.hip -- host side of libGPUPolyaUrnLDA: the five-call ABI of the reference
+// (src/cu/train.cu:31-177), the device-resident corpus and the stage entry points.
+//
+// Device state (one process, one device, not re-entrant -- the reference's contract):
+//   phi      f32  V x K   word-major; phi~ after the Poisson draw, phi after normalisation
+//   alias    {f32 prob, u32 alias}  V x K word-major
+//   sigma_a  f32  V
+//   n_wk     u32  V x K   word-major counts (reduced from the sweep)
+//   topic_sum u64 K       exact sums of phi~ over words
+//   pois     {f32, u32}   100 x 200 Poisson alias tables (lambda = n + beta)
+//   segments: device copies of corpus slices (w u32, z u16, doc offsets u64, word-sorted
+//             permutation u32, reduction tasks), cached by host pointers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gpulda_abi.h"
+#include "../../include/gpulda_ext.h"
+#include "kernels.h"
+
+using namespace lda;
+
+namespace {
+
+std::string g_err;
+
+int fail(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return -1;
+}
+
+#define HCK(expr)                                                                                \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess) return fail("HIP error: %s %s:%d (%s)", hipGetErrorString(e_), __FILE__, \
+                                      __LINE__, #expr);                                          \
+  } while (0)
+
+#define CK(expr)            \
+  do {                      \
+    int r_ = (expr);        \
+    if (r_ != 0) return r_; \
+  } while (0)
+
+// The ABI calls have no return codes: like the reference (error.cuh:13-18) they print and exit.
+void abi_die(int rc) {
+  if (rc == 0) return;
+  printf("GPUPolyaUrnLDA error: %s\n", g_err.c_str());
+  fflush(stdout);
+  exit(rc < 0 ? 1 : rc);
+}
+
+template <class T>
+int dalloc(T** p, u64 n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  HCK(hipMalloc((void**)p, n * sizeof(T)));
+  return 0;
+}
+
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+u32 grid_for(u64 n, u32 block) {
+  u64 g = (n + block - 1) / block;
+  if (g > 65535u * 16) g = 65535u * 16;
+  return (u32)std::max<u64>(g, 1);
+}
+
+int fill_u32(u32* p, u64 n, u32 v, hipStream_t st) {
+  if (n == 0) return 0;
+  k_fill_u32<<<grid_for(n, 256), 256, 0, st>>>(p, n, v);
+  HCK(hipGetLastError());
+  return 0;
+}
+
+int fill_u64(unsigned long long* p, u64 n, u64 v, hipStream_t st) {
+  if (n == 0) return 0;
+  k_fill_u64<<<grid_for(n, 256), 256, 0, st>>>(p, n, (unsigned long long)v);
+  HCK(hipGetLastError());
+  return 0;
+}
+
+int max_lds_per_block() {
+  int dev = 0, v = 65536;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  return v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launchers shared by the ABI, the resident corpus and the stage entry points.
+// ---------------------------------------------------------------------------------------------
+int launch_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
+                      AliasSlot* out, hipStream_t st) {
+  if (n_rows == 0) return 0;
+  u64 per = alias_lds_per_wave(m);
+  u64 lim = (u64)max_lds_per_block();
+  if (per > lim) return fail("row length %u needs %llu B of LDS (limit %llu)", m, (unsigned long long)per,
+                             (unsigned long long)lim);
+  u32 wpb = (u32)std::min<u64>(4, std::max<u64>(1, lim / per));
+  u32 grid = (n_rows + wpb - 1) / wpb;
+  k_alias_rows<<<grid, 64 * wpb, per * wpb, st>>>(rows, n_rows, m, divisor, sigma, alpha, out, wpb);
+  HCK(hipGetLastError());
+  return 0;
+}
+
+constexpr u32 kPhiRows = 64;
+
+int launch_phi_draw(const u32* n, f32* phi, unsigned long long* tsum, u32 V, u32 K, f32 beta, u32 iter, u64 seed,
+                    const AliasSlot* tables, hipStream_t st) {
+  CK(fill_u64(tsum, K, 0, st));
+  dim3 grid((K + 255) / 256, (V + kPhiRows - 1) / kPhiRows);
+  k_phi_draw<<<grid, 256, 0, st>>>(n, phi, tsum, V, K, beta, iter, (u32)seed, (u32)(seed >> 32), tables, kPhiRows);
+  HCK(hipGetLastError());
+  return 0;
+}
+
+int launch_sweep(const SweepParams& P, u32 max_len, hipStream_t st) {
+  if (P.n_docs == 0) return 0;
+  u32 cap = map_capacity(max_len, P.K);
+  u64 lds = (u64)cap * sizeof(u32);
+  if (lds > (u64)max_lds_per_block()) return fail("doc-topic map of %u slots exceeds LDS", cap);
+  k_sweep<<<P.n_docs, 64, lds, st>>>(P, cap);
+  HCK(hipGetLastError());
+  return 0;
+}
+
+int launch_count(const CountTask* tasks, u32 n_tasks, const u32* perm, const u16* z, u32* n, u32 K, hipStream_t st) {
+  if (n_tasks == 0) return 0;
+  const u32 wpb = 4;
+  u64 lds = (u64)wpb * K * sizeof(u32);
+  k_count<<<(n_tasks + wpb - 1) / wpb, 64 * wpb, lds, st>>>(tasks, n_tasks, perm, z, n, K, wpb);
+  HCK(hipGetLastError());
+  return 0;
+}
+
+// Poisson PMF of lambda = n + beta over 0..199, in double, stored as f32 (poisson.cu:9-20).
+void poisson_pmf_rows(f32 beta, std::vector<f32>& pmf) {
+  pmf.resize((u64)kPoisTables * kPoisValues);
+  for (u32 n = 0; n < kPoisTables; ++n) {
+    double lam = (double)n + (double)beta;
+    for (u32 x = 0; x < kPoisValues; ++x)
+      pmf[(u64)n * kPoisValues + x] = (f32)std::exp((double)x * std::log(lam) - lam - std::lgamma((double)x + 1.0));
+  }
+}
+
+int build_poisson_tables(f32 beta, AliasSlot* tables, hipStream_t st) {
+  std::vector<f32> pmf;
+  poisson_pmf_rows(beta, pmf);
+  f32* d_pmf = nullptr;
+  CK(dalloc(&d_pmf, pmf.size()));
+  HCK(hipMemcpyAsync(d_pmf, pmf.data(), pmf.size() * sizeof(f32), hipMemcpyHostToDevice, st));
+  int rc = launch_alias_rows(d_pmf, kPoisTables, kPoisValues, nullptr, nullptr, 0.0f, tables, st);
+  if (rc == 0) {
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = fail("HIP error: %s building Poisson tables", hipGetErrorString(e));
+  }
+  dfree(d_pmf);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Corpus segments.
+// ---------------------------------------------------------------------------------------------
+struct Segment {
+  const u32* key_w = nullptr;
+  const u32* key_d = nullptr;
+  u32 n_docs = 0;
+  u64 n_tokens = 0;
+  u64 fingerprint = 0;
+  u32 max_len = 0;
+  u32 n_tasks = 0;
+  u32* w = nullptr;
+  u16* z = nullptr;
+  u64* doc_off = nullptr;
+  u32* perm = nullptr;
+  CountTask* tasks = nullptr;
+};
+
+void free_segment(Segment* s) {
+  if (!s) return;
+  dfree(s->w);
+  dfree(s->z);
+  dfree(s->doc_off);
+  dfree(s->perm);
+  dfree(s->tasks);
+  delete s;
+}
+
+u64 mix64(u64 h, u64 v) {
+  h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+  return h * 0xBF58476D1CE4E5B9ull;
+}
+
+// Cheap content fingerprint of a slice: all document lengths and 4096 sampled word ids.
+u64 slice_fingerprint(const u32* w, const u32* d, u32 n_docs, u64 n_tokens) {
+  u64 h = mix64(n_docs, n_tokens);
+  for (u32 i = 0; i < n_docs; ++i) h = mix64(h, d[i]);
+  if (n_tokens) {
+    u64 step = std::max<u64>(1, n_tokens / 4096);
+    for (u64 i = 0; i < n_tokens; i += step) h = mix64(h, w[i]);
+    h = mix64(h, w[n_tokens - 1]);
+  }
+  return h;
+}
+
+// Builds device copies of a slice: w, doc offsets, the word-sorted permutation (counting sort)
+// and the n_wk reduction tasks.  z is allocated, not filled.
+int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_tokens, u32 V, u32 K) {
+  std::vector<u64> off((u64)n_docs + 1);
+  off[0] = 0;
+  u32 max_len = 0;
+  for (u32 i = 0; i < n_docs; ++i) {
+    if (d[i] > kMaxDocLen) return fail("document %u has %u tokens (limit %u)", i, d[i], kMaxDocLen);
+    off[i + 1] = off[i] + d[i];
+    max_len = std::max(max_len, d[i]);
+  }
+  if (off[n_docs] != n_tokens)
+    return fail("document lengths sum to %llu, expected %llu tokens", (unsigned long long)off[n_docs],
+                (unsigned long long)n_tokens);
+  std::vector<u64> start((u64)V + 1, 0);
+  for (u64 i = 0; i < n_tokens; ++i) {
+    if (w[i] >= V) return fail("word id %u at token %llu is >= V = %u", w[i], (unsigned long long)i, V);
+    start[(u64)w[i] + 1]++;
+  }
+  std::vector<CountTask> tasks;
+  for (u32 v = 0; v < V; ++v) {
+    u64 cnt = start[(u64)v + 1];
+    start[(u64)v + 1] = start[v] + cnt;
+    for (u64 p = 0; p < cnt; p += kCountTask) {
+      CountTask t;
+      t.p0 = start[v] + p;
+      t.len = (u32)std::min<u64>(kCountTask, cnt - p);
+      t.word = v;
+      tasks.push_back(t);
+    }
+  }
+  if (n_tokens >= (1ull << 32)) return fail("a slice must hold fewer than 2^32 tokens");
+  std::vector<u32> perm(n_tokens);
+  {
+    std::vector<u64> cur(start.begin(), start.end() - 1);
+    for (u64 i = 0; i < n_tokens; ++i) perm[cur[w[i]]++] = (u32)i;
+  }
+  s->n_docs = n_docs;
+  s->n_tokens = n_tokens;
+  s->max_len = max_len;
+  s->n_tasks = (u32)tasks.size();
+  CK(dalloc(&s->w, n_tokens));
+  CK(dalloc(&s->z, n_tokens));
+  CK(dalloc(&s->doc_off, (u64)n_docs + 1));
+  CK(dalloc(&s->perm, n_tokens));
+  CK(dalloc(&s->tasks, tasks.size()));
+  if (n_tokens) {
+    HCK(hipMemcpy(s->w, w, n_tokens * sizeof(u32), hipMemcpyHostToDevice));
+    HCK(hipMemcpy(s->perm, perm.data(), n_tokens * sizeof(u32), hipMemcpyHostToDevice));
+  }
+  HCK(hipMemcpy(s->doc_off, off.data(), off.size() * sizeof(u64), hipMemcpyHostToDevice));
+  if (!tasks.empty()) HCK(hipMemcpy(s->tasks, tasks.data(), tasks.size() * sizeof(CountTask), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Library state.
+// ---------------------------------------------------------------------------------------------
+struct BufferState {
+  hipStream_t stream = nullptr;
+  u32* staging = nullptr;  // u32 z for the PCIe copies (buffer_size)
+  u32* bad = nullptr;      // device count of indicators >= K
+  Segment* seg = nullptr;  // segment of the last call (uncached mode owns it)
+};
+
+struct State {
+  bool ready = false;
+  Args* args = nullptr;
+  u32 K = 0, V = 0;
+  f32 alpha = 0, beta = 0;
+  u64 seed = 0;
+  u32 iter = 0;
+  u32 doc_counter = 0;
+  bool slice_cache = true;
+  hipStream_t stream = nullptr;  // Phi stream / resident stream
+  f32* phi = nullptr;
+  AliasSlot* alias = nullptr;
+  f32* sigma_a = nullptr;
+  u32* n = nullptr;
+  u32* C = nullptr;
+  unsigned long long* topic_sum = nullptr;
+  unsigned long long* visited = nullptr;
+  AliasSlot* pois = nullptr;
+  std::vector<Segment*> cache;
+  Segment* resident = nullptr;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+State S;
+u64 g_seed = 0;
+
+int require_ready() {
+  if (!S.ready) return fail("library not initialised (call initialize first)");
+  return 0;
+}
+
+int phi_step(hipStream_t st) {
+  S.iter += 1;
+  S.doc_counter = 0;
+  CK(launch_phi_draw(S.n, S.phi, S.topic_sum, S.V, S.K, S.beta, S.iter, S.seed, S.pois, st));
+  CK(launch_alias_rows(S.phi, S.V, S.K, S.topic_sum, S.sigma_a, S.alpha, S.alias, st));
+  CK(fill_u32(S.n, (u64)S.V * S.K, 0, st));
+  return 0;
+}
+
+SweepParams sweep_params(Segment* sg, u32 doc_base) {
+  SweepParams P;
+  P.w = sg->w;
+  P.z = sg->z;
+  P.doc_off = sg->doc_off;
+  P.doc_order = nullptr;
+  P.n_docs = sg->n_docs;
+  P.doc_base = doc_base;
+  P.iter = S.iter;
+  P.k0 = (u32)S.seed;
+  P.k1 = (u32)(S.seed >> 32);
+  P.K = S.K;
+  P.phi = S.phi;
+  P.sigma_a = S.sigma_a;
+  P.alias = S.alias;
+  P.visited = S.visited;
+  return P;
+}
+
+int sweep_and_count(Segment* sg, u32 doc_base, hipStream_t st) {
+  CK(launch_sweep(sweep_params(sg, doc_base), sg->max_len, st));
+  CK(launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, S.n, S.K, st));
+  return 0;
+}
+
+Segment* find_cached(const u32* w, const u32* d, u32 n_docs, u64 n_tokens, u64 fp) {
+  for (Segment* s : S.cache)
+    if (s->key_w == w && s->key_d == d && s->n_docs == n_docs && s->n_tokens == n_tokens && s->fingerprint == fp)
+      return s;
+  return nullptr;
+}
+
+int get_segment(BufferState* bs, const Buffer* B, Segment** out) {
+  const u32* w = B->w;
+  const u32* d = B->d;
+  if (S.slice_cache) {
+    u64 fp = slice_fingerprint(w, d, B->n_docs, B->n_tokens);
+    Segment* s = find_cached(w, d, B->n_docs, B->n_tokens, fp);
+    if (!s) {
+      s = new Segment();
+      int rc = build_segment(s, w, d, B->n_docs, B->n_tokens, S.V, S.K);
+      if (rc != 0 && g_err.find("out of memory") != std::string::npos) {
+        // drop every cached slice and retry once
+        HCK(hipDeviceSynchronize());
+        for (Segment* c : S.cache) free_segment(c);
+        S.cache.clear();
+        free_segment(s);
+        s = new Segment();
+        rc = build_segment(s, w, d, B->n_docs, B->n_tokens, S.V, S.K);
+      }
+      if (rc != 0) {
+        free_segment(s);
+        return rc;
+      }
+      s->key_w = w;
+      s->key_d = d;
+      s->fingerprint = fp;
+      S.cache.push_back(s);
+    }
+    *out = s;
+    return 0;
+  }
+  HCK(hipStreamSynchronize(bs->stream));
+  free_segment(bs->seg);
+  bs->seg = new Segment();
+  CK(build_segment(bs->seg, w, d, B->n_docs, B->n_tokens, S.V, S.K));
+  *out = bs->seg;
+  return 0;
+}
+
+void release_all() {
+  if (S.stream) (void)hipDeviceSynchronize();
+  for (Segment* s : S.cache) free_segment(s);
+  S.cache.clear();
+  free_segment(S.resident);
+  S.resident = nullptr;
+  dfree(S.phi);
+  dfree(S.alias);
+  dfree(S.sigma_a);
+  dfree(S.n);
+  dfree(S.C);
+  dfree(S.topic_sum);
+  dfree(S.visited);
+  dfree(S.pois);
+  for (auto& e : S.ev)
+    if (e) {
+      (void)hipEventDestroy(e);
+      e = nullptr;
+    }
+  if (S.stream) (void)hipStreamDestroy(S.stream);
+  S.stream = nullptr;
+  S.ready = false;
+}
+
+int do_initialize(Args* args, Buffer* buffers, u32 n_buffers) {
+  if (!args) return fail("initialize: args is NULL");
+  if (S.ready) release_all();
+  if (args->K == 0 || args->K > 4096) return fail("K = %u outside 1..4096", args->K);
+  if (args->V == 0) return fail("V must be positive");
+  if (!(args->alpha > 0.0f) || !(args->beta > 0.0f)) return fail("alpha and beta must be positive");
+  if (alias_lds_per_wave(args->K) > (u64)max_lds_per_block())
+    return fail("K = %u needs more LDS per alias row than a workgroup has", args->K);
+  S.args = args;
+  S.K = args->K;
+  S.V = args->V;
+  S.alpha = args->alpha;
+  S.beta = args->beta;
+  S.seed = g_seed;
+  S.iter = 0;
+  S.doc_counter = 0;
+  HCK(hipStreamCreate(&S.stream));
+  for (auto& e : S.ev) HCK(hipEventCreate(&e));
+  const u64 VK = (u64)S.V * S.K;
+  CK(dalloc(&S.phi, VK));
+  CK(dalloc(&S.alias, VK));
+  CK(dalloc(&S.sigma_a, S.V));
+  CK(dalloc(&S.n, VK));
+  CK(dalloc(&S.C, S.V));
+  CK(dalloc(&S.topic_sum, S.K));
+  CK(dalloc(&S.visited, 1));
+  CK(dalloc(&S.pois, (u64)kPoisTables * kPoisValues));
+  CK(fill_u64(S.visited, 1, 0, S.stream));
+  CK(fill_u64(S.topic_sum, S.K, 0, S.stream));
+  CK(build_poisson_tables(S.beta, S.pois, S.stream));
+  if (args->C) {
+    HCK(hipMemcpy(S.C, args->C, S.V * sizeof(u32), hipMemcpyHostToDevice));
+  } else {
+    CK(fill_u32(S.C, S.V, 0, S.stream));
+  }
+  // initial counts n_vk ~ Pois(floor(C_v / K) + beta) (train.cu:92, polyaurn.cu:61-79)
+  dim3 grid((S.K + 255) / 256, (S.V + kPhiRows - 1) / kPhiRows);
+  k_init_counts<<<grid, 256, 0, S.stream>>>(S.C, S.n, S.V, S.K, S.beta, (u32)S.seed, (u32)(S.seed >> 32), S.pois,
+                                           kPhiRows);
+  HCK(hipGetLastError());
+  // phi / alias hold the uniform table until the first sample_phi
+  CK(fill_u32((u32*)S.phi, VK, 0, S.stream));
+  CK(fill_u32((u32*)S.sigma_a, S.V, 0, S.stream));
+  for (u32 i = 0; i < n_buffers; ++i) {
+    Buffer* B = &buffers[i];
+    BufferState* bs = new BufferState();
+    HCK(hipStreamCreate(&bs->stream));
+    CK(dalloc(&bs->staging, std::max<u32>(1, args->buffer_size)));
+    CK(dalloc(&bs->bad, 1));
+    B->gpu_z = bs->staging;
+    B->gpu_w = nullptr;
+    B->gpu_d_len = nullptr;
+    B->gpu_d_idx = nullptr;
+    B->gpu_K_d = nullptr;
+    B->gpu_rng = bs;
+    B->stream = &bs->stream;
+  }
+  HCK(hipStreamSynchronize(S.stream));
+  S.ready = true;
+  return 0;
+}
+
+int do_sample_phi() {
+  CK(require_ready());
+  HCK(hipDeviceSynchronize());  // buffer sweeps of the previous iteration feed n_wk
+  CK(phi_step(S.stream));
+  HCK(hipStreamSynchronize(S.stream));
+  return 0;
+}
+
+int do_sample_z_async(Buffer* B) {
+  CK(require_ready());
+  if (!B || !B->gpu_rng) return fail("sample_z_async: buffer was not passed to initialize");
+  BufferState* bs = (BufferState*)B->gpu_rng;
+  if (B->n_docs > S.args->max_D) return fail("buffer holds %u documents, max_D is %u", B->n_docs, S.args->max_D);
+  if (B->n_tokens > S.args->buffer_size)
+    return fail("buffer holds %u tokens, buffer_size is %u", B->n_tokens, S.args->buffer_size);
+  Segment* sg = nullptr;
+  CK(get_segment(bs, B, &sg));
+  hipStream_t st = bs->stream;
+  const u64 N = B->n_tokens;
+  CK(fill_u32(bs->bad, 1, 0, st));
+  if (N) {
+    HCK(hipMemcpyAsync(bs->staging, B->z, N * sizeof(u32), hipMemcpyHostToDevice, st));
+    k_z_narrow<<<grid_for(N, 256), 256, 0, st>>>(bs->staging, sg->z, N, S.K, bs->bad);
+    HCK(hipGetLastError());
+  }
+  CK(sweep_and_count(sg, S.doc_counter, st));
+  if (N) {
+    k_z_widen<<<grid_for(N, 256), 256, 0, st>>>(sg->z, bs->staging, N);
+    HCK(hipGetLastError());
+    HCK(hipMemcpyAsync(B->z, bs->staging, N * sizeof(u32), hipMemcpyDeviceToHost, st));
+  }
+  S.doc_counter += B->n_docs;
+  B->gpu_w = sg->w;
+  B->gpu_d_len = sg->doc_off;
+  return 0;
+}
+
+int do_sync_buffer(Buffer* B) {
+  CK(require_ready());
+  if (!B || !B->gpu_rng) return fail("sync_buffer: buffer was not passed to initialize");
+  BufferState* bs = (BufferState*)B->gpu_rng;
+  HCK(hipStreamSynchronize(bs->stream));
+  u32 bad = 0;
+  HCK(hipMemcpy(&bad, bs->bad, sizeof(u32), hipMemcpyDeviceToHost));
+  if (bad) return fail("%u topic indicators were >= K = %u (treated as topic 0)", bad, S.K);
+  return 0;
+}
+
+int do_cleanup(Buffer* buffers, u32 n_buffers) {
+  if (!S.ready) return fail("cleanup: library not initialised");
+  HCK(hipDeviceSynchronize());
+  for (u32 i = 0; i < n_buffers; ++i) {
+    Buffer* B = &buffers[i];
+    BufferState* bs = (BufferState*)B->gpu_rng;
+    if (!bs) continue;
+    free_segment(bs->seg);
+    dfree(bs->staging);
+    dfree(bs->bad);
+    if (bs->stream) (void)hipStreamDestroy(bs->stream);
+    delete bs;
+    B->gpu_z = B->gpu_w = B->gpu_d_len = B->gpu_d_idx = B->gpu_K_d = B->gpu_rng = B->stream = nullptr;
+  }
+  release_all();
+  S.args = nullptr;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Log-likelihood (double): per-word and per-document partials, summed on the host in order.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_ll_words(const u32* __restrict__ n, u32 V, u32 K, double beta, double* out_word,
+                           unsigned long long* topic_tot) {
+  u32 v = blockIdx.x;
+  if (v >= V) return;
+  __shared__ double part[256];
+  double s = 0.0;
+  double lgb = lgamma(beta);
+  for (u32 k = threadIdx.x; k < K; k += blockDim.x) {
+    u32 c = n[(u64)v * K + k];
+    if (c) {
+      s += lgamma(beta + (double)c) - lgb;
+      atomicAdd(&topic_tot[k], (unsigned long long)c);
+    }
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (u32 o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out_word[v] = part[0];
+}
+
+__global__ void k_ll_docs(const u16* __restrict__ z, const u64* __restrict__ doc_off, u32 n_docs, u32 K, double alpha,
+                          double* out_doc) {
+  extern __shared__ u32 hist[];
+  u32 doc = blockIdx.x;
+  if (doc >= n_docs) return;
+  __shared__ double part[256];
+  for (u32 k = threadIdx.x; k < K; k += blockDim.x) hist[k] = 0;
+  __syncthreads();
+  u64 b = doc_off[doc];
+  u32 L = (u32)(doc_off[doc + 1] - b);
+  for (u32 i = threadIdx.x; i < L; i += blockDim.x) atomicAdd(&hist[z[b + i]], 1u);
+  __syncthreads();
+  double s = 0.0, lga = lgamma(alpha);
+  for (u32 k = threadIdx.x; k < K; k += blockDim.x)
+    if (hist[k]) s += lgamma(alpha + (double)hist[k]) - lga;
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (u32 o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out_doc[doc] = part[0] + lgamma((double)K * alpha) - lgamma((double)K * alpha + (double)L);
+}
+
+}  // namespace
+
+// =============================================================================================
+// The five-call ABI (gpulda_abi.h).
+// =============================================================================================
+extern "C" void initialize(Args* args, Buffer* buffers, uint32_t n_buffers) {
+  abi_die(do_initialize(args, buffers, n_buffers));
+}
+extern "C" void sample_phi(void) { abi_die(do_sample_phi()); }
+extern "C" void sample_z_async(Buffer* buffer) { abi_die(do_sample_z_async(buffer)); }
+extern "C" void sync_buffer(Buffer* buffer) { abi_die(do_sync_buffer(buffer)); }
+extern "C" void cleanup(Buffer* buffers, uint32_t n_buffers) { abi_die(do_cleanup(buffers, n_buffers)); }
+
+// =============================================================================================
+// Extensions (gpulda_ext.h).
+// =============================================================================================
+extern "C" int gpulda_set_seed(uint64_t seed) {
+  g_seed = seed;
+  return 0;
+}
+extern "C" uint64_t gpulda_get_seed(void) { return g_seed; }
+extern "C" const char* gpulda_last_error(void) { return g_err.c_str(); }
+
+extern "C" int gpulda_device_info(char* name, int name_len, int* cu_count, uint64_t* hbm_bytes, int* lds_per_block) {
+  int dev = 0;
+  HCK(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  HCK(hipGetDeviceProperties(&p, dev));
+  if (name && name_len > 0) {
+    snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  if (cu_count) *cu_count = p.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = (uint64_t)p.totalGlobalMem;
+  if (lds_per_block) *lds_per_block = max_lds_per_block();
+  return 0;
+}
+
+extern "C" int gpulda_set_slice_cache(int enabled) {
+  S.slice_cache = enabled != 0;
+  return 0;
+}
+
+extern "C" uint32_t gpulda_iteration(void) { return S.iter; }
+
+#define COPY_OUT(dst, src, count)                                                                  \
+  do {                                                                                             \
+    CK(require_ready());                                                                           \
+    HCK(hipDeviceSynchronize());                                                                   \
+    HCK(hipMemcpy((dst), (src), (u64)(count) * sizeof(*(src)), hipMemcpyDeviceToHost));            \
+    return 0;                                                                                      \
+  } while (0)
+
+extern "C" int gpulda_copy_phi(float* out) { COPY_OUT(out, S.phi, (u64)S.V * S.K); }
+extern "C" int gpulda_copy_sigma_a(float* out) { COPY_OUT(out, S.sigma_a, S.V); }
+extern "C" int gpulda_copy_counts(uint32_t* out) { COPY_OUT(out, S.n, (u64)S.V * S.K); }
+extern "C" int gpulda_copy_topic_sums(uint64_t* out) {
+  COPY_OUT(reinterpret_cast<unsigned long long*>(out), S.topic_sum, S.K);
+}
+
+static int split_alias(const AliasSlot* d, u64 count, float* prob, uint32_t* alias) {
+  std::vector<AliasSlot> h(count);
+  HCK(hipDeviceSynchronize());
+  HCK(hipMemcpy(h.data(), d, count * sizeof(AliasSlot), hipMemcpyDeviceToHost));
+  for (u64 i = 0; i < count; ++i) {
+    if (prob) prob[i] = h[i].prob;
+    if (alias) alias[i] = h[i].alias;
+  }
+  return 0;
+}
+
+extern "C" int gpulda_copy_alias(float* prob, uint32_t* alias) {
+  CK(require_ready());
+  return split_alias(S.alias, (u64)S.V * S.K, prob, alias);
+}
+
+extern "C" int gpulda_copy_poisson_tables(float* prob, uint32_t* alias) {
+  CK(require_ready());
+  return split_alias(S.pois, (u64)kPoisTables * kPoisValues, prob, alias);
+}
+
+extern "C" int gpulda_set_counts(const uint32_t* n) {
+  CK(require_ready());
+  HCK(hipDeviceSynchronize());
+  HCK(hipMemcpy(S.n, n, (u64)S.V * S.K * sizeof(u32), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int gpulda_sweep_stats(uint64_t* visited) {
+  COPY_OUT(reinterpret_cast<unsigned long long*>(visited), S.visited, 1);
+}
+
+extern "C" int gpulda_reset_stats(void) {
+  CK(require_ready());
+  CK(fill_u64(S.visited, 1, 0, S.stream));
+  HCK(hipStreamSynchronize(S.stream));
+  return 0;
+}
+
+// ---- resident corpus --------------------------------------------------------------------------
+extern "C" int gpulda_corpus_load(const uint32_t* w, const uint32_t* d_len, uint32_t n_docs, const uint32_t* z) {
+  CK(require_ready());
+  if (!w || !d_len) return fail("corpus_load: w and d_len are required");
+  HCK(hipDeviceSynchronize());
+  free_segment(S.resident);
+  S.resident = nullptr;
+  u64 N = 0;
+  for (u32 i = 0; i < n_docs; ++i) N += d_len[i];
+  Segment* sg = new Segment();
+  int rc = build_segment(sg, w, d_len, n_docs, N, S.V, S.K);
+  if (rc) {
+    free_segment(sg);
+    return rc;
+  }
+  if (z) {
+    u32* stage = nullptr;
+    u32* bad = nullptr;
+    CK(dalloc(&stage, N));
+    CK(dalloc(&bad, 1));
+    CK(fill_u32(bad, 1, 0, S.stream));
+    HCK(hipMemcpyAsync(stage, z, N * sizeof(u32), hipMemcpyHostToDevice, S.stream));
+    k_z_narrow<<<grid_for(N, 256), 256, 0, S.stream>>>(stage, sg->z, N, S.K, bad);
+    HCK(hipGetLastError());
+    u32 hbad = 0;
+    HCK(hipMemcpyAsync(&hbad, bad, sizeof(u32), hipMemcpyDeviceToHost, S.stream));
+    HCK(hipStreamSynchronize(S.stream));
+    dfree(stage);
+    dfree(bad);
+    if (hbad) {
+      free_segment(sg);
+      return fail("%u initial topic indicators are >= K", hbad);
+    }
+  } else if (n_docs) {
+    k_z_init<<<n_docs, 256, 0, S.stream>>>(sg->z, sg->doc_off, n_docs, 0, S.K, (u32)S.seed, (u32)(S.seed >> 32));
+    HCK(hipGetLastError());
+    HCK(hipStreamSynchronize(S.stream));
+  }
+  S.resident = sg;
+  return 0;
+}
+
+extern "C" int gpulda_corpus_counts_from_z(void) {
+  CK(require_ready());
+  if (!S.resident) return fail("no resident corpus");
+  Segment* sg = S.resident;
+  CK(fill_u32(S.n, (u64)S.V * S.K, 0, S.stream));
+  CK(launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, S.n, S.K, S.stream));
+  HCK(hipStreamSynchronize(S.stream));
+  return 0;
+}
+
+extern "C" int gpulda_corpus_copy_z(uint32_t* z) {
+  CK(require_ready());
+  if (!S.resident) return fail("no resident corpus");
+  Segment* sg = S.resident;
+  u64 N = sg->n_tokens;
+  u32* stage = nullptr;
+  CK(dalloc(&stage, N));
+  if (N) {
+    k_z_widen<<<grid_for(N, 256), 256, 0, S.stream>>>(sg->z, stage, N);
+    HCK(hipGetLastError());
+    HCK(hipMemcpyAsync(z, stage, N * sizeof(u32), hipMemcpyDeviceToHost, S.stream));
+  }
+  HCK(hipStreamSynchronize(S.stream));
+  dfree(stage);
+  return 0;
+}
+
+extern "C" int gpulda_corpus_iterate(uint32_t n_iters, float* stage_ms) {
+  CK(require_ready());
+  if (!S.resident) return fail("no resident corpus");
+  Segment* sg = S.resident;
+  hipStream_t st = S.stream;
+  double acc[4] = {0, 0, 0, 0};
+  for (u32 it = 0; it < n_iters; ++it) {
+    S.iter += 1;
+    S.doc_counter = 0;
+    HCK(hipEventRecord(S.ev[0], st));
+    CK(launch_phi_draw(S.n, S.phi, S.topic_sum, S.V, S.K, S.beta, S.iter, S.seed, S.pois, st));
+    HCK(hipEventRecord(S.ev[1], st));
+    CK(launch_alias_rows(S.phi, S.V, S.K, S.topic_sum, S.sigma_a, S.alpha, S.alias, st));
+    CK(fill_u32(S.n, (u64)S.V * S.K, 0, st));
+    HCK(hipEventRecord(S.ev[2], st));
+    CK(launch_sweep(sweep_params(sg, 0), sg->max_len, st));
+    HCK(hipEventRecord(S.ev[3], st));
+    CK(launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, S.n, S.K, st));
+    HCK(hipEventRecord(S.ev[4], st));
+    if (stage_ms) {
+      HCK(hipEventSynchronize(S.ev[4]));
+      for (int s = 0; s < 4; ++s) {
+        float ms = 0;
+        HCK(hipEventElapsedTime(&ms, S.ev[s], S.ev[s + 1]));
+        acc[s] += ms;
+      }
+    }
+    S.doc_counter = sg->n_docs;
+  }
+  HCK(hipStreamSynchronize(st));
+  if (stage_ms)
+    for (int s = 0; s < 4; ++s) stage_ms[s] = n_iters ? (float)(acc[s] / n_iters) : 0.0f;
+  return 0;
+}
+
+extern "C" int gpulda_corpus_loglik(double* out) {
+  CK(require_ready());
+  if (!S.resident) return fail("no resident corpus");
+  Segment* sg = S.resident;
+  double* dw = nullptr;
+  double* dd = nullptr;
+  unsigned long long* tot = nullptr;
+  CK(dalloc(&dw, S.V));
+  CK(dalloc(&dd, std::max<u32>(1, sg->n_docs)));
+  CK(dalloc(&tot, S.K));
+  CK(fill_u64(tot, S.K, 0, S.stream));
+  k_ll_words<<<S.V, 256, 0, S.stream>>>(S.n, S.V, S.K, (double)S.beta, dw, tot);
+  HCK(hipGetLastError());
+  if (sg->n_docs) {
+    k_ll_docs<<<sg->n_docs, 256, (u64)S.K * sizeof(u32), S.stream>>>(sg->z, sg->doc_off, sg->n_docs, S.K,
+                                                                       (double)S.alpha, dd);
+    HCK(hipGetLastError());
+  }
+  std::vector<double> hw(S.V), hd(sg->n_docs);
+  std::vector<unsigned long long> ht(S.K);
+  HCK(hipStreamSynchronize(S.stream));
+  HCK(hipMemcpy(hw.data(), dw, S.V * sizeof(double), hipMemcpyDeviceToHost));
+  if (sg->n_docs) HCK(hipMemcpy(hd.data(), dd, sg->n_docs * sizeof(double), hipMemcpyDeviceToHost));
+  HCK(hipMemcpy(ht.data(), tot, S.K * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  dfree(dw);
+  dfree(dd);
+  dfree(tot);
+  double ll = 0.0;
+  for (double x : hw) ll += x;
+  double vb = (double)S.V * (double)S.beta;
+  for (u32 k = 0; k < S.K; ++k) ll += std::lgamma(vb) - std::lgamma(vb + (double)ht[k]);
+  for (double x : hd) ll += x;
+  *out = ll;
+  return 0;
+}
+
+extern "C" int gpulda_corpus_release(void) {
+  HCK(hipDeviceSynchronize());
+  free_segment(S.resident);
+  S.resident = nullptr;
+  return 0;
+}
+
+// ---- stage entry points -----------------------------------------------------------------------
+namespace {
+// RAII bundle of temporary device buffers for the stage entry points.
+struct Scratch {
+  std::vector<void*> ptrs;
+  ~Scratch() {
+    (void)hipDeviceSynchronize();
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <class T>
+  int alloc(T** p, u64 n) {
+    CK(dalloc(p, n));
+    ptrs.push_back((void*)*p);
+    return 0;
+  }
+  template <class T>
+  int upload(T** p, const T* h, u64 n) {
+    CK(alloc(p, n));
+    if (n) HCK(hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+  }
+};
+
+int pack_alias(Scratch& sc, const float* prob, const uint32_t* alias, u64 n, AliasSlot** out) {
+  std::vector<AliasSlot> h(n);
+  for (u64 i = 0; i < n; ++i) {
+    h[i].prob = prob[i];
+    h[i].alias = alias[i];
+  }
+  return sc.upload(out, h.data(), n);
+}
+
+int stage_tables(Scratch& sc, f32 beta, AliasSlot** out) {
+  CK(sc.alloc(out, (u64)kPoisTables * kPoisValues));
+  return build_poisson_tables(beta, *out, 0);
+}
+}  // namespace
+
+extern "C" int gpulda_stage_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n) {
+  Scratch sc;
+  u32 *dc, *dk, *dout;
+  CK(sc.upload(&dc, ctr, 4ull * n));
+  CK(sc.upload(&dk, key, 2ull * n));
+  CK(sc.alloc(&dout, 4ull * n));
+  if (n) {
+    k_stage_philox<<<(n + 255) / 256, 256>>>(dc, dk, dout, n);
+    HCK(hipGetLastError());
+    HCK(hipMemcpy(out, dout, 16ull * n, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+extern "C" int gpulda_stage_scan64(const float* in, float* out, uint32_t n_groups) {
+  Scratch sc;
+  f32 *di, *dout;
+  CK(sc.upload(&di, in, 64ull * n_groups));
+  CK(sc.alloc(&dout, 64ull * n_groups));
+  if (n_groups) {
+    k_stage_scan<<<n_groups, 64>>>(di, dout, n_groups);
+    HCK(hipGetLastError());
+    HCK(hipMemcpy(out, dout, 256ull * n_groups, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+extern "C" int gpulda_stage_map(const uint32_t* ops, uint32_t n_ops, uint32_t cap, uint32_t* table_out) {
+  if (cap == 0 || cap % kMapLine) return fail("map capacity must be a positive multiple of 64");
+  if ((u64)cap * 4 > (u64)max_lds_per_block()) return fail("map capacity exceeds LDS");
+  Scratch sc;
+  u32 *dops, *dout;
+  CK(sc.upload(&dops, ops, n_ops));
+  CK(sc.alloc(&dout, cap));
+  k_stage_map<<<1, 64, (u64)cap * 4>>>(dops, n_ops, cap, dout);
+  HCK(hipGetLastError());
+  HCK(hipMemcpy(table_out, dout, (u64)cap * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int gpulda_stage_alias(const float* rows, uint32_t n_rows, uint32_t m, const uint64_t* divisor, float alpha,
+                                  float* rows_out, float* sums_out, float* prob_out, uint32_t* alias_out) {
+  Scratch sc;
+  f32 *dr, *ds;
+  unsigned long long* dd = nullptr;
+  AliasSlot* da;
+  const u64 total = (u64)n_rows * m;
+  CK(sc.upload(&dr, rows, total));
+  CK(sc.alloc(&ds, n_rows));
+  CK(sc.alloc(&da, total));
+  if (divisor) CK(sc.upload(&dd, reinterpret_cast<const unsigned long long*>(divisor), m));
+  CK(launch_alias_rows(dr, n_rows, m, dd, ds, alpha, da, 0));
+  HCK(hipDeviceSynchronize());
+  if (rows_out) HCK(hipMemcpy(rows_out, dr, total * 4, hipMemcpyDeviceToHost));
+  if (sums_out) HCK(hipMemcpy(sums_out, ds, (u64)n_rows * 4, hipMemcpyDeviceToHost));
+  return split_alias(da, total, prob_out, alias_out);
+}
+
+extern "C" int gpulda_stage_alias_draw(const float* prob, const uint32_t* alias, uint32_t m, const uint32_t* r_slot,
+                                       const uint32_t* r_keep, uint32_t count, uint32_t* out) {
+  Scratch sc;
+  AliasSlot* da;
+  u32 *ds, *dk, *dout;
+  CK(pack_alias(sc, prob, alias, m, &da));
+  CK(sc.upload(&ds, r_slot, count));
+  CK(sc.upload(&dk, r_keep, count));
+  CK(sc.alloc(&dout, count));
+  if (count) {
+    k_stage_alias_draw<<<(count + 255) / 256, 256>>>(da, m, ds, dk, count, dout);
+    HCK(hipGetLastError());
+    HCK(hipMemcpy(out, dout, (u64)count * 4, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+extern "C" int gpulda_stage_poisson(const uint32_t* n, uint32_t count, float beta, uint32_t c1, uint32_t c2,
+                                    uint32_t tag, uint64_t seed, uint32_t* out) {
+  Scratch sc;
+  AliasSlot* tables;
+  u32 *dn, *dout;
+  CK(stage_tables(sc, beta, &tables));
+  CK(sc.upload(&dn, n, count));
+  CK(sc.alloc(&dout, count));
+  if (count) {
+    k_stage_poisson<<<(count + 255) / 256, 256>>>(dn, count, beta, tables, c1, c2, tag, (u32)seed, (u32)(seed >> 32),
+                                                 dout);
+    HCK(hipGetLastError());
+    HCK(hipMemcpy(out, dout, (u64)count * 4, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+extern "C" int gpulda_stage_phi_draw(const uint32_t* n, uint32_t V, uint32_t K, float beta, uint32_t iter,
+                                     uint64_t seed, float* phi_tilde, uint64_t* topic_sum) {
+  Scratch sc;
+  AliasSlot* tables;
+  u32* dn;
+  f32* dphi;
+  unsigned long long* dts;
+  const u64 VK = (u64)V * K;
+  CK(stage_tables(sc, beta, &tables));
+  CK(sc.upload(&dn, n, VK));
+  CK(sc.alloc(&dphi, VK));
+  CK(sc.alloc(&dts, K));
+  CK(launch_phi_draw(dn, dphi, dts, V, K, beta, iter, seed, tables, 0));
+  HCK(hipDeviceSynchronize());
+  HCK(hipMemcpy(phi_tilde, dphi, VK * 4, hipMemcpyDeviceToHost));
+  HCK(hipMemcpy(topic_sum, dts, (u64)K * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int gpulda_stage_sweep(uint32_t* z, const uint32_t* w, const uint64_t* doc_off, uint32_t n_docs,
+                                  uint32_t doc_base, uint32_t iter, uint64_t seed, uint32_t K, uint32_t V,
+                                  const float* phi, const float* sigma_a, const float* prob, const uint32_t* alias,
+                                  uint64_t* visited) {
+  if (K == 0 || K > kMaxTopics) return fail("bad K");
+  const u64 N = doc_off[n_docs];
+  u32 max_len = 0;
+  for (u32 d = 0; d < n_docs; ++d) {
+    u64 L = doc_off[d + 1] - doc_off[d];
+    if (L > kMaxDocLen) return fail("document %u longer than %u tokens", d, kMaxDocLen);
+    max_len = std::max<u32>(max_len, (u32)L);
+  }
+  for (u64 i = 0; i < N; ++i)
+    if (z[i] >= K || w[i] >= V) return fail("token %llu has z >= K or w >= V", (unsigned long long)i);
+  const u64 VK = (u64)V * K;
+  Scratch sc;
+  u32 *dw, *dz32;
+  u16* dz;
+  u64* doff;
+  f32 *dphi, *dsig;
+  AliasSlot* dal;
+  unsigned long long* dvis;
+  u32* bad;
+  CK(sc.upload(&dw, w, N));
+  CK(sc.upload(&dz32, z, N));
+  CK(sc.alloc(&dz, N));
+  CK(sc.upload(&doff, doc_off, (u64)n_docs + 1));
+  CK(sc.upload(&dphi, phi, VK));
+  CK(sc.upload(&dsig, sigma_a, V));
+  CK(pack_alias(sc, prob, alias, VK, &dal));
+  CK(sc.alloc(&dvis, 1));
+  CK(sc.alloc(&bad, 1));
+  CK(fill_u64(dvis, 1, 0, 0));
+  CK(fill_u32(bad, 1, 0, 0));
+  if (N) {
+    k_z_narrow<<<grid_for(N, 256), 256>>>(dz32, dz, N, K, bad);
+    HCK(hipGetLastError());
+  }
+  SweepParams P;
+  P.w = dw;
+  P.z = dz;
+  P.doc_off = doff;
+  P.doc_order = nullptr;
+  P.n_docs = n_docs;
+  P.doc_base = doc_base;
+  P.iter = iter;
+  P.k0 = (u32)seed;
+  P.k1 = (u32)(seed >> 32);
+  P.K = K;
+  P.phi = dphi;
+  P.sigma_a = dsig;
+  P.alias = dal;
+  P.visited = dvis;
+  CK(launch_sweep(P, max_len, 0));
+  if (N) {
+    k_z_widen<<<grid_for(N, 256), 256>>>(dz, dz32, N);
+    HCK(hipGetLastError());
+  }
+  HCK(hipDeviceSynchronize());
+  if (N) HCK(hipMemcpy(z, dz32, N * 4, hipMemcpyDeviceToHost));
+  if (visited) HCK(hipMemcpy(visited, dvis, 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int gpulda_stage_count(const uint32_t* w, const uint32_t* z, uint64_t n_tokens, uint32_t V, uint32_t K,
+                                  uint32_t* n_out) {
+  if (K == 0 || K > kMaxTopics) return fail("bad K");
+  for (u64 i = 0; i < n_tokens; ++i)
+    if (z[i] >= K) return fail("token %llu has z >= K", (unsigned long long)i);
+  if (n_tokens >= (1ull << 32)) return fail("too many tokens");
+  // one document holding every token (the reduction does not look at documents)
+  u32 one = (u32)n_tokens;
+  Segment* sg = new Segment();
+  int rc = build_segment(sg, w, &one, n_tokens ? 1 : 0, n_tokens, V, K);
+  if (rc == 0 && n_tokens == 0) rc = 0;
+  if (rc) {
+    free_segment(sg);
+    return rc;
+  }
+  Scratch sc;
+  u32 *dz32, *dn, *bad;
+  const u64 VK = (u64)V * K;
+  int r2 = sc.upload(&dz32, z, n_tokens);
+  if (!r2) r2 = sc.alloc(&dn, VK);
+  if (!r2) r2 = sc.alloc(&bad, 1);
+  if (!r2) r2 = fill_u32(dn, VK, 0, 0);
+  if (!r2) r2 = fill_u32(bad, 1, 0, 0);
+  if (!r2 && n_tokens) {
+    k_z_narrow<<<grid_for(n_tokens, 256), 256>>>(dz32, sg->z, n_tokens, K, bad);
+    if (hipGetLastError() != hipSuccess) r2 = fail("launch failed");
+  }
+  if (!r2) r2 = launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, dn, K, 0);
+  if (!r2 && hipDeviceSynchronize() != hipSuccess) r2 = fail("count failed");
+  if (!r2 && hipMemcpy(n_out, dn, VK * 4, hipMemcpyDeviceToHost) != hipSuccess) r2 = fail("copy failed");
+  free_segment(sg);
+  return r2;
+}

@@ -1,0 +1,58 @@
+// kernels.h -- device kernel declarations and their parameter blocks.
+#pragma once
+
+#include "lda_core.h"
+
+namespace lda {
+
+// One word-sorted run of tokens for the n_wk reduction.
+struct CountTask {
+  u64 p0;    // first position in the word-sorted permutation
+  u32 len;   // number of tokens (<= kCountTask)
+  u32 word;  // word id of the run
+};
+constexpr u32 kCountTask = 4096;
+
+struct SweepParams {
+  const u32* w;          // word ids, doc-major
+  u16* z;                // topic indicators (read, then overwritten)
+  const u64* doc_off;    // n_docs + 1 offsets into w / z
+  const u32* doc_order;  // optional launch order (block -> doc); null = identity
+  u32 n_docs;
+  u32 doc_base;          // global id of document 0 (Philox counter word 1)
+  u32 iter;              // iteration (Philox counter word 2)
+  u32 k0, k1;            // Philox key = seed
+  u32 K;
+  const f32* phi;        // V x K word-major
+  const f32* sigma_a;    // V
+  const AliasSlot* alias;  // V x K word-major
+  unsigned long long* visited;  // optional: += map entries visited by the sparse sums
+};
+
+__global__ void k_fill_u32(u32* p, u64 n, u32 v);
+__global__ void k_fill_u64(unsigned long long* p, u64 n, unsigned long long v);
+__global__ void k_z_narrow(const u32* in, u16* out, u64 n, u32 K, u32* bad);
+__global__ void k_z_widen(const u16* in, u32* out, u64 n);
+__global__ void k_z_init(u16* z, const u64* doc_off, u32 n_docs, u32 doc_base, u32 K, u32 k0, u32 k1);
+__global__ void k_phi_draw(const u32* n, f32* phi, unsigned long long* topic_sum, u32 V, u32 K, f32 beta, u32 iter,
+                           u32 k0, u32 k1, const AliasSlot* tables, u32 rows_per_block);
+__global__ void k_init_counts(const u32* C, u32* n, u32 V, u32 K, f32 beta, u32 k0, u32 k1, const AliasSlot* tables,
+                              u32 rows_per_block);
+__global__ void k_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
+                             AliasSlot* out, u32 rows_per_block_waves);
+__global__ void k_sweep(SweepParams P, u32 cap_max);
+__global__ void k_count(const CountTask* tasks, u32 n_tasks, const u32* perm, const u16* z, u32* n, u32 K,
+                        u32 waves_per_block);
+
+__global__ void k_stage_philox(const u32* ctr, const u32* key, u32* out, u32 n);
+__global__ void k_stage_scan(const f32* in, f32* out, u32 n_groups);
+__global__ void k_stage_map(const u32* ops, u32 n_ops, u32 cap, u32* out);
+__global__ void k_stage_poisson(const u32* nval, u32 count, f32 beta, const AliasSlot* tables, u32 c1, u32 c2,
+                                u32 tag, u32 k0, u32 k1, u32* out);
+__global__ void k_stage_alias_draw(const AliasSlot* row, u32 m, const u32* r_slot, const u32* r_keep, u32 count,
+                                   u32* out);
+
+// LDS bytes per alias row of length m (one wavefront).
+inline u64 alias_lds_per_wave(u32 m) { return (u64)((m + 1) & ~1u) * 5 * sizeof(u32); }
+
+}  // namespace lda

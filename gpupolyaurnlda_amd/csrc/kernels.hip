@@ -1,0 +1,421 @@
+// kernels.hip -- gfx950 kernels of one Polya-urn LDA iteration.
+//
+//   k_phi_draw      Poisson Polya-urn draw phi~_vk ~ Pois(n_vk + beta) and exact topic sums
+//                   (replaces polya_urn_sample, src/cu/polyaurn.cu:87-123, and polya_urn_init,
+//                   :61-79, whose non-integer Gaussian branch and float atomics are dropped)
+//   k_alias_rows    per-row normalise + canonical row sum + sigma_a + exact alias split
+//                   (replaces polya_urn_colsums, polyaurn.cu:144-178, build_alias,
+//                   src/cu/spalias.cu:88-174, and the cuBLAS transpose, polyaurn.cu:127-131,
+//                   which word-major storage makes unnecessary)
+//   k_sweep         one wavefront per document: the z-sweep over an LDS Robin Hood map
+//                   (replaces sample_topics, src/cu/topics.cu:36-105)
+//   k_count         word-sorted n_wk reduction through per-wave LDS histograms
+//                   (replaces the per-token atomicAdd of topics.cu:95 and polya_urn_reset,
+//                   polyaurn.cu:133-141)
+#include "kernels.h"
+#include "wave.h"
+
+namespace lda {
+
+// ---------------------------------------------------------------------------------------------
+// Small utility kernels (fills are kernels, not hipMemsetAsync, so that an iteration can be
+// captured into a hipGraph and replayed).
+// ---------------------------------------------------------------------------------------------
+__global__ void k_fill_u32(u32* p, u64 n, u32 v) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, u64 n, unsigned long long v) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// u32 topic indicators at the ABI -> u16 on the device; counts indicators >= K into *bad.
+__global__ void k_z_narrow(const u32* in, u16* out, u64 n, u32 K, u32* bad) {
+  u32 nbad = 0;
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
+    u32 z = in[i];
+    if (z >= K) {
+      ++nbad;
+      z = 0;
+    }
+    out[i] = (u16)z;
+  }
+  if (nbad) atomicAdd(bad, nbad);
+}
+
+__global__ void k_z_widen(const u16* in, u32* out, u64 n) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) out[i] = in[i];
+}
+
+// Initial topic indicators z_i ~ Uniform{0..K-1} keyed by (token, doc).
+__global__ void k_z_init(u16* z, const u64* doc_off, u32 n_docs, u32 doc_base, u32 K, u32 k0, u32 k1) {
+  u32 doc = blockIdx.x;
+  if (doc >= n_docs) return;
+  u64 b = doc_off[doc];
+  u32 L = (u32)(doc_off[doc + 1] - b);
+  for (u32 i = threadIdx.x; i < L; i += blockDim.x) {
+    u32x4 r = philox_tagged(i, doc_base + doc, 0, kTagInitZ, 0, k0, k1);
+    z[b + i] = (u16)bounded(r.x, K);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Phi step.  Thread = topic column k, block.y = a tile of words; topic sums are exact u64.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_phi_draw(const u32* __restrict__ n, f32* __restrict__ phi, unsigned long long* topic_sum,
+                           u32 V, u32 K, f32 beta, u32 iter, u32 k0, u32 k1,
+                           const AliasSlot* __restrict__ tables, u32 rows_per_block) {
+  u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  u32 v0 = blockIdx.y * rows_per_block;
+  u32 v1 = v0 + rows_per_block < V ? v0 + rows_per_block : V;
+  u64 acc = 0;
+  for (u32 v = v0; v < v1; ++v) {
+    u64 idx = (u64)v * K + k;
+    u32 x = poisson_draw(n[idx], beta, tables, k, v, iter, kTagPhi, k0, k1);
+    phi[idx] = (f32)x;
+    acc += x;
+  }
+  if (acc) atomicAdd(&topic_sum[k], (unsigned long long)acc);
+}
+
+// n_vk ~ Pois(floor(C_v / K) + beta) at initialisation (polya_urn_init, polyaurn.cu:61-79).
+__global__ void k_init_counts(const u32* __restrict__ C, u32* __restrict__ n, u32 V, u32 K, f32 beta, u32 k0,
+                              u32 k1, const AliasSlot* __restrict__ tables, u32 rows_per_block) {
+  u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  u32 v0 = blockIdx.y * rows_per_block;
+  u32 v1 = v0 + rows_per_block < V ? v0 + rows_per_block : V;
+  for (u32 v = v0; v < v1; ++v)
+    n[(u64)v * K + k] = poisson_draw(C[v] / K, beta, tables, k, v, 0, kTagInitN, k0, k1);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Alias rows.  One wavefront per row of length m:
+//   A. (optional) x_k /= (f32)divisor_k; canonical row sum T; sigma = alpha * T.
+//   B. q_k = floor(x_k / T * m * 2^32) (exact double ops), slot capacity Q = sum_k q_k and item
+//      weight a_k = m q_k, so that total deficit equals total surplus exactly.  a < Q is light
+//      (deficit Q - a), otherwise heavy (surplus a - Q).  Exclusive deficit prefixes D over the
+//      lights and inclusive surplus prefixes E over the heavies, in index order, are exact u64
+//      sums; lights and heavies are compacted to LDS in index order.
+//   C. light i: alias = first heavy with E > D_i, prob = a_i / Q.
+//      heavy j: its turn ends at D_end = D of the first light with D >= E_j, or the total deficit
+//      when there is none; if D_end > E_j it keeps prob = (Q + E_j - D_end) / Q and its
+//      alias is the next heavy, otherwise (or with no next heavy) prob = 1, alias = j.
+// This is Vose's pairing expressed through prefix sums (the sweep of Huebschle-Schneider &
+// Sanders, "Parallel Weighted Random Sampling", 2019), so every slot is computed independently.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ u64 readlane_u64(u64 v, u32 l) {
+  u32 lo = readlane_u32((u32)v, l);
+  u32 hi = readlane_u32((u32)(v >> 32), l);
+  return ((u64)hi << 32) | lo;
+}
+
+__global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const unsigned long long* __restrict__ divisor,
+                             f32* __restrict__ sigma, f32 alpha, AliasSlot* __restrict__ out, u32 rows_per_block_waves) {
+  extern __shared__ unsigned char alias_smem[];
+  const u32 lane = lane_id();
+  const u32 wave = threadIdx.x >> 6;
+  const u32 row = blockIdx.x * rows_per_block_waves + wave;
+  if (row >= n_rows) return;
+  const u32 m_even = (m + 1) & ~1u;
+  u32* hidx = reinterpret_cast<u32*>(alias_smem) + (u64)wave * m_even * 5;
+  u64* hE = reinterpret_cast<u64*>(hidx + m_even);
+  u64* lD = hE + m_even;
+
+  f32* x = rows + (u64)row * m;
+  AliasSlot* o = out + (u64)row * m;
+
+  f32 acc = 0.0f;
+  for (u32 c = 0; c < m; c += 64) {
+    u32 k = c + lane;
+    if (k < m) {
+      f32 p = x[k];
+      if (divisor) {
+        f32 d = (f32)(f64)divisor[k];
+        p = d > 0.0f ? p / d : 0.0f;
+        x[k] = p;
+      }
+      acc = acc + p;
+    }
+  }
+  f32 T = readlane_f32(wave_scan_f32(acc), 63);
+  if (sigma && lane == 0) sigma[row] = alpha * T;
+  if (!(T > 0.0f)) {
+    for (u32 k = lane; k < m; k += 64) {
+      AliasSlot s;
+      s.prob = 1.0f;
+      s.alias = k;
+      o[k] = s;
+    }
+    return;
+  }
+
+  // B1: fixed-point weights and their exact total Q (the capacity of one slot)
+  u64 qsum = 0;
+  for (u32 c = 0; c < m; c += 64) {
+    u32 k = c + lane;
+    qsum += k < m ? alias_fixed_weight(x[k], T, m) : 0;
+  }
+  const u64 Q = readlane_u64(wave_scan_u64(qsum), 63);
+  const u64 lt = lanemask_lt();
+
+  // B2: item weight a = m q; lights (a < Q) and heavies, exact prefix sums, compaction
+  u32 nh = 0, nl = 0;
+  u64 sE = 0, sD = 0;
+  for (u32 c = 0; c < m; c += 64) {
+    u32 k = c + lane;
+    bool valid = k < m;
+    u64 a = valid ? alias_fixed_weight(x[k], T, m) * m : 0;
+    bool heavy = valid && a >= Q;
+    bool light = valid && a < Q;
+    u64 hb = __ballot(heavy);
+    u64 lb = __ballot(light);
+    u64 sur = heavy ? a - Q : 0;
+    u64 def = light ? Q - a : 0;
+    u64 incS = wave_scan_u64(sur);
+    u64 incD = wave_scan_u64(def);
+    if (heavy) {
+      u32 pos = nh + (u32)__popcll(hb & lt);
+      hidx[pos] = k;
+      hE[pos] = sE + incS;
+    }
+    if (light) {
+      u32 pos = nl + (u32)__popcll(lb & lt);
+      lD[pos] = sD + incD - def;
+    }
+    sE += readlane_u64(incS, 63);
+    sD += readlane_u64(incD, 63);
+    nh += (u32)__popcll(hb);
+    nl += (u32)__popcll(lb);
+  }
+  wave_sync();
+
+  // C: every slot independently
+  u32 hc = 0, lc = 0;
+  for (u32 c = 0; c < m; c += 64) {
+    u32 k = c + lane;
+    bool valid = k < m;
+    u64 a = valid ? alias_fixed_weight(x[k], T, m) * m : 0;
+    bool heavy = valid && a >= Q;
+    bool light = valid && a < Q;
+    u64 hb = __ballot(heavy);
+    u64 lb = __ballot(light);
+    AliasSlot s;
+    s.prob = 1.0f;
+    s.alias = k;
+    if (light) {
+      u32 pos = lc + (u32)__popcll(lb & lt);
+      u64 D = lD[pos];
+      u32 lo = 0, hi = nh;  // first j with hE[j] > D
+      while (lo < hi) {
+        u32 mid = (lo + hi) >> 1;
+        if (hE[mid] > D) hi = mid;
+        else lo = mid + 1;
+      }
+      if (lo == nh) lo = nh - 1;
+      s.prob = (f32)((f64)a / (f64)Q);
+      s.alias = hidx[lo];
+    }
+    if (heavy) {
+      u32 pos = hc + (u32)__popcll(hb & lt);
+      u64 E = hE[pos];
+      u32 lo = 0, hi = nl;  // first b with lD[b] >= E
+      while (lo < hi) {
+        u32 mid = (lo + hi) >> 1;
+        if (lD[mid] >= E) hi = mid;
+        else lo = mid + 1;
+      }
+      const u64 dend = lo < nl ? lD[lo] : sD;
+      if (dend > E && pos + 1 < nh) {
+        s.prob = (f32)((f64)(Q + E - dend) / (f64)Q);
+        s.alias = hidx[pos + 1];
+      }
+    }
+    if (valid) o[k] = s;
+    hc += (u32)__popcll(hb);
+    lc += (u32)__popcll(lb);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// z-sweep, one wavefront per document, the doc-topic map resident in LDS.
+// Per token i (word w, old topic z), with r = Philox(i, doc, iter, SWEEP):
+//   remove z; if sigma_a[w] == 0 (phi_w. is all zero) keep z; else
+//   sigma_b = canonical scan of count*phi[w][key] over the map's 64-slot lines, carried line by
+//   line; sparse if u(r.x) * (sigma_a + sigma_b) > sigma_a: target t = u(r.w) * sigma_b, first
+//   line whose running total exceeds t, then inside it the first occupied slot whose prefix
+//   exceeds t (the last occupied one if rounding leaves none); otherwise the alias table of w
+//   (slot from r.y, keep test on r.z); add the new topic.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ f32 slot_mass(u32 e, const f32* __restrict__ prow) {
+  return e == kMapEmpty ? 0.0f : (f32)map_count(e) * prow[map_key(e)];
+}
+
+__global__ __launch_bounds__(64) void k_sweep(SweepParams P, u32 cap_max) {
+  extern __shared__ u32 sweep_smem[];
+  const u32 lane = lane_id();
+  const u32 doc = P.doc_order ? P.doc_order[blockIdx.x] : blockIdx.x;
+  if (doc >= P.n_docs) return;
+  u32* T = sweep_smem;
+  const u64 b = P.doc_off[doc];
+  const u32 L = (u32)(P.doc_off[doc + 1] - b);
+  const u32 K = P.K;
+  const u32 cap = map_capacity(L, K);
+  const u32 lines = cap / kMapLine;
+  const u32 gdoc = P.doc_base + doc;
+  (void)cap_max;
+
+  for (u32 s = lane; s < cap; s += 64) T[s] = kMapEmpty;
+  u32 occupied = 0;
+  for (u32 c0 = 0; c0 < L; c0 += 64) {
+    u32 zz = c0 + lane < L ? (u32)P.z[b + c0 + lane] : 0;
+    u32 nt = L - c0 < 64 ? L - c0 : 64;
+    for (u32 t = 0; t < nt; ++t) occupied += map_inc(T, cap, readlane_u32(zz, t));
+  }
+
+  u64 visited = 0;
+  for (u32 c0 = 0; c0 < L; c0 += 64) {
+    const u32 i = c0 + lane;
+    const bool valid = i < L;
+    const u32 ww = valid ? P.w[b + i] : 0;
+    const u32 zz = valid ? (u32)P.z[b + i] : 0;
+    const u32x4 r = philox_tagged(i, gdoc, P.iter, kTagSweep, 0, P.k0, P.k1);
+    u32 znew_lane = zz;
+    const u32 nt = L - c0 < 64 ? L - c0 : 64;
+    for (u32 t = 0; t < nt; ++t) {
+      const u32 wt = readlane_u32(ww, t);
+      const u32 zt = readlane_u32(zz, t);
+      occupied -= map_dec(T, cap, zt);
+      const f32 sa = P.sigma_a[wt];
+      u32 znew = zt;
+      if (sa > 0.0f) {
+        const f32* prow = P.phi + (u64)wt * K;
+        wave_sync();
+        f32 carry = 0.0f;
+        f32 cw0 = 0.0f, cw1 = 0.0f;  // running total after line j, held by lane j (j < 128)
+        for (u32 j = 0; j < lines; ++j) {
+          f32 sc = wave_scan_f32(slot_mass(T[j * 64 + lane], prow));
+          carry = carry + readlane_f32(sc, 63);
+          if (j < 64) {
+            if (lane == j) cw0 = carry;
+          } else if (lane == j - 64) {
+            cw1 = carry;
+          }
+        }
+        visited += occupied;
+        const f32 sb = carry;
+        const f32 u1 = u24(readlane_u32(r.x, t));
+        if (u1 * (sa + sb) > sa) {
+          const f32 tgt = u24(readlane_u32(r.w, t)) * sb;
+          u32 jsel = lines - 1;  // the last line always qualifies (tgt < sigma_b)
+          u64 bw = __ballot(lane < lines && cw0 > tgt);
+          if (bw) {
+            jsel = first_lane(bw);
+          } else {
+            bw = __ballot(lane + 64 < lines && cw1 > tgt);
+            if (bw) jsel = 64 + first_lane(bw);
+          }
+          f32 cprev = 0.0f;
+          if (jsel > 0) cprev = jsel - 1 < 64 ? readlane_f32(cw0, jsel - 1) : readlane_f32(cw1, jsel - 65);
+          const u32 e = T[jsel * 64 + lane];
+          const f32 v = slot_mass(e, prow);
+          const f32 pv = cprev + wave_scan_f32(v);
+          const u64 cand = __ballot(v > 0.0f && pv > tgt);
+          const u64 pos = __ballot(v > 0.0f);
+          if (pos) {  // a line with a running total above tgt holds positive mass
+            const u32 l = cand ? first_lane(cand) : 63u - (u32)__clzll((long long)pos);
+            znew = map_key(readlane_u32(e, l));
+          }
+        } else {
+          znew = draw_alias_slot(P.alias + (u64)wt * K, K, readlane_u32(r.y, t), readlane_u32(r.z, t));
+        }
+      }
+      occupied += map_inc(T, cap, znew);
+      if (lane == t) znew_lane = znew;
+    }
+    if (valid) P.z[b + i] = (u16)znew_lane;
+  }
+  if (lane == 0 && P.visited) atomicAdd(P.visited, (unsigned long long)visited);
+}
+
+// ---------------------------------------------------------------------------------------------
+// n_wk reduction over the word-sorted token permutation.  One wavefront per task (a run of at
+// most kCountTask tokens of one word): LDS histogram over K, then nonzero bins added to n_wk.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_count(const CountTask* __restrict__ tasks, u32 n_tasks, const u32* __restrict__ perm,
+                        const u16* __restrict__ z, u32* __restrict__ n, u32 K, u32 waves_per_block) {
+  extern __shared__ u32 count_smem[];
+  const u32 lane = lane_id();
+  const u32 wave = threadIdx.x >> 6;
+  const u32 task = blockIdx.x * waves_per_block + wave;
+  if (task >= n_tasks) return;
+  u32* H = count_smem + (u64)wave * K;
+  for (u32 k = lane; k < K; k += 64) H[k] = 0;
+  wave_sync();
+  const CountTask tk = tasks[task];
+  for (u32 p = lane; p < tk.len; p += 64) {
+    u32 pos = perm[tk.p0 + p];
+    atomicAdd(&H[z[pos]], 1u);
+  }
+  wave_sync();
+  u32* row = n + (u64)tk.word * K;
+  for (u32 k = lane; k < K; k += 64) {
+    u32 c = H[k];
+    if (c) atomicAdd(&row[k], c);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Stage kernels used by the conformance tests (same device functions as the iteration).
+// ---------------------------------------------------------------------------------------------
+__global__ void k_stage_philox(const u32* ctr, const u32* key, u32* out, u32 n) {
+  u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u32x4 c;
+  c.x = ctr[4 * i];
+  c.y = ctr[4 * i + 1];
+  c.z = ctr[4 * i + 2];
+  c.w = ctr[4 * i + 3];
+  u32x4 r = philox4x32_10(c, key[2 * i], key[2 * i + 1]);
+  out[4 * i] = r.x;
+  out[4 * i + 1] = r.y;
+  out[4 * i + 2] = r.z;
+  out[4 * i + 3] = r.w;
+}
+
+__global__ void k_stage_scan(const f32* in, f32* out, u32 n_groups) {
+  u32 g = blockIdx.x;
+  if (g >= n_groups) return;
+  out[g * 64 + lane_id()] = wave_scan_f32(in[g * 64 + lane_id()]);
+}
+
+// ops[i] = key | (1u << 31) for a decrement; the final table is written to out.
+__global__ void k_stage_map(const u32* ops, u32 n_ops, u32 cap, u32* out) {
+  extern __shared__ u32 map_smem[];
+  const u32 lane = lane_id();
+  for (u32 s = lane; s < cap; s += 64) map_smem[s] = kMapEmpty;
+  for (u32 i = 0; i < n_ops; ++i) {
+    u32 op = ops[i];
+    if (op >> 31) map_dec(map_smem, cap, op & 0x7FFFFFFFu);
+    else map_inc(map_smem, cap, op);
+  }
+  wave_sync();
+  for (u32 s = lane; s < cap; s += 64) out[s] = map_smem[s];
+}
+
+__global__ void k_stage_poisson(const u32* nval, u32 count, f32 beta, const AliasSlot* tables, u32 c1, u32 c2,
+                                u32 tag, u32 k0, u32 k1, u32* out) {
+  u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  out[i] = poisson_draw(nval[i], beta, tables, i, c1, c2, tag, k0, k1);
+}
+
+__global__ void k_stage_alias_draw(const AliasSlot* row, u32 m, const u32* r_slot, const u32* r_keep, u32 count,
+                                   u32* out) {
+  u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  out[i] = draw_alias_slot(row, m, r_slot[i], r_keep[i]);
+}
+
+}  // namespace lda

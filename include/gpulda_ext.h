@@ -1,0 +1,103 @@
+/*
+ * gpulda_ext.h -- extension entry points of libGPUPolyaUrnLDA.so.
+ *
+ * The reference's Rust host never calls these; they exist for callers that need more than the
+ * five-call ABI of gpulda_abi.h: a seed, state read-back, a device-resident corpus (no per-call
+ * PCIe traffic), stage-level entry points for conformance tests, and the fold-in server.
+ * All return 0 on success and a nonzero code on failure; gpulda_last_error() describes the last
+ * failure.  None of them terminates the process.
+ *
+ * Semantics shared with the ABI path (each fixes a reference defect; reference paths are
+ * relative to the reference repository root):
+ *   - Randomness is a counter-based Philox4x32-10 keyed by the seed (default 0) with counters
+ *     (token-in-document, global document id, iteration, purpose).  The reference reuses one
+ *     cuRAND state whose per-buffer copy is never advanced (src/cu/train.cu:168) and ignores
+ *     --seed (src/args.rs:36-37).
+ *   - Global document ids count documents in the order sample_z_async sees them within an
+ *     iteration (train.rs walks the corpus in file order); iteration t is the t-th sample_phi.
+ *   - The sparse term uses phi[w][k] (the reference reads phi[k] only, src/cu/topics.cuh:174).
+ *   - Alias tables store the probability of keeping a slot (scaled convention; the reference
+ *     mixes conventions, src/cu/spalias.cu:135 vs src/cu/topics.cuh:69-76).
+ *   - All reductions are exact (integer) or use one fixed floating-point order, so results are
+ *     bit-identical run to run and independent of launch shape.
+ *   - Poisson draws are integers for every lambda (the reference's n >= 100 branch returns a
+ *     non-integer Gaussian, src/cu/polyaurn.cu:30-35).
+ *   - A token whose word has an all-zero phi row keeps its topic (the reference would read 0/0).
+ *
+ * Limits: K <= 4096 (LDS-resident doc-topic maps and alias split), documents <= 65535 tokens.
+ *
+ * Layouts: phi, n_wk and alias tables are word-major, element [v * K + k].
+ */
+#ifndef GPULDA_EXT_H
+#define GPULDA_EXT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- configuration and diagnostics ------------------------------------------------------- */
+int gpulda_set_seed(uint64_t seed);           /* before initialize(); default 0 */
+uint64_t gpulda_get_seed(void);
+const char* gpulda_last_error(void);
+/* Device name, compute units, HBM bytes, max LDS bytes per workgroup of the current device. */
+int gpulda_device_info(char* name, int name_len, int* cu_count, uint64_t* hbm_bytes, int* lds_per_block);
+/* 1 = cache device copies of buffer slices keyed by their host pointers (default), 0 = re-upload
+ * w and d on every sample_z_async (for callers that refill one host buffer). */
+int gpulda_set_slice_cache(int enabled);
+
+/* ---- model state (valid between initialize and cleanup) ---------------------------------- */
+uint32_t gpulda_iteration(void);                         /* number of sample_phi calls so far */
+int gpulda_copy_phi(float* out);                         /* V*K */
+int gpulda_copy_alias(float* prob, uint32_t* alias);     /* V*K each */
+int gpulda_copy_sigma_a(float* out);                     /* V */
+int gpulda_copy_counts(uint32_t* out);                   /* n_wk, V*K */
+int gpulda_set_counts(const uint32_t* n);                /* overwrite n_wk, V*K */
+int gpulda_copy_topic_sums(uint64_t* out);               /* sum_v phi~_vk of the last Phi step, K */
+int gpulda_copy_poisson_tables(float* prob, uint32_t* alias);  /* 100 x 200 each */
+/* Doc-topic map entries read by the sparse sums since the last reset (sum over tokens). */
+int gpulda_sweep_stats(uint64_t* visited);
+int gpulda_reset_stats(void);
+
+/* ---- device-resident corpus ------------------------------------------------------------- */
+/* Uploads a whole corpus (word ids w, document lengths d_len) once; z == NULL draws the
+ * initial topics on the device (Philox, uniform over K), otherwise z is uploaded. */
+int gpulda_corpus_load(const uint32_t* w, const uint32_t* d_len, uint32_t n_docs, const uint32_t* z);
+int gpulda_corpus_counts_from_z(void);                   /* n_wk := counts of the current z */
+int gpulda_corpus_copy_z(uint32_t* z);
+/* n_iters full iterations (Phi step, alias tables, z-sweep, n_wk reduction) on the device.
+ * stage_ms (optional, 4 floats): mean device ms per iteration of
+ * [Poisson draw, alias rows, z-sweep, n_wk reduction], timed with hipEvents on the library
+ * stream. */
+int gpulda_corpus_iterate(uint32_t n_iters, float* stage_ms);
+/* Collapsed joint log-likelihood log p(w, z | alpha, beta) of the current z and n_wk. */
+int gpulda_corpus_loglik(double* out);
+int gpulda_corpus_release(void);
+
+/* ---- stage entry points (host arrays in and out; the library stages device copies) ------- */
+int gpulda_stage_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n);
+int gpulda_stage_scan64(const float* in, float* out, uint32_t n_groups);
+/* ops[i] = topic (increment) or topic | 1u<<31 (decrement); returns the final table. */
+int gpulda_stage_map(const uint32_t* ops, uint32_t n_ops, uint32_t cap, uint32_t* table_out);
+/* Row normalisation (by divisor[k] when divisor != NULL), row sums and alias tables. */
+int gpulda_stage_alias(const float* rows, uint32_t n_rows, uint32_t m, const uint64_t* divisor, float alpha,
+                       float* rows_out, float* sums_out, float* prob_out, uint32_t* alias_out);
+int gpulda_stage_alias_draw(const float* prob, const uint32_t* alias, uint32_t m, const uint32_t* r_slot,
+                            const uint32_t* r_keep, uint32_t count, uint32_t* out);
+/* Poisson draws with counters (i, c1, c2, tag) for i = 0..count-1, tables built for beta. */
+int gpulda_stage_poisson(const uint32_t* n, uint32_t count, float beta, uint32_t c1, uint32_t c2, uint32_t tag,
+                         uint64_t seed, uint32_t* out);
+int gpulda_stage_phi_draw(const uint32_t* n, uint32_t V, uint32_t K, float beta, uint32_t iter, uint64_t seed,
+                          float* phi_tilde, uint64_t* topic_sum);
+int gpulda_stage_sweep(uint32_t* z, const uint32_t* w, const uint64_t* doc_off, uint32_t n_docs, uint32_t doc_base,
+                       uint32_t iter, uint64_t seed, uint32_t K, uint32_t V, const float* phi, const float* sigma_a,
+                       const float* prob, const uint32_t* alias, uint64_t* visited);
+int gpulda_stage_count(const uint32_t* w, const uint32_t* z, uint64_t n_tokens, uint32_t V, uint32_t K,
+                       uint32_t* n_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPULDA_EXT_H */

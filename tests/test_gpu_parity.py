@@ -1,0 +1,252 @@
+"""GPU parity: every stage of the HIP library against the CPU oracle, bit for bit, plus the
+five-call ABI end to end (tiny corpus, 100 iterations, reference train.rs buffer loop).
+
+Integer results (z, n_wk, topic sums, alias indices, Poisson draws, map layouts) must be equal;
+floating-point results (phi, sigma_a, alias probabilities, scans) must be bit-identical because
+both sides evaluate the same operations in the same order.  The log-likelihood is computed in
+double with different lgamma implementations: tolerance 1e-9 relative.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def S():
+    from gpupolyaurnlda_amd import stages
+    return stages
+
+
+def say(msg):
+    print(f"[gpu-parity] {msg}", flush=True)
+
+
+def test_device_is_gfx950():
+    from gpupolyaurnlda_amd._lib import lib
+    name = C.create_string_buffer(256)
+    cu = C.c_int(0)
+    hbm = C.c_uint64(0)
+    lds = C.c_int(0)
+    assert lib().gpulda_device_info(name, 256, C.byref(cu), C.byref(hbm), C.byref(lds)) == 0
+    say(f"device {name.value.decode()} CUs={cu.value} HBM={hbm.value / 2**30:.0f} GiB LDS/block={lds.value}")
+    assert b"gfx950" in name.value
+
+
+def test_philox(S, orc):
+    vec = json.load(open(os.path.join(HERE, "golden", "philox_rocrand.json")))
+    ctr = np.array([v[0] for v in vec], dtype=np.uint32)
+    key = np.array([v[1] for v in vec], dtype=np.uint32)
+    exp = np.array([v[2] for v in vec], dtype=np.uint32)
+    assert np.array_equal(S.philox(ctr, key), exp)
+    rng = np.random.default_rng(0)
+    ctr = rng.integers(0, 2**32, size=(5000, 4), dtype=np.uint64).astype(np.uint32)
+    key = rng.integers(0, 2**32, size=(5000, 2), dtype=np.uint64).astype(np.uint32)
+    assert np.array_equal(S.philox(ctr, key), orc.philox(ctr, key))
+
+
+def test_scan64(S, orc):
+    rng = np.random.default_rng(1)
+    x = (rng.random((300, 64)) * rng.choice([0, 1, 1e-3, 1e3], size=(300, 64))).astype(np.float32)
+    assert np.array_equal(S.scan64(x).view(np.uint32), orc.scan64(x).view(np.uint32))
+
+
+@pytest.mark.parametrize("K,L", [(20, 50), (1024, 600), (300, 64), (4096, 5000), (5, 3)])
+def test_map_layout(S, orc, K, L):
+    rng = np.random.default_rng(K * 7 + L)
+    cap = orc.map_capacity(L, K)
+    counts = {}
+    ops = []
+    total = 0
+    for _ in range(4 * L):
+        if total > 0 and (total >= L or rng.random() < 0.45):
+            keys = [k for k, c in counts.items() if c > 0]
+            k = int(rng.choice(keys))
+            ops.append(k | (1 << 31))
+            counts[k] -= 1
+            total -= 1
+        else:
+            k = int(rng.integers(0, K))
+            ops.append(k)
+            counts[k] = counts.get(k, 0) + 1
+            total += 1
+    assert np.array_equal(S.map_apply(ops, cap), orc.map_apply(ops, cap))
+
+
+@pytest.mark.parametrize("m", [1, 5, 20, 64, 200, 1024, 3000])
+def test_alias_rows(S, orc, m):
+    rng = np.random.default_rng(m)
+    rows = np.stack([rng.random(m), rng.random(m) * (rng.random(m) < 0.1), np.ones(m), np.zeros(m),
+                     np.exp(rng.normal(0, 6, m))]).astype(np.float32)
+    ro, sums, prob, al = S.alias(rows, alpha=0.25)
+    for r in range(len(rows)):
+        t = orc.sum64(rows[r])
+        assert sums[r].view(np.uint32) == np.float32(0.25 * t).view(np.uint32) or sums[r] == np.float32(0.25) * t
+        p2, a2 = orc.alias_build(rows[r], t)
+        assert np.array_equal(prob[r].view(np.uint32), p2.view(np.uint32)), r
+        assert np.array_equal(al[r], a2), r
+
+
+def test_phi_step_bit_exact(S, orc):
+    rng = np.random.default_rng(2)
+    V, K = 300, 64
+    n = rng.integers(0, 30, size=(V, K)).astype(np.uint32)
+    n[rng.random((V, K)) < 0.7] = 0
+    n[5, 3] = 100
+    n[7, 9] = 5000
+    n[11, 0] = 2_000_000
+    pt, ts = S.phi_draw(n, 0.01, 4, seed=9)
+    pt2, ts2 = orc.phi_draw(n, 0.01, 4, 9)
+    assert np.array_equal(pt, pt2) and np.array_equal(ts, ts2)
+    rows_out, sums, prob, al = S.alias(pt, divisor=ts, alpha=0.1)
+    phi, sig, p2, a2 = orc.phi_finish(pt2, ts2, 0.1)
+    assert np.array_equal(rows_out.view(np.uint32), phi.view(np.uint32))
+    assert np.array_equal(sums.view(np.uint32), sig.view(np.uint32))
+    assert np.array_equal(prob.view(np.uint32), p2.view(np.uint32)) and np.array_equal(al, a2)
+
+
+@pytest.mark.parametrize("n", [0, 1, 10, 99, 100, 1000, 100000])
+def test_poisson_bit_exact(S, orc, n):
+    vals = np.full(4000, n, np.uint32)
+    g = S.poisson(vals, 0.01, c1=3, c2=2, tag=3, seed=77)
+    c = orc.poisson(vals, 0.01, c1=3, c2=2, tag=3, seed=77)
+    assert np.array_equal(g, c)
+
+
+def _tiny_phi(orc, V, K, seed=5, alpha=0.3):
+    rng = np.random.default_rng(seed)
+    pt = rng.integers(0, 5, size=(V, K)).astype(np.float32) * (rng.random((V, K)) < 0.6)
+    pt[:, 0] += 1
+    ts = pt.sum(0).astype(np.uint64)
+    return orc.phi_finish(pt, ts, alpha)
+
+
+@pytest.mark.parametrize("K,V,D,meanL", [(5, 3, 200, 6), (20, 1000, 500, 50), (256, 2000, 300, 300),
+                                         (1024, 3000, 40, 1500)])
+def test_sweep_bit_exact(S, orc, K, V, D, meanL):
+    rng = np.random.default_rng(K + D)
+    phi, sig, prob, al = _tiny_phi(orc, V, K, seed=K)
+    L = np.maximum(1, rng.poisson(meanL, D)).astype(np.uint32)
+    L[0] = 1
+    off = np.concatenate([[0], np.cumsum(L)]).astype(np.uint64)
+    N = int(off[-1])
+    w = rng.integers(0, V, N).astype(np.uint32)
+    z = rng.integers(0, K, N).astype(np.uint32)
+    say(f"sweep K={K} V={V} D={D} N={N}")
+    zg, vg = S.sweep(z, w, off, 17, 3, 5, K, phi, sig, prob, al)
+    zc, vc = orc.sweep(z, w, off, 17, 3, 5, K, phi, sig, prob, al, threads=8)
+    assert vg == vc
+    assert np.array_equal(zg, zc)
+    assert not np.array_equal(zg, z)
+
+
+def test_sweep_exact_conditional_gpu(S, orc):
+    """Two-token documents: the joint law of (z0', z1') after one sweep, enumerated exactly."""
+    from scipy import stats
+    phi, sig, prob, al = _tiny_phi(orc, 3, 4, seed=5)
+    K = 4
+    w = np.array([1, 2], np.uint32)
+    z0 = np.array([2, 3], np.uint32)
+    n_docs = 100000
+    off = np.arange(0, 2 * n_docs + 1, 2, dtype=np.uint64)
+    zg, _ = S.sweep(np.tile(z0, n_docs), np.tile(w, n_docs), off, 0, 1, 123, K, phi, sig, prob, al)
+    pairs = zg.reshape(-1, 2)
+    obs = np.bincount(pairs[:, 0] * K + pairs[:, 1], minlength=K * K).astype(float)
+    ph = phi.astype(np.float64)
+    alpha = 0.3
+    ex = np.zeros((K, K))
+    p0 = ph[w[0]] * (alpha + (np.arange(K) == z0[1]))
+    p0 /= p0.sum()
+    for a in range(K):
+        p1 = ph[w[1]] * (alpha + (np.arange(K) == a))
+        ex[a] = p0[a] * p1 / p1.sum()
+    ex = ex.reshape(-1) * n_docs
+    keep = ex > 0
+    assert obs[~keep].sum() == 0
+    assert stats.chisquare(obs[keep], ex[keep]).pvalue > 1e-4
+
+
+def test_count(S, orc):
+    rng = np.random.default_rng(8)
+    V, K, N = 2000, 300, 300000
+    w = (rng.zipf(1.3, N) % V).astype(np.uint32)
+    z = rng.integers(0, K, N).astype(np.uint32)
+    assert np.array_equal(S.count(w, z, V, K), orc.count(w, z, V, K))
+
+
+def _tiny_corpus():
+    from gpupolyaurnlda_amd import corpus
+    return corpus.generate(corpus.TINY)
+
+
+def test_abi_tiny_100_iterations(orc):
+    """train.rs buffer loop through the five ABI calls vs the oracle Model (counts from C)."""
+    from gpupolyaurnlda_amd.corpus import TINY
+    from gpupolyaurnlda_amd.train import Sampler, train
+    c = _tiny_corpus()
+    w, d, z0 = c["w"], c["d_len"], c["z"]
+    k_d = np.array([len(np.unique(z0[c["doc_off"][i]:c["doc_off"][i + 1]])) for i in range(len(d))], np.uint32)
+    buffer_size, max_D = 4096, 64  # several buffers per iteration, as train.rs streams the corpus
+    model = orc.Model(w, c["doc_off"], z0, TINY.V, TINY.K, TINY.alpha, TINY.beta, seed=0, counts="C", threads=8)
+    z = z0.copy()
+    checkpoints = {1, 10, 100}
+    with Sampler(TINY.alpha, TINY.beta, TINY.K, c["C"], buffer_size, max_D, seed=0) as smp:
+        def on_it(it):
+            # the oracle replays the same iteration: phi step, sweep of every buffer, counts
+            model.phi_step()
+            doc = 0
+            tok = 0
+            from gpupolyaurnlda_amd.train import fill_buffer
+            while True:
+                nd, nt = fill_buffer(d, doc, buffer_size, max_D)
+                if nd == 0:
+                    break
+                model.z_step(doc_base=doc, lo=doc, hi=doc + nd)
+                doc += nd
+                tok += nt
+            model.n = orc.count(model.w, model.z, TINY.V, TINY.K)
+            if it + 1 in checkpoints:
+                say(f"ABI iteration {it + 1}")
+                assert np.array_equal(z, model.z), f"z differs after iteration {it + 1}"
+                assert np.array_equal(smp.counts(), model.n), f"n_wk differs after iteration {it + 1}"
+                assert np.array_equal(smp.phi().view(np.uint32), model.phi.view(np.uint32))
+                p, a = smp.alias()
+                assert np.array_equal(p.view(np.uint32), model.prob.view(np.uint32)) and np.array_equal(a, model.al)
+        train(smp, z, w, d, k_d, n_mc=100, buffer_size=buffer_size, max_D=max_D, on_iteration=on_it)
+        assert smp.iteration == 100
+
+
+def test_resident_matches_oracle_and_loglik(orc):
+    from gpupolyaurnlda_amd.corpus import TINY
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+    c = _tiny_corpus()
+    model = orc.Model(c["w"], c["doc_off"], c["z"], TINY.V, TINY.K, TINY.alpha, TINY.beta, seed=3, counts="z",
+                      threads=8)
+    with ResidentCorpus(c["w"], c["d_len"], TINY.K, TINY.V, TINY.alpha, TINY.beta, seed=3, z=c["z"]) as rc:
+        assert np.array_equal(rc.counts(), model.n)
+        for it in range(1, 21):
+            rc.iterate_untimed(1)
+            model.iterate()
+            if it in (1, 5, 20):
+                assert np.array_equal(rc.z(), model.z), it
+                assert np.array_equal(rc.counts(), model.n), it
+                ll_g = rc.loglik()
+                ll_c = orc.loglik(model.n, model.z, c["doc_off"], float(np.float32(TINY.alpha)),
+                                  float(np.float32(TINY.beta)))
+                say(f"resident it={it} LL gpu={ll_g:.6f} cpu={ll_c:.6f}")
+                assert abs(ll_g - ll_c) <= 1e-9 * abs(ll_c)
+        assert rc.visited() == model.visited
+
+
+def test_resident_device_init_equals_generator_init():
+    from gpupolyaurnlda_amd.corpus import TINY
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+    c = _tiny_corpus()
+    with ResidentCorpus(c["w"], c["d_len"], TINY.K, TINY.V, TINY.alpha, TINY.beta, seed=0, z=None) as rc:
+        assert np.array_equal(rc.z(), c["z"])
