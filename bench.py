@@ -1,0 +1,170 @@
+"""Throughput of the Polya-urn LDA iteration on one MI355X.
+
+Workload: the large synthetic corpus (V=100,000 Zipf(1.07), K=1,024, D=1,000,000 documents with
+lognormal(median 200, sigma 1.0) lengths capped at 4,096, alpha=50/K, beta=0.01, seed 2),
+generated in memory and made resident in HBM before timing.  One step is one full Gibbs iteration:
+Poisson Polya-urn draw of phi, per-word alias tables, the z-sweep of every token and the
+word-sorted n_wk reduction.
+
+Prints one JSON line.  `value` is tokens sampled per second over the K timed iterations (the
+corpus and model already in HBM).  `roofline` describes the z-sweep kernel (the dominant kernel):
+algorithmic bytes per launch = N * (18 + 4 * S_bar) with S_bar the mean number of doc-topic map
+entries read per token, over its mean device time measured with HIP events on the library's
+stream.  `cpu_baseline` is the CPU oracle (OpenMP) on a bounded sample of the same iteration, run
+from the same model state; the sampled documents are also checked bit for bit against the GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweep_large_traffic.json")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="large", choices=["tiny", "medium", "large", "batched"])
+    ap.add_argument("--docs", type=int, default=0, help="override D (smaller runs for profiling)")
+    ap.add_argument("--cpu-docs", type=int, default=20000, help="documents in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
+                                   "(documents shard across GPUs with one n_wk all-reduce per iteration; "
+                                   "see the sharding design notes)"}))
+        sys.exit(2)
+
+    from gpupolyaurnlda_amd import corpus
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+
+    spec = corpus.SPECS[args.workload]
+    if args.docs:
+        spec = spec.scaled(args.docs)
+    t0 = time.time()
+    c = corpus.generate(spec)
+    N = len(c["w"])
+    log(f"generated {spec.name}: N={N} tokens in {time.time() - t0:.1f}s")
+
+    t0 = time.time()
+    rc = ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=spec.seed, z=c["z"])
+    log(f"resident in HBM in {time.time() - t0:.1f}s")
+
+    if args.warmup:
+        rc.iterate(args.warmup)
+        log(f"warmup {args.warmup} done")
+    n_before = rc.counts() if not args.no_cpu else None
+    z_before = rc.z() if not args.no_cpu else None
+    rc.reset_stats()
+    it0 = rc.iteration
+
+    t0 = time.perf_counter()
+    stage_ms = rc.iterate(args.steps)  # synchronises before returning
+    elapsed = time.perf_counter() - t0
+    visited = rc.visited()
+    log(f"timed {args.steps} iterations in {elapsed:.3f}s; stage ms {stage_ms.tolist()}")
+
+    tokens_per_s = N * args.steps / elapsed
+    s_bar = visited / (N * args.steps)
+    sweep_ms = float(stage_ms[2])
+    alg_bytes = N * (18.0 + 4.0 * s_bar)
+    achieved = alg_bytes / (sweep_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        try:
+            tr = json.load(open(TRAFFIC_FILE))
+            if tr.get("workload") == spec.name:
+                traffic = tr.get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_sweep", "kernel_ms": round(sweep_ms, 3), "alg_bytes_per_launch": int(alg_bytes),
+                "s_bar": round(s_bar, 3)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(rc, spec, c, n_before, z_before, it0, args.steps, args.cpu_docs)
+
+    out = {
+        "metric": "tokens sampled/sec at K=1024 (% HBM roofline); fold-in docs/s + p99 latency",
+        "value": round(tokens_per_s, 1),
+        "unit": "tokens/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded LDA generator, resident in HBM)",
+        "config": {"workload": spec.name, "V": spec.V, "K": spec.K, "D": spec.D, "N": N,
+                   "alpha": spec.alpha, "beta": spec.beta, "seed": spec.seed,
+                   "stage_ms": {"phi_draw": round(float(stage_ms[0]), 3), "alias": round(float(stage_ms[1]), 3),
+                                "sweep": round(float(stage_ms[2]), 3), "count": round(float(stage_ms[3]), 3)},
+                   "parallelism": "1 GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    rc.close()
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(rc, spec, c, n_before, z_before, it0, steps, cpu_docs):
+    """Times the oracle on the iteration after the timed ones: the full phi step (all V x K
+    cells) plus the z-sweep of the first `cpu_docs` documents, scaled to the whole corpus.  The
+    GPU runs the same iteration; phi and the sampled documents' z are compared bit for bit."""
+    from oracle import oracle as O
+
+    N = len(c["w"])
+    n_now = rc.counts()
+    z_now = rc.z()
+    threads = int(O.lib().orc_threads())
+    rc.iterate_untimed(1)  # GPU: iteration it0 + steps + 1
+    z_gpu = rc.z()
+    phi_gpu = rc.phi()
+    t0 = time.perf_counter()
+    tables = O.poisson_tables(spec.beta)[2]
+    pt, ts = O.phi_draw(n_now, spec.beta, it0 + steps + 1, spec.seed, tables)
+    phi, sig, prob, al = O.phi_finish(pt, ts, spec.alpha)
+    t_phi = time.perf_counter() - t0
+    D_s = min(cpu_docs, spec.D)
+    off = c["doc_off"]
+    n_s = int(off[D_s])
+    t0 = time.perf_counter()
+    z_s, _ = O.sweep(z_now[:n_s], c["w"][:n_s], off[:D_s + 1], 0, it0 + steps + 1, spec.seed, spec.K, phi, sig,
+                     prob, al, threads=threads)
+    t_sw = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.count(c["w"][:n_s], z_s, spec.V, spec.K)
+    t_cnt = time.perf_counter() - t0
+    t_iter = t_phi + (t_sw + t_cnt) * (N / n_s)
+    same_phi = bool(np.array_equal(phi.view(np.uint32), phi_gpu.view(np.uint32)))
+    same_z = bool(np.array_equal(z_s, z_gpu[:n_s]))
+    log(f"cpu: phi {t_phi:.2f}s, sweep {t_sw:.2f}s + count {t_cnt:.2f}s for {n_s} tokens; "
+        f"bit-exact phi={same_phi} z={same_z}")
+    del n_before, z_before
+    return {"value": round(N / t_iter, 1), "unit": "tokens/s", "cores": threads, "kind": "port",
+            "sample": f"phi step over all V*K cells + z-sweep and count of the first {D_s} documents "
+                      f"({n_s} tokens), scaled to N={N}; phi and sampled z bit-identical to the GPU: "
+                      f"{same_phi and same_z}",
+            "bit_exact_sample": same_phi and same_z}
+
+
+if __name__ == "__main__":
+    main()

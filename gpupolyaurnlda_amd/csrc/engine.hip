@@ -130,16 +130,6 @@ int launch_phi_draw(const u32* n, f32* phi, unsigned long long* tsum, u32 V, u32
   return 0;
 }
 
-int launch_sweep(const SweepParams& P, u32 max_len, hipStream_t st) {
-  if (P.n_docs == 0) return 0;
-  u32 cap = map_capacity(max_len, P.K);
-  u64 lds = (u64)cap * sizeof(u32);
-  if (lds > (u64)max_lds_per_block()) return fail("doc-topic map of %u slots exceeds LDS", cap);
-  k_sweep<<<P.n_docs, 64, lds, st>>>(P, cap);
-  HCK(hipGetLastError());
-  return 0;
-}
-
 int launch_count(const CountTask* tasks, u32 n_tasks, const u32* perm, const u16* z, u32* n, u32 K, hipStream_t st) {
   if (n_tasks == 0) return 0;
   const u32 wpb = 4;
@@ -190,7 +180,37 @@ struct Segment {
   u64* doc_off = nullptr;
   u32* perm = nullptr;
   CountTask* tasks = nullptr;
+  u32* doc_order = nullptr;  // documents grouped by sweep bucket (longest first inside a bucket)
+  struct Bucket {
+    int lmax;      // register-resident lines of the kernel (0 = generic)
+    u32 first;     // offset into doc_order
+    u32 count;
+    u32 max_cap;   // largest map of the bucket (LDS slots)
+  };
+  std::vector<Bucket> buckets;
 };
+
+// z-sweep of a segment: one launch per length bucket, the longest maps first.
+int launch_sweep(SweepParams P, const Segment* sg, hipStream_t st) {
+  const u64 lim = (u64)max_lds_per_block();
+  for (const Segment::Bucket& bk : sg->buckets) {
+    if (bk.count == 0) continue;
+    u64 lds = (u64)bk.max_cap * sizeof(u32);
+    if (lds > lim) return fail("doc-topic map of %u slots exceeds LDS", bk.max_cap);
+    P.doc_order = sg->doc_order + bk.first;
+    switch (bk.lmax) {
+      case 1: k_sweep_lines<1><<<bk.count, 64, lds, st>>>(P); break;
+      case 2: k_sweep_lines<2><<<bk.count, 64, lds, st>>>(P); break;
+      case 4: k_sweep_lines<4><<<bk.count, 64, lds, st>>>(P); break;
+      case 8: k_sweep_lines<8><<<bk.count, 64, lds, st>>>(P); break;
+      case 16: k_sweep_lines<16><<<bk.count, 64, lds, st>>>(P); break;
+      case 32: k_sweep_lines<32><<<bk.count, 64, lds, st>>>(P); break;
+      default: k_sweep_lines<0><<<bk.count, 64, lds, st>>>(P); break;
+    }
+    HCK(hipGetLastError());
+  }
+  return 0;
+}
 
 void free_segment(Segment* s) {
   if (!s) return;
@@ -199,6 +219,7 @@ void free_segment(Segment* s) {
   dfree(s->doc_off);
   dfree(s->perm);
   dfree(s->tasks);
+  dfree(s->doc_order);
   delete s;
 }
 
@@ -256,10 +277,32 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
     std::vector<u64> cur(start.begin(), start.end() - 1);
     for (u64 i = 0; i < n_tokens; ++i) perm[cur[w[i]]++] = (u32)i;
   }
+  // sweep buckets: by register-resident line count, largest first; longest documents first
+  std::vector<u32> order(n_docs);
+  std::vector<int> lmax_of(n_docs);
+  for (u32 i = 0; i < n_docs; ++i) {
+    order[i] = i;
+    lmax_of[i] = sweep_bucket_lmax(map_capacity(d[i], K) / kMapLine);
+  }
+  auto rank = [](int lm) { return lm == 0 ? 1000 : lm; };
+  std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) {
+    if (lmax_of[a] != lmax_of[b]) return rank(lmax_of[a]) > rank(lmax_of[b]);
+    return d[a] > d[b];
+  });
+  s->buckets.clear();
+  for (u32 i = 0; i < n_docs; ++i) {
+    u32 doc = order[i];
+    u32 cap = map_capacity(d[doc], K);
+    if (s->buckets.empty() || s->buckets.back().lmax != lmax_of[doc]) s->buckets.push_back({lmax_of[doc], i, 0, 0});
+    s->buckets.back().count++;
+    s->buckets.back().max_cap = std::max(s->buckets.back().max_cap, cap);
+  }
   s->n_docs = n_docs;
   s->n_tokens = n_tokens;
   s->max_len = max_len;
   s->n_tasks = (u32)tasks.size();
+  CK(dalloc(&s->doc_order, n_docs));
+  if (n_docs) HCK(hipMemcpy(s->doc_order, order.data(), (u64)n_docs * sizeof(u32), hipMemcpyHostToDevice));
   CK(dalloc(&s->w, n_tokens));
   CK(dalloc(&s->z, n_tokens));
   CK(dalloc(&s->doc_off, (u64)n_docs + 1));
@@ -344,7 +387,7 @@ SweepParams sweep_params(Segment* sg, u32 doc_base) {
 }
 
 int sweep_and_count(Segment* sg, u32 doc_base, hipStream_t st) {
-  CK(launch_sweep(sweep_params(sg, doc_base), sg->max_len, st));
+  CK(launch_sweep(sweep_params(sg, doc_base), sg, st));
   CK(launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, S.n, S.K, st));
   return 0;
 }
@@ -779,7 +822,7 @@ extern "C" int gpulda_corpus_iterate(uint32_t n_iters, float* stage_ms) {
     CK(launch_alias_rows(S.phi, S.V, S.K, S.topic_sum, S.sigma_a, S.alpha, S.alias, st));
     CK(fill_u32(S.n, (u64)S.V * S.K, 0, st));
     HCK(hipEventRecord(S.ev[2], st));
-    CK(launch_sweep(sweep_params(sg, 0), sg->max_len, st));
+    CK(launch_sweep(sweep_params(sg, 0), sg, st));
     HCK(hipEventRecord(S.ev[3], st));
     CK(launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, S.n, S.K, st));
     HCK(hipEventRecord(S.ev[4], st));
@@ -997,27 +1040,32 @@ extern "C" int gpulda_stage_sweep(uint32_t* z, const uint32_t* w, const uint64_t
                                   uint64_t* visited) {
   if (K == 0 || K > kMaxTopics) return fail("bad K");
   const u64 N = doc_off[n_docs];
-  u32 max_len = 0;
+  std::vector<u32> lens(n_docs);
   for (u32 d = 0; d < n_docs; ++d) {
     u64 L = doc_off[d + 1] - doc_off[d];
     if (L > kMaxDocLen) return fail("document %u longer than %u tokens", d, kMaxDocLen);
-    max_len = std::max<u32>(max_len, (u32)L);
+    lens[d] = (u32)L;
   }
   for (u64 i = 0; i < N; ++i)
-    if (z[i] >= K || w[i] >= V) return fail("token %llu has z >= K or w >= V", (unsigned long long)i);
+    if (z[i] >= K) return fail("token %llu has z >= K", (unsigned long long)i);
   const u64 VK = (u64)V * K;
+  Segment* sg = new Segment();
+  int rc = build_segment(sg, w, lens.data(), n_docs, N, V, K);
+  if (rc) {
+    free_segment(sg);
+    return rc;
+  }
+  struct Guard {
+    Segment* s;
+    ~Guard() { free_segment(s); }
+  } guard{sg};
   Scratch sc;
-  u32 *dw, *dz32;
-  u16* dz;
-  u64* doff;
+  u32* dz32;
   f32 *dphi, *dsig;
   AliasSlot* dal;
   unsigned long long* dvis;
   u32* bad;
-  CK(sc.upload(&dw, w, N));
   CK(sc.upload(&dz32, z, N));
-  CK(sc.alloc(&dz, N));
-  CK(sc.upload(&doff, doc_off, (u64)n_docs + 1));
   CK(sc.upload(&dphi, phi, VK));
   CK(sc.upload(&dsig, sigma_a, V));
   CK(pack_alias(sc, prob, alias, VK, &dal));
@@ -1026,13 +1074,13 @@ extern "C" int gpulda_stage_sweep(uint32_t* z, const uint32_t* w, const uint64_t
   CK(fill_u64(dvis, 1, 0, 0));
   CK(fill_u32(bad, 1, 0, 0));
   if (N) {
-    k_z_narrow<<<grid_for(N, 256), 256>>>(dz32, dz, N, K, bad);
+    k_z_narrow<<<grid_for(N, 256), 256>>>(dz32, sg->z, N, K, bad);
     HCK(hipGetLastError());
   }
   SweepParams P;
-  P.w = dw;
-  P.z = dz;
-  P.doc_off = doff;
+  P.w = sg->w;
+  P.z = sg->z;
+  P.doc_off = sg->doc_off;
   P.doc_order = nullptr;
   P.n_docs = n_docs;
   P.doc_base = doc_base;
@@ -1044,9 +1092,9 @@ extern "C" int gpulda_stage_sweep(uint32_t* z, const uint32_t* w, const uint64_t
   P.sigma_a = dsig;
   P.alias = dal;
   P.visited = dvis;
-  CK(launch_sweep(P, max_len, 0));
+  CK(launch_sweep(P, sg, 0));
   if (N) {
-    k_z_widen<<<grid_for(N, 256), 256>>>(dz, dz32, N);
+    k_z_widen<<<grid_for(N, 256), 256>>>(sg->z, dz32, N);
     HCK(hipGetLastError());
   }
   HCK(hipDeviceSynchronize());
@@ -1061,11 +1109,15 @@ extern "C" int gpulda_stage_count(const uint32_t* w, const uint32_t* z, uint64_t
   for (u64 i = 0; i < n_tokens; ++i)
     if (z[i] >= K) return fail("token %llu has z >= K", (unsigned long long)i);
   if (n_tokens >= (1ull << 32)) return fail("too many tokens");
-  // one document holding every token (the reduction does not look at documents)
-  u32 one = (u32)n_tokens;
+  // the reduction does not look at documents: cut the tokens into maximal-length documents
+  std::vector<u32> lens;
+  for (u64 left = n_tokens; left > 0;) {
+    u32 l = (u32)std::min<u64>(left, kMaxDocLen);
+    lens.push_back(l);
+    left -= l;
+  }
   Segment* sg = new Segment();
-  int rc = build_segment(sg, w, &one, n_tokens ? 1 : 0, n_tokens, V, K);
-  if (rc == 0 && n_tokens == 0) rc = 0;
+  int rc = build_segment(sg, w, lens.data(), (u32)lens.size(), n_tokens, V, K);
   if (rc) {
     free_segment(sg);
     return rc;

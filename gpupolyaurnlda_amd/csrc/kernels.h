@@ -40,7 +40,16 @@ __global__ void k_init_counts(const u32* C, u32* n, u32 V, u32 K, f32 beta, u32 
                               u32 rows_per_block);
 __global__ void k_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
                              AliasSlot* out, u32 rows_per_block_waves);
-__global__ void k_sweep(SweepParams P, u32 cap_max);
+template <int LMAX>
+__global__ void k_sweep_lines(SweepParams P);
+// Register-resident line counts of the specialised sweep kernels (0 = any size).
+constexpr int kSweepBuckets[] = {1, 2, 4, 8, 16, 32};
+// Kernel family member for a map of `lines` 64-slot lines.
+inline int sweep_bucket_lmax(u32 lines) {
+  for (int b : kSweepBuckets)
+    if (lines <= (u32)b) return b;
+  return 0;
+}
 __global__ void k_count(const CountTask* tasks, u32 n_tasks, const u32* perm, const u16* z, u32* n, u32 K,
                         u32 waves_per_block);
 

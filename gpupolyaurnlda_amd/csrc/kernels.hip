@@ -252,28 +252,60 @@ __device__ __forceinline__ f32 slot_mass(u32 e, const f32* __restrict__ prow) {
   return e == kMapEmpty ? 0.0f : (f32)map_count(e) * prow[map_key(e)];
 }
 
-__global__ __launch_bounds__(64) void k_sweep(SweepParams P, u32 cap_max) {
-  extern __shared__ u32 sweep_smem[];
+// Builds the doc-topic map of one document from its current topics (count_topics,
+// src/cu/topics.cuh:142-158); returns the number of distinct topics.
+__device__ __forceinline__ u32 build_doc_map(u32* T, u32 cap, const u16* __restrict__ z, u64 b, u32 L) {
   const u32 lane = lane_id();
-  const u32 doc = P.doc_order ? P.doc_order[blockIdx.x] : blockIdx.x;
-  if (doc >= P.n_docs) return;
-  u32* T = sweep_smem;
+  for (u32 s = lane; s < cap; s += 64) T[s] = kMapEmpty;
+  u32 occupied = 0;
+  for (u32 c0 = 0; c0 < L; c0 += 64) {
+    u32 zz = c0 + lane < L ? (u32)z[b + c0 + lane] : 0;
+    u32 nt = L - c0 < 64 ? L - c0 : 64;
+    for (u32 t = 0; t < nt; ++t) occupied += map_inc(T, cap, readlane_u32(zz, t));
+  }
+  return occupied;
+}
+
+// Picks the sparse topic inside line `jsel` (prefix of the line = cprev + canonical scan).
+__device__ __forceinline__ u32 pick_in_line(u32 e, f32 v, f32 cprev, f32 tgt, u32 fallback) {
+  const f32 pv = cprev + wave_scan_f32(v);
+  const u64 cand = __ballot(v > 0.0f && pv > tgt);
+  const u64 pos = __ballot(v > 0.0f);
+  if (!pos) return fallback;  // unreachable: the selected line holds positive mass
+  const u32 l = cand ? first_lane(cand) : 63u - (u32)__clzll((long long)pos);
+  return map_key(readlane_u32(e, l));
+}
+
+// Line index whose running total first exceeds tgt (lane j of cw0 / cw1 holds the total after
+// line j / j + 64).
+__device__ __forceinline__ u32 pick_line(f32 cw0, f32 cw1, u32 lines, f32 tgt) {
+  const u32 lane = lane_id();
+  u64 bw = __ballot(lane < lines && cw0 > tgt);
+  if (bw) return first_lane(bw);
+  bw = __ballot(lane + 64 < lines && cw1 > tgt);
+  if (bw) return 64 + first_lane(bw);
+  return lines - 1;  // unreachable: the last line's total is sigma_b > tgt
+}
+
+__device__ __forceinline__ f32 line_total_before(f32 cw0, f32 cw1, u32 jsel) {
+  if (jsel == 0) return 0.0f;
+  return jsel - 1 < 64 ? readlane_f32(cw0, jsel - 1) : readlane_f32(cw1, jsel - 65);
+}
+
+// One document.  LMAX > 0: the map has at most LMAX lines and the whole line set is held in
+// registers (all phi gathers of a token are issued before the scans).  LMAX == 0: any size, one
+// line at a time.
+template <int LMAX>
+__device__ __forceinline__ void sweep_doc(const SweepParams& P, u32 doc, u32* T) {
+  const u32 lane = lane_id();
   const u64 b = P.doc_off[doc];
   const u32 L = (u32)(P.doc_off[doc + 1] - b);
   const u32 K = P.K;
   const u32 cap = map_capacity(L, K);
   const u32 lines = cap / kMapLine;
   const u32 gdoc = P.doc_base + doc;
-  (void)cap_max;
 
-  for (u32 s = lane; s < cap; s += 64) T[s] = kMapEmpty;
-  u32 occupied = 0;
-  for (u32 c0 = 0; c0 < L; c0 += 64) {
-    u32 zz = c0 + lane < L ? (u32)P.z[b + c0 + lane] : 0;
-    u32 nt = L - c0 < 64 ? L - c0 : 64;
-    for (u32 t = 0; t < nt; ++t) occupied += map_inc(T, cap, readlane_u32(zz, t));
-  }
-
+  u32 occupied = build_doc_map(T, cap, P.z, b, L);
   u64 visited = 0;
   for (u32 c0 = 0; c0 < L; c0 += 64) {
     const u32 i = c0 + lane;
@@ -281,54 +313,80 @@ __global__ __launch_bounds__(64) void k_sweep(SweepParams P, u32 cap_max) {
     const u32 ww = valid ? P.w[b + i] : 0;
     const u32 zz = valid ? (u32)P.z[b + i] : 0;
     const u32x4 r = philox_tagged(i, gdoc, P.iter, kTagSweep, 0, P.k0, P.k1);
+    // per-token values that do not depend on the map: prefetched for the whole chunk
+    const f32 sa_l = valid ? P.sigma_a[ww] : 0.0f;
+    const u32 slot_l = bounded(r.y, K);
+    AliasSlot al_l;
+    al_l.prob = 0.0f;
+    al_l.alias = 0;
+    if (valid) al_l = P.alias[(u64)ww * K + slot_l];
+    const u32 dense_l = u24(r.z) < al_l.prob ? slot_l : al_l.alias;
     u32 znew_lane = zz;
     const u32 nt = L - c0 < 64 ? L - c0 : 64;
     for (u32 t = 0; t < nt; ++t) {
       const u32 wt = readlane_u32(ww, t);
       const u32 zt = readlane_u32(zz, t);
       occupied -= map_dec(T, cap, zt);
-      const f32 sa = P.sigma_a[wt];
+      const f32 sa = readlane_f32(sa_l, t);
       u32 znew = zt;
       if (sa > 0.0f) {
         const f32* prow = P.phi + (u64)wt * K;
         wave_sync();
         f32 carry = 0.0f;
-        f32 cw0 = 0.0f, cw1 = 0.0f;  // running total after line j, held by lane j (j < 128)
-        for (u32 j = 0; j < lines; ++j) {
-          f32 sc = wave_scan_f32(slot_mass(T[j * 64 + lane], prow));
-          carry = carry + readlane_f32(sc, 63);
-          if (j < 64) {
-            if (lane == j) cw0 = carry;
-          } else if (lane == j - 64) {
-            cw1 = carry;
+        f32 cw0 = 0.0f, cw1 = 0.0f;
+        if constexpr (LMAX > 0) {
+          u32 e[LMAX];
+          f32 ph[LMAX];
+#pragma unroll
+          for (int j = 0; j < LMAX; ++j) e[j] = (u32)j < lines ? T[j * 64 + lane] : kMapEmpty;
+#pragma unroll
+          for (int j = 0; j < LMAX; ++j) ph[j] = e[j] != kMapEmpty ? prow[map_key(e[j])] : 0.0f;
+#pragma unroll
+          for (int j = 0; j < LMAX; ++j) {
+            if ((u32)j < lines) {
+              f32 v = e[j] != kMapEmpty ? (f32)map_count(e[j]) * ph[j] : 0.0f;
+              carry = carry + readlane_f32(wave_scan_f32(v), 63);
+              if (lane == (u32)j) cw0 = carry;
+            }
           }
-        }
-        visited += occupied;
-        const f32 sb = carry;
-        const f32 u1 = u24(readlane_u32(r.x, t));
-        if (u1 * (sa + sb) > sa) {
-          const f32 tgt = u24(readlane_u32(r.w, t)) * sb;
-          u32 jsel = lines - 1;  // the last line always qualifies (tgt < sigma_b)
-          u64 bw = __ballot(lane < lines && cw0 > tgt);
-          if (bw) {
-            jsel = first_lane(bw);
+          visited += occupied;
+          const f32 sb = carry;
+          if (u24(readlane_u32(r.x, t)) * (sa + sb) > sa) {
+            const f32 tgt = u24(readlane_u32(r.w, t)) * sb;
+            const u32 jsel = pick_line(cw0, cw1, lines, tgt);
+            u32 es = kMapEmpty;
+            f32 ps = 0.0f;
+#pragma unroll
+            for (int j = 0; j < LMAX; ++j)
+              if ((u32)j == jsel) {
+                es = e[j];
+                ps = ph[j];
+              }
+            const f32 v = es != kMapEmpty ? (f32)map_count(es) * ps : 0.0f;
+            znew = pick_in_line(es, v, line_total_before(cw0, cw1, jsel), tgt, zt);
           } else {
-            bw = __ballot(lane + 64 < lines && cw1 > tgt);
-            if (bw) jsel = 64 + first_lane(bw);
-          }
-          f32 cprev = 0.0f;
-          if (jsel > 0) cprev = jsel - 1 < 64 ? readlane_f32(cw0, jsel - 1) : readlane_f32(cw1, jsel - 65);
-          const u32 e = T[jsel * 64 + lane];
-          const f32 v = slot_mass(e, prow);
-          const f32 pv = cprev + wave_scan_f32(v);
-          const u64 cand = __ballot(v > 0.0f && pv > tgt);
-          const u64 pos = __ballot(v > 0.0f);
-          if (pos) {  // a line with a running total above tgt holds positive mass
-            const u32 l = cand ? first_lane(cand) : 63u - (u32)__clzll((long long)pos);
-            znew = map_key(readlane_u32(e, l));
+            znew = readlane_u32(dense_l, t);
           }
         } else {
-          znew = draw_alias_slot(P.alias + (u64)wt * K, K, readlane_u32(r.y, t), readlane_u32(r.z, t));
+          for (u32 j = 0; j < lines; ++j) {
+            f32 sc = wave_scan_f32(slot_mass(T[j * 64 + lane], prow));
+            carry = carry + readlane_f32(sc, 63);
+            if (j < 64) {
+              if (lane == j) cw0 = carry;
+            } else if (lane == j - 64) {
+              cw1 = carry;
+            }
+          }
+          visited += occupied;
+          const f32 sb = carry;
+          if (u24(readlane_u32(r.x, t)) * (sa + sb) > sa) {
+            const f32 tgt = u24(readlane_u32(r.w, t)) * sb;
+            const u32 jsel = pick_line(cw0, cw1, lines, tgt);
+            const u32 e = T[jsel * 64 + lane];
+            znew = pick_in_line(e, slot_mass(e, prow), line_total_before(cw0, cw1, jsel), tgt, zt);
+          } else {
+            znew = readlane_u32(dense_l, t);
+          }
         }
       }
       occupied += map_inc(T, cap, znew);
@@ -338,6 +396,23 @@ __global__ __launch_bounds__(64) void k_sweep(SweepParams P, u32 cap_max) {
   }
   if (lane == 0 && P.visited) atomicAdd(P.visited, (unsigned long long)visited);
 }
+
+// One wavefront per document, documents taken from P.doc_order (a length bucket) or in order.
+template <int LMAX>
+__global__ __launch_bounds__(64) void k_sweep_lines(SweepParams P) {
+  extern __shared__ u32 sweep_smem[];
+  const u32 doc = P.doc_order ? P.doc_order[blockIdx.x] : blockIdx.x;
+  if (doc >= P.n_docs) return;
+  sweep_doc<LMAX>(P, doc, sweep_smem);
+}
+
+template __global__ void k_sweep_lines<0>(SweepParams);
+template __global__ void k_sweep_lines<1>(SweepParams);
+template __global__ void k_sweep_lines<2>(SweepParams);
+template __global__ void k_sweep_lines<4>(SweepParams);
+template __global__ void k_sweep_lines<8>(SweepParams);
+template __global__ void k_sweep_lines<16>(SweepParams);
+template __global__ void k_sweep_lines<32>(SweepParams);
 
 // ---------------------------------------------------------------------------------------------
 // n_wk reduction over the word-sorted token permutation.  One wavefront per task (a run of at

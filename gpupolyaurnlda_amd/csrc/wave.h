@@ -42,18 +42,20 @@ __device__ __forceinline__ u32 readlane_u32(u32 v, u32 l) {
 //   2. row carries c1 = T0, c2 = T0 + T1, c3 = c2 + T2 (T_r = total of row r), added to rows
 //      1..3.
 // The CPU checker evaluates the same network, so results agree bit for bit.
+// Row shift-right by n inside 16-lane rows; lanes below n read 0 (DPP row_shr:n, bound_ctrl).
+// Adding that 0 leaves the value unchanged, which is the "keep" of the row-local step; values
+// here are never -0.0, so x + 0 == x bitwise.
+template <int N>
+__device__ __forceinline__ f32 row_shr_zero(f32 v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 + N, 0xF, 0xF, true));
+}
+
 __device__ __forceinline__ f32 wave_scan_f32(f32 v) {
   const u32 lane = lane_id();
-  const u32 r = lane & 15;
-  f32 t;
-  t = __shfl_up(v, 1, 16);
-  if (r >= 1) v = v + t;
-  t = __shfl_up(v, 2, 16);
-  if (r >= 2) v = v + t;
-  t = __shfl_up(v, 4, 16);
-  if (r >= 4) v = v + t;
-  t = __shfl_up(v, 8, 16);
-  if (r >= 8) v = v + t;
+  v = v + row_shr_zero<1>(v);
+  v = v + row_shr_zero<2>(v);
+  v = v + row_shr_zero<4>(v);
+  v = v + row_shr_zero<8>(v);
   f32 t0 = readlane_f32(v, 15);
   f32 t1 = readlane_f32(v, 31);
   f32 t2 = readlane_f32(v, 47);

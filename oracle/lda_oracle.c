@@ -27,6 +27,9 @@
  * Parity status of each function is recorded in the file header of tests/test_oracle.py.
  */
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -272,12 +275,26 @@ u32 orc_poisson(u32 n, float beta, const OrcAlias* tables, u32 c0, u32 c1, u32 c
 void orc_phi_draw(const u32* n, u32 V, u32 K, float beta, u32 iter, u64 seed, const OrcAlias* tables,
                   float* phi_tilde, u64* topic_sum) {
   for (u32 k = 0; k < K; ++k) topic_sum[k] = 0;
-  for (u32 v = 0; v < V; ++v)
-    for (u32 k = 0; k < K; ++k) {
-      u32 x = orc_poisson(n[(u64)v * K + k], beta, tables, k, v, iter, TAG_PHI, seed);
-      phi_tilde[(u64)v * K + k] = (float)x;
-      topic_sum[k] += x;
-    }
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+  {
+    u64* part = (u64*)calloc(K, sizeof(u64));
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (long v = 0; v < (long)V; ++v)
+      for (u32 k = 0; k < K; ++k) {
+        u32 x = orc_poisson(n[(u64)v * K + k], beta, tables, k, (u32)v, iter, TAG_PHI, seed);
+        phi_tilde[(u64)v * K + k] = (float)x;
+        part[k] += x;
+      }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+    for (u32 k = 0; k < K; ++k) topic_sum[k] += part[k]; /* integer sums: order-free */
+    free(part);
+  }
 }
 
 /* polya_urn_init (polyaurn.cu:61-79): n_vk ~ Pois(floor(C_v / K) + beta), as integers. */
@@ -293,7 +310,10 @@ void orc_phi_finish(float* phi /* in: phi~, out: phi */, const u64* topic_sum, u
                     float* sigma_a, OrcAlias* alias) {
   float* sf = (float*)malloc(sizeof(float) * K);
   for (u32 k = 0; k < K; ++k) sf[k] = (float)(double)topic_sum[k];
-  for (u32 v = 0; v < V; ++v) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+  for (long v = 0; v < (long)V; ++v) {
     float* row = phi + (u64)v * K;
     for (u32 k = 0; k < K; ++k) row[k] = sf[k] > 0.0f ? row[k] / sf[k] : 0.0f;
     float t = orc_sum64(row, K);
@@ -466,6 +486,15 @@ u64 orc_sweep(u32* z, const u32* w, const u64* doc_off, u32 n_docs, u32 doc_base
 void orc_count(const u32* w, const u32* z, u64 N, u32 K, u32* n, u64 nsize) {
   memset(n, 0, sizeof(u32) * nsize);
   for (u64 i = 0; i < N; ++i) n[(u64)w[i] * K + z[i]] += 1;
+}
+
+/* Threads OpenMP will use for the parallel loops above. */
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }
 
 /* Map layout after building from a list of keys with increments (+1) and decrements (-1),

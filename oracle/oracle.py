@@ -43,6 +43,7 @@ _SIGS = {
     "orc_sweep": (C.c_uint64, [u32p, u32p, u64p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, f32p,
                                f32p, C.POINTER(OrcAlias), C.c_int]),
     "orc_count": (None, [u32p, u32p, C.c_uint64, C.c_uint32, u32p, C.c_uint64]),
+    "orc_threads": (C.c_int, []),
     "orc_loglik": (C.c_double, [u32p, C.c_uint32, C.c_uint32, u32p, u64p, C.c_uint32, C.c_double, C.c_double]),
 }
 

@@ -1,0 +1,92 @@
+"""CPU-side checks of the boundary: the library loads, exports exactly what include/*.h declares,
+the struct layouts match the reference's repr(C) structs (src/ffi.rs:4-13, src/buffer.rs:4-19),
+the train.rs buffer packing is restated faithfully, and the corpus generator is deterministic.
+No GPU call is made here."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = set()
+    for h in ("gpulda_abi.h", "gpulda_ext.h"):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s+\**([a-z_][a-z0-9_]*)\s*\(", src, re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from gpupolyaurnlda_amd import _lib
+    assert os.path.exists(_lib.LIB_PATH), "build the library first"
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    declared = _declared_functions()
+    assert {"initialize", "sample_phi", "sample_z_async", "sync_buffer", "cleanup"} <= declared
+    missing = declared - exported
+    assert not missing, missing
+    # the ctypes table covers exactly the declared set
+    assert set(_lib.EXPORTED) == declared
+
+
+def test_library_loads_without_gpu():
+    from gpupolyaurnlda_amd import _lib
+    lib = _lib.lib()
+    assert lib.gpulda_set_seed(5) == 0 and lib.gpulda_get_seed() == 5
+    assert lib.gpulda_set_seed(0) == 0
+
+
+def test_struct_layouts_match_reference():
+    from gpupolyaurnlda_amd._lib import Args, Buffer
+    assert C.sizeof(Args) == 32
+    assert [Args.alpha.offset, Args.beta.offset, Args.K.offset, Args.V.offset, Args.C.offset,
+            Args.buffer_size.offset, Args.max_D.offset] == [0, 4, 8, 12, 16, 24, 28]
+    assert C.sizeof(Buffer) == 96
+    offs = [getattr(Buffer, f).offset for f, _ in Buffer._fields_]
+    assert offs == [0, 8, 16, 24, 32, 36, 40, 48, 56, 64, 72, 80, 88]
+
+
+def test_header_static_asserts_compile():
+    src = '#include "gpulda_abi.h"\n#include "gpulda_ext.h"\nint main(){return 0;}\n'
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{ROOT}/include", "-x", "c++", "-"], input=src,
+                   text=True, check=True)
+    subprocess.run(["gcc", "-std=c11", "-fsyntax-only", f"-I{ROOT}/include", "-x", "c", "-"], input=src,
+                   text=True, check=True)
+
+
+def test_fill_buffer_restates_train_rs():
+    from gpupolyaurnlda_amd.train import fill_buffer
+    d = np.array([3, 4, 2, 5, 1, 1], np.uint32)
+    # stops before exceeding buffer_size (train.rs:30) or max_D
+    assert fill_buffer(d, 0, buffer_size=9, max_D=100) == (3, 9)
+    assert fill_buffer(d, 3, buffer_size=9, max_D=100) == (3, 7)
+    assert fill_buffer(d, 0, buffer_size=100, max_D=2) == (2, 7)
+    assert fill_buffer(d, 6, buffer_size=100, max_D=2) == (0, 0)
+    # train.rs:27-29 panics as soon as it reads a document larger than a buffer
+    with pytest.raises(ValueError):
+        fill_buffer(np.array([3, 4, 2, 10], np.uint32), 0, buffer_size=9, max_D=100)
+    # empty documents pass through instead of ending the iteration
+    assert fill_buffer(np.array([2, 0, 3], np.uint32), 0, 100, 100) == (3, 5)
+
+
+def test_generator_is_deterministic_and_shaped():
+    from gpupolyaurnlda_amd import corpus
+    spec = corpus.TINY
+    a = corpus.generate(spec)
+    b = corpus.generate(spec)
+    assert np.array_equal(a["w"], b["w"]) and np.array_equal(a["z"], b["z"])
+    assert len(a["d_len"]) == spec.D and a["d_len"].min() >= 1
+    assert abs(a["d_len"].mean() - 50) < 1.5
+    assert a["w"].max() < spec.V and a["z"].max() < spec.K
+    med = corpus.lengths(corpus.MEDIUM)
+    assert med.max() <= 2000 and abs(np.median(med) - 100) <= 3
+    bat = corpus.lengths(corpus.BATCHED.scaled(200000))
+    assert bat.min() >= 4 and bat.max() <= 20000
