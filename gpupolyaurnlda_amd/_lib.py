@@ -103,6 +103,7 @@ _SIGS = {
     "gpulda_corpus_iterate": (C.c_int, [C.c_uint32, f32p]),
     "gpulda_corpus_loglik": (C.c_int, [f64p]),
     "gpulda_corpus_release": (C.c_int, []),
+    "gpulda_calibrate_copy": (C.c_int, [C.c_uint64, C.c_uint32, f32p]),
     "gpulda_stage_philox": (C.c_int, [u32p, u32p, u32p, C.c_uint32]),
     "gpulda_stage_scan64": (C.c_int, [f32p, f32p, C.c_uint32]),
     "gpulda_stage_map": (C.c_int, [u32p, C.c_uint32, C.c_uint32, u32p]),

@@ -195,7 +195,7 @@ int launch_sweep(SweepParams P, const Segment* sg, hipStream_t st) {
   const u64 lim = (u64)max_lds_per_block();
   for (const Segment::Bucket& bk : sg->buckets) {
     if (bk.count == 0) continue;
-    u64 lds = (u64)bk.max_cap * sizeof(u32);
+    u64 lds = ((u64)bk.max_cap + (P.K + 31) / 32) * sizeof(u32);  // map + distinct-topic bitmap
     if (lds > lim) return fail("doc-topic map of %u slots exceeds LDS", bk.max_cap);
     P.doc_order = sg->doc_order + bk.first;
     switch (bk.lmax) {
@@ -875,6 +875,34 @@ extern "C" int gpulda_corpus_loglik(double* out) {
   for (u32 k = 0; k < S.K; ++k) ll += std::lgamma(vb) - std::lgamma(vb + (double)ht[k]);
   for (double x : hd) ll += x;
   *out = ll;
+  return 0;
+}
+
+// Plain copy kernel of known size, for scaling hardware byte counters (FETCH_SIZE/WRITE_SIZE).
+__global__ void k_calibrate_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, u64 n) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+extern "C" int gpulda_calibrate_copy(uint64_t bytes, uint32_t reps, float* ms_per_copy) {
+  const u64 n = bytes / sizeof(uint4);
+  uint4 *a = nullptr, *b = nullptr;
+  CK(dalloc(&a, n));
+  CK(dalloc(&b, n));
+  hipEvent_t e0, e1;
+  HCK(hipEventCreate(&e0));
+  HCK(hipEventCreate(&e1));
+  k_fill_u32<<<grid_for(n * 4, 256), 256>>>((u32*)a, n * 4, 1u);
+  HCK(hipEventRecord(e0, 0));
+  for (u32 r = 0; r < reps; ++r) k_calibrate_copy<<<4096, 256>>>(a, b, n);
+  HCK(hipEventRecord(e1, 0));
+  HCK(hipEventSynchronize(e1));
+  float ms = 0;
+  HCK(hipEventElapsedTime(&ms, e0, e1));
+  if (ms_per_copy) *ms_per_copy = reps ? ms / reps : 0.0f;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dfree(a);
+  dfree(b);
   return 0;
 }
 

@@ -252,18 +252,53 @@ __device__ __forceinline__ f32 slot_mass(u32 e, const f32* __restrict__ prow) {
   return e == kMapEmpty ? 0.0f : (f32)map_count(e) * prow[map_key(e)];
 }
 
+// Number of distinct topics among z[b .. b+L): a K-bit LDS bitmap, then a popcount reduction.
+__device__ __forceinline__ u32 distinct_topics(u32* bits, u32 K, const u16* __restrict__ z, u64 b, u32 L) {
+  const u32 lane = lane_id();
+  const u32 words = (K + 31) / 32;
+  for (u32 i = lane; i < words; i += 64) bits[i] = 0;
+  wave_sync();
+  for (u32 i = lane; i < L; i += 64) {
+    u32 t = z[b + i];
+    atomicOr(&bits[t >> 5], 1u << (t & 31));
+  }
+  wave_sync();
+  u32 c = 0;
+  for (u32 i = lane; i < words; i += 64) c += (u32)__popc(bits[i]);
+  for (u32 off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  return c;
+}
+
 // Builds the doc-topic map of one document from its current topics (count_topics,
-// src/cu/topics.cuh:142-158); returns the number of distinct topics.
-__device__ __forceinline__ u32 build_doc_map(u32* T, u32 cap, const u16* __restrict__ z, u64 b, u32 L) {
+// src/cu/topics.cuh:142-158).
+__device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u16* __restrict__ z, u64 b, u32 L) {
   const u32 lane = lane_id();
   for (u32 s = lane; s < cap; s += 64) T[s] = kMapEmpty;
-  u32 occupied = 0;
   for (u32 c0 = 0; c0 < L; c0 += 64) {
     u32 zz = c0 + lane < L ? (u32)z[b + c0 + lane] : 0;
     u32 nt = L - c0 < 64 ? L - c0 : 64;
-    for (u32 t = 0; t < nt; ++t) occupied += map_inc(T, cap, readlane_u32(zz, t));
+    for (u32 t = 0; t < nt; ++t) map_inc(T, cap, readlane_u32(zz, t));
   }
-  return occupied;
+}
+
+// Regrows the map to capacity ncap in the middle of a sweep: clears it and re-inserts the topic
+// of every token except token t of the current chunk c0.  Tokens before c0 hold their new topics
+// in z, lanes < t of the chunk in znew_lane, lanes > t in zz, and tokens after the chunk their
+// old topics in z -- exactly the multiset the map held, so (the layout being a function of the
+// key set) the result equals any other way of moving the entries.
+__device__ __noinline__ void regrow_doc_map(u32* T, u32 ncap, const u16* __restrict__ z, u64 b, u32 L, u32 c0,
+                                            u32 t, u32 znew_lane, u32 zz) {
+  const u32 lane = lane_id();
+  wave_sync();
+  for (u32 s = lane; s < ncap; s += 64) T[s] = kMapEmpty;
+  for (u32 c = 0; c < L; c += 64) {
+    u32 nt = L - c < 64 ? L - c : 64;
+    u32 v;
+    if (c == c0) v = lane < t ? znew_lane : zz;
+    else v = c + lane < L ? (u32)z[b + c + lane] : 0;
+    for (u32 tt = 0; tt < nt; ++tt)
+      if (!(c == c0 && tt == t)) map_inc(T, ncap, readlane_u32(v, tt));
+  }
 }
 
 // Picks the sparse topic inside line `jsel` (prefix of the line = cprev + canonical scan).
@@ -301,11 +336,15 @@ __device__ __forceinline__ void sweep_doc(const SweepParams& P, u32 doc, u32* T)
   const u64 b = P.doc_off[doc];
   const u32 L = (u32)(P.doc_off[doc + 1] - b);
   const u32 K = P.K;
-  const u32 cap = map_capacity(L, K);
-  const u32 lines = cap / kMapLine;
+  const u32 cap_max = map_capacity(L, K);  // never exceeded; LDS holds cap_max slots + a K-bit map
+  const u32 kmax = L < K ? L : K;
   const u32 gdoc = P.doc_base + doc;
 
-  u32 occupied = build_doc_map(T, cap, P.z, b, L);
+  // Capacity rule: start sized for the n0 distinct topics; grow (rebuild) when a new topic would
+  // exceed the load limit.  The reference starts from the K_d hint (hashmap.cuh:98).
+  u32 occupied = distinct_topics(T + cap_max, K, P.z, b, L);
+  u32 cap = map_capacity(occupied, K);
+  build_doc_map(T, cap, P.z, b, L);
   u64 visited = 0;
   for (u32 c0 = 0; c0 < L; c0 += 64) {
     const u32 i = c0 + lane;
@@ -328,6 +367,7 @@ __device__ __forceinline__ void sweep_doc(const SweepParams& P, u32 doc, u32* T)
       const u32 zt = readlane_u32(zz, t);
       occupied -= map_dec(T, cap, zt);
       const f32 sa = readlane_f32(sa_l, t);
+      const u32 lines = cap / kMapLine;
       u32 znew = zt;
       if (sa > 0.0f) {
         const f32* prow = P.phi + (u64)wt * K;
@@ -389,7 +429,15 @@ __device__ __forceinline__ void sweep_doc(const SweepParams& P, u32 doc, u32* T)
           }
         }
       }
-      occupied += map_inc(T, cap, znew);
+      if (!map_probe(T, cap, znew).found) {
+        if (map_capacity(occupied + 1, K) > cap) {
+          const u32 want = 2 * (occupied + 1) < kmax ? 2 * (occupied + 1) : kmax;
+          cap = map_capacity(want, K);
+          regrow_doc_map(T, cap, P.z, b, L, c0, t, znew_lane, zz);
+        }
+        ++occupied;
+      }
+      map_inc(T, cap, znew);
       if (lane == t) znew_lane = znew;
     }
     if (valid) P.z[b + i] = (u16)znew_lane;

@@ -75,6 +75,10 @@ int gpulda_corpus_iterate(uint32_t n_iters, float* stage_ms);
 int gpulda_corpus_loglik(double* out);
 int gpulda_corpus_release(void);
 
+/* Measurement aid: `reps` launches of a plain device-to-device copy kernel over `bytes` bytes
+ * (reads `bytes`, writes `bytes`), used to scale hardware byte counters; mean ms per copy. */
+int gpulda_calibrate_copy(uint64_t bytes, uint32_t reps, float* ms_per_copy);
+
 /* ---- stage entry points (host arrays in and out; the library stages device copies) ------- */
 int gpulda_stage_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n);
 int gpulda_stage_scan64(const float* in, float* out, uint32_t n_groups);

@@ -371,6 +371,34 @@ void orc_map_inc(u32* T, u32 cap, u32 key) {
   }
 }
 
+/* Inserts an absent key with its count (Robin Hood order). */
+static void map_put(u32* T, u32 cap, u32 cur) {
+  u32 slot = home_of(cur >> 16, cap);
+  u32 d = 0;
+  for (;;) {
+    u32 e = T[slot];
+    if (e == MAP_EMPTY) { T[slot] = cur; return; }
+    u32 de = dist(slot, home_of(e >> 16, cap), cap);
+    if (de < d || (de == d && (e >> 16) > (cur >> 16))) {
+      T[slot] = cur;
+      cur = e;
+      d = de;
+    }
+    slot = (slot + 1) % cap;
+    d += 1;
+  }
+}
+
+/* Moves every entry of a table of capacity `cap` into a cleared table of capacity `ncap`
+ * (same array, ncap >= cap). */
+static void map_regrow(u32* T, u32 cap, u32 ncap, u32* scratch) {
+  u32 n = 0;
+  for (u32 s = 0; s < cap; ++s)
+    if (T[s] != MAP_EMPTY) scratch[n++] = T[s];
+  for (u32 s = 0; s < ncap; ++s) T[s] = MAP_EMPTY;
+  for (u32 i = 0; i < n; ++i) map_put(T, ncap, scratch[i]);
+}
+
 void orc_map_dec(u32* T, u32 cap, u32 key) {
   long s = orc_map_find(T, cap, key);
   if (s < 0) abort();
@@ -392,27 +420,41 @@ void orc_map_dec(u32* T, u32 cap, u32 key) {
 /* interval holds the target), draw_alias (topics.cuh:63-83).                                  */
 /* Returns the number of map entries that the sparse sums visited (sum over tokens).           */
 /* ------------------------------------------------------------------------------------------ */
+/* Capacity rule: the map starts sized for the document's distinct topics n0
+ * (orc_map_capacity(n0, K)) and, when a new topic would exceed its load
+ * (orc_map_capacity(keys + 1, K) > cap), grows to orc_map_capacity(min(2 (keys + 1), min(L, K)), K).
+ * The reference starts from the K_d hint and regrows by 1.2x (hashmap.cuh:98, 159-207). */
 u64 orc_sweep_doc(u32* z, const u32* w, u32 L, u32 doc, u32 iter, u64 seed, u32 K, const float* phi,
-                  const float* sigma_a, const OrcAlias* alias, u32* T /* scratch >= 4*capacity */) {
-  u32 cap = orc_map_capacity(L, K);
-  u32 lines = cap / 64;
-  float* mass = (float*)(T + cap);   /* count * phi per slot */
-  float* pre = mass + cap;           /* carry of the line + inclusive scan inside the line */
-  float* after = pre + cap;          /* running total after each line */
-  for (u32 s = 0; s < cap; ++s) T[s] = MAP_EMPTY;
-  for (u32 i = 0; i < L; ++i) orc_map_inc(T, cap, z[i]);
+                  const float* sigma_a, const OrcAlias* alias, u32* T /* scratch >= 5*capacity(L,K) */) {
+  const u32 cap_max = orc_map_capacity(L, K);
+  float* mass = (float*)(T + cap_max);   /* count * phi per slot */
+  float* pre = mass + cap_max;           /* carry of the line + inclusive scan inside the line */
+  float* after = pre + cap_max;          /* running total after each line */
+  u32* scratch = (u32*)(after + cap_max);
+  u32 n0 = 0;
+  for (u32 s = 0; s < cap_max; ++s) T[s] = MAP_EMPTY;
+  for (u32 i = 0; i < L; ++i) {
+    if (orc_map_find(T, cap_max, z[i]) < 0) ++n0;
+    orc_map_inc(T, cap_max, z[i]);
+  }
+  u32 cap = orc_map_capacity(n0, K);
+  map_regrow(T, cap_max, cap, scratch);  /* the same key set at the starting capacity */
+  u32 keys = n0;
+  const u32 kmax = L < K ? L : K;
   u64 visited = 0;
   float P[64];
   for (u32 i = 0; i < L; ++i) {
     u32 wi = w[i], zi = z[i];
     u32 r[4];
     draw4(i, doc, iter, TAG_SWEEP, 0, seed, r);
+    long at = orc_map_find(T, cap, zi);
+    if ((T[at] & 0xFFFFu) == 1) --keys;
     orc_map_dec(T, cap, zi);
     float sa = sigma_a[wi];
     u32 znew = zi;
     if (sa > 0.0f) {
       const float* prow = phi + (u64)wi * K;
-      /* sigma_b: canonical scan, line by line, carried left to right */
+      const u32 lines = cap / 64;
       float carry = 0.0f;
       for (u32 j = 0; j < lines; ++j) {
         for (u32 l = 0; l < 64; ++l) {
@@ -447,6 +489,15 @@ u64 orc_sweep_doc(u32* z, const u32* w, u32 L, u32 doc, u32 iter, u64 seed, u32 
         znew = draw_alias(alias + (u64)wi * K, K, r[1], r[2]);
       }
     }
+    if (orc_map_find(T, cap, znew) < 0) {
+      if (orc_map_capacity(keys + 1, K) > cap) {
+        u32 want = 2 * (keys + 1) < kmax ? 2 * (keys + 1) : kmax;
+        u32 ncap = orc_map_capacity(want, K);
+        map_regrow(T, cap, ncap, scratch);
+        cap = ncap;
+      }
+      ++keys;
+    }
     orc_map_inc(T, cap, znew);
     z[i] = znew;
   }
@@ -461,7 +512,7 @@ u64 orc_sweep(u32* z, const u32* w, const u64* doc_off, u32 n_docs, u32 doc_base
     u32 L = (u32)(doc_off[d + 1] - doc_off[d]);
     if (L > maxL) maxL = L;
   }
-  u32 maxcap = 4 * orc_map_capacity(maxL, K);
+  u32 maxcap = 5 * orc_map_capacity(maxL, K);
   u64 visited = 0;
 #ifdef _OPENMP
 #pragma omp parallel num_threads(threads > 0 ? threads : 1) reduction(+ : visited)

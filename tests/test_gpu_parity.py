@@ -250,3 +250,28 @@ def test_resident_device_init_equals_generator_init():
     c = _tiny_corpus()
     with ResidentCorpus(c["w"], c["d_len"], TINY.K, TINY.V, TINY.alpha, TINY.beta, seed=0, z=None) as rc:
         assert np.array_equal(rc.z(), c["z"])
+
+
+def test_sweep_growth_bit_exact(S, orc):
+    """Documents whose distinct-topic count grows far beyond the starting map (regrow path)."""
+    rng = np.random.default_rng(11)
+    V, K = 400, 1024
+    pt = (rng.random((V, K)) < 0.02).astype(np.float32) * rng.integers(1, 4, (V, K)).astype(np.float32)
+    pt[:, 0] += 1
+    phi, sig, prob, al = orc.phi_finish(pt, pt.sum(0).astype(np.uint64), 0.5)
+    L = np.array([1, 64, 300, 900, 2000], np.uint32)
+    off = np.concatenate([[0], np.cumsum(L)]).astype(np.uint64)
+    N = int(off[-1])
+    w = rng.integers(0, V, N).astype(np.uint32)
+    z = np.zeros(N, np.uint32)  # one distinct topic per document at the start
+    zg, vg = S.sweep(z, w, off, 5, 2, 9, K, phi, sig, prob, al)
+    zc, vc = orc.sweep(z, w, off, 5, 2, 9, K, phi, sig, prob, al)
+    assert vg == vc and np.array_equal(zg, zc)
+    assert len(np.unique(zg[int(off[3]):int(off[4])])) > 100
+    # known answer: one-hot rows force topic 3v+1
+    pt1 = np.zeros((300, K), np.float32)
+    pt1[np.arange(300), (3 * np.arange(300) + 1) % K] = 5
+    phi1, sig1, prob1, al1 = orc.phi_finish(pt1, np.maximum(pt1.sum(0), 1).astype(np.uint64), 0.05)
+    w1 = np.arange(300, dtype=np.uint32)
+    z1, _ = S.sweep(np.zeros(300, np.uint32), w1, np.array([0, 300], np.uint64), 0, 1, 3, K, phi1, sig1, prob1, al1)
+    assert np.array_equal(z1, (3 * w1 + 1) % K)

@@ -376,3 +376,19 @@ def test_phi_step_properties(orc):
     colsum = phi.astype(np.float64).sum(0)
     assert np.allclose(colsum[ts > 0], 1.0, atol=1e-5)
     assert np.allclose(sig, 0.5 * phi.astype(np.float64).sum(1), rtol=1e-5)
+
+
+def test_sweep_growth_known_answer(orc):
+    # Every token starts at topic 0 (one distinct topic: a one-line map); word i has all its phi
+    # mass on topic 3i+1, so after one sweep token i must hold topic 3i+1 -- 300 distinct
+    # topics, reached only through repeated map growth.
+    V, K, L = 300, 1024, 300
+    phi_t = np.zeros((V, K), np.float32)
+    for v in range(V):
+        phi_t[v, (3 * v + 1) % K] = 5
+    phi, sig, prob, al = orc.phi_finish(phi_t, np.maximum(phi_t.sum(0), 1).astype(np.uint64), 0.05)
+    w = np.arange(L, dtype=np.uint32)
+    z = np.zeros(L, np.uint32)
+    znew, visited = orc.sweep(z, w, np.array([0, L], np.uint64), 0, 1, 3, K, phi, sig, prob, al)
+    assert np.array_equal(znew, (3 * w + 1) % K)
+    assert visited > 0
