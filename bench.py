@@ -41,6 +41,11 @@ def main():
     ap.add_argument("--docs", type=int, default=0, help="override D (smaller runs for profiling)")
     ap.add_argument("--cpu-docs", type=int, default=20000, help="documents in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--schedule", default="auto", choices=["auto", "buckets", "persistent", "block"],
+                    help="sweep schedule; auto = the autotuner's choice after warmup")
+    ap.add_argument("--compare-graph", action="store_true",
+                    help="also time the iteration as a replayed hipGraph against plain per-bucket launches "
+                         "(identical Philox streams, z compared)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -67,6 +72,14 @@ def main():
     if args.warmup:
         rc.iterate(args.warmup)
         log(f"warmup {args.warmup} done")
+    tune = None
+    if args.schedule == "auto":
+        rows, chosen = rc.autotune()
+        tune = {"candidates": rows, "chosen": rows[chosen]}
+        log(f"autotune: chose {rows[chosen]} from {rows}")
+    else:
+        kind = {"buckets": 0, "persistent": 1, "block": 2}[args.schedule]
+        rc.schedule(kind, 16, 4096 if kind == 2 else 0)
     n_before = rc.counts() if not args.no_cpu else None
     z_before = rc.z() if not args.no_cpu else None
     rc.reset_stats()
@@ -93,12 +106,16 @@ def main():
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_sweep", "kernel_ms": round(sweep_ms, 3), "alg_bytes_per_launch": int(alg_bytes),
+                "kernel": "z-sweep (k_sweep_*: all launches of one sweep)", "kernel_ms": round(sweep_ms, 3), "alg_bytes_per_launch": int(alg_bytes),
                 "s_bar": round(s_bar, 3)}
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(rc, spec, c, n_before, z_before, it0, args.steps, args.cpu_docs)
+
+    graph = None
+    if args.compare_graph:  # last: it releases rc (the library holds one model at a time)
+        graph = compare_graph(rc, c, spec, args.steps)
 
     out = {
         "metric": "tokens sampled/sec at K=1024 (% HBM roofline); fold-in docs/s + p99 latency",
@@ -117,12 +134,43 @@ def main():
                    "alpha": spec.alpha, "beta": spec.beta, "seed": spec.seed,
                    "stage_ms": {"phi_draw": round(float(stage_ms[0]), 3), "alias": round(float(stage_ms[1]), 3),
                                 "sweep": round(float(stage_ms[2]), 3), "count": round(float(stage_ms[3]), 3)},
-                   "parallelism": "1 GPU"},
+                   "parallelism": "1 GPU",
+                   "schedule": tune if tune is not None else args.schedule,
+                   "graph_vs_plain": graph},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     rc.close()
     print(json.dumps(out), flush=True)
+
+
+def compare_graph(rc, c, spec, steps):
+    """Batched form vs plain form on the same chain: the current schedule replayed as a hipGraph
+    against per-bucket launches, from the same state; z must come out identical."""
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+
+    z0 = rc.z()
+    it0 = rc.iteration
+    t0 = time.perf_counter()
+    rc.use_graph(True)
+    rc.iterate(steps)
+    t_graph = (time.perf_counter() - t0) / steps
+    rc.use_graph(False)
+    z_graph = rc.z()
+    rc.close()
+    # plain form: the same chain replayed from the start with per-bucket launches
+    plain = ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=spec.seed, z=c["z"])
+    plain.schedule(0)
+    plain.iterate_untimed(it0)
+    same_start = bool(np.array_equal(plain.z(), z0))
+    t0 = time.perf_counter()
+    plain.iterate_untimed(steps)
+    t_plain = (time.perf_counter() - t0) / steps
+    same = bool(np.array_equal(plain.z(), z_graph))
+    plain.close()
+    log(f"graph {t_graph * 1e3:.1f} ms/iter vs plain per-bucket {t_plain * 1e3:.1f} ms/iter; identical z: {same}")
+    return {"graph_ms_per_iter": round(t_graph * 1e3, 3), "plain_buckets_ms_per_iter": round(t_plain * 1e3, 3),
+            "identical_z": same and same_start}
 
 
 def cpu_baseline(rc, spec, c, n_before, z_before, it0, steps, cpu_docs):

@@ -102,6 +102,9 @@ _SIGS = {
     "gpulda_corpus_copy_z": (C.c_int, [u32p]),
     "gpulda_corpus_iterate": (C.c_int, [C.c_uint32, f32p]),
     "gpulda_corpus_loglik": (C.c_int, [f64p]),
+    "gpulda_corpus_schedule": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32]),
+    "gpulda_corpus_use_graph": (C.c_int, [C.c_int]),
+    "gpulda_corpus_autotune": (C.c_int, [f32p, C.c_uint32, u32p, C.POINTER(C.c_int)]),
     "gpulda_corpus_release": (C.c_int, []),
     "gpulda_calibrate_copy": (C.c_int, [C.c_uint64, C.c_uint32, f32p]),
     "gpulda_stage_philox": (C.c_int, [u32p, u32p, u32p, C.c_uint32]),
@@ -114,6 +117,15 @@ _SIGS = {
     "gpulda_stage_sweep": (C.c_int, [u32p, u32p, u64p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
                                      C.c_uint32, f32p, f32p, f32p, u32p, u64p]),
     "gpulda_stage_count": (C.c_int, [u32p, u32p, C.c_uint64, C.c_uint32, C.c_uint32, u32p]),
+    "gpulda_stage_foldin": (C.c_int, [u32p, u64p, C.c_uint32, u32p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
+                                      f32p, f32p, f32p, u32p, u32p, u64p]),
+    "gpulda_serve_open": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "gpulda_serve_submit": (C.c_int, [u32p, C.c_uint32, C.c_uint32, u32p]),
+    "gpulda_serve_flush": (C.c_int, []),
+    "gpulda_serve_poll": (C.c_int, [u32p, C.c_uint32, u32p]),
+    "gpulda_serve_result": (C.c_int, [C.c_uint32, u32p, u32p]),
+    "gpulda_serve_stats": (C.c_int, [u32p, u32p, u64p]),
+    "gpulda_serve_close": (C.c_int, []),
 }
 
 # Declared in include/*.h; the "library exports every declared symbol" test reads this list.

@@ -121,13 +121,21 @@ int launch_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* di
 
 constexpr u32 kPhiRows = 64;
 
-int launch_phi_draw(const u32* n, f32* phi, unsigned long long* tsum, u32 V, u32 K, f32 beta, u32 iter, u64 seed,
-                    const AliasSlot* tables, hipStream_t st) {
+int launch_phi_draw(const u32* n, f32* phi, unsigned long long* tsum, u32 V, u32 K, f32 beta, u32 iter,
+                    const u32* iter_ptr, u64 seed, const AliasSlot* tables, hipStream_t st) {
   CK(fill_u64(tsum, K, 0, st));
   dim3 grid((K + 255) / 256, (V + kPhiRows - 1) / kPhiRows);
-  k_phi_draw<<<grid, 256, 0, st>>>(n, phi, tsum, V, K, beta, iter, (u32)seed, (u32)(seed >> 32), tables, kPhiRows);
+  k_phi_draw<<<grid, 256, 0, st>>>(n, phi, tsum, V, K, beta, iter, iter_ptr, (u32)seed, (u32)(seed >> 32), tables,
+                                   kPhiRows);
   HCK(hipGetLastError());
   return 0;
+}
+
+int lds_per_cu() {
+  int dev = 0, v = 65536;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+  return v;
 }
 
 int launch_count(const CountTask* tasks, u32 n_tasks, const u32* perm, const u16* z, u32* n, u32 K, hipStream_t st) {
@@ -181,6 +189,8 @@ struct Segment {
   u32* perm = nullptr;
   CountTask* tasks = nullptr;
   u32* doc_order = nullptr;  // documents grouped by sweep bucket (longest first inside a bucket)
+  u32* doc_lpt = nullptr;    // all documents, longest first (queue of the persistent schedules)
+  std::vector<u32> lens_desc;  // document lengths in doc_lpt order (host copy)
   struct Bucket {
     int lmax;      // register-resident lines of the kernel (0 = generic)
     u32 first;     // offset into doc_order
@@ -220,6 +230,7 @@ void free_segment(Segment* s) {
   dfree(s->perm);
   dfree(s->tasks);
   dfree(s->doc_order);
+  dfree(s->doc_lpt);
   delete s;
 }
 
@@ -303,6 +314,13 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
   s->n_tasks = (u32)tasks.size();
   CK(dalloc(&s->doc_order, n_docs));
   if (n_docs) HCK(hipMemcpy(s->doc_order, order.data(), (u64)n_docs * sizeof(u32), hipMemcpyHostToDevice));
+  std::vector<u32> lpt(n_docs);
+  for (u32 i = 0; i < n_docs; ++i) lpt[i] = i;
+  std::stable_sort(lpt.begin(), lpt.end(), [&](u32 a, u32 b) { return d[a] > d[b]; });
+  s->lens_desc.resize(n_docs);
+  for (u32 i = 0; i < n_docs; ++i) s->lens_desc[i] = d[lpt[i]];
+  CK(dalloc(&s->doc_lpt, n_docs));
+  if (n_docs) HCK(hipMemcpy(s->doc_lpt, lpt.data(), (u64)n_docs * sizeof(u32), hipMemcpyHostToDevice));
   CK(dalloc(&s->w, n_tokens));
   CK(dalloc(&s->z, n_tokens));
   CK(dalloc(&s->doc_off, (u64)n_docs + 1));
@@ -327,6 +345,8 @@ struct BufferState {
   Segment* seg = nullptr;  // segment of the last call (uncached mode owns it)
 };
 
+enum Schedule : int { kSchedBuckets = 0, kSchedPersistent = 1, kSchedPersistentBlock = 2 };
+
 struct State {
   bool ready = false;
   Args* args = nullptr;
@@ -348,20 +368,42 @@ struct State {
   std::vector<Segment*> cache;
   Segment* resident = nullptr;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  u32* iter_dev = nullptr;   // iteration counter on the device (bumped at each Phi step)
+  u32* heads = nullptr;      // work-queue heads of the persistent sweeps
+  // resident-corpus sweep schedule (see launch_sweep_schedule)
+  int schedule = kSchedBuckets;
+  u32 waves_per_cu = 16;
+  u32 block_min_len = 0;
+  bool use_graph = false;
+  hipGraphExec_t graph = nullptr;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  int cu_count = 0;
 };
 
 State S;
 u64 g_seed = 0;
+
+// A captured iteration refers to the resident segment and the schedule: rebuilt on change.
+void drop_graph() {
+  if (S.graph) (void)hipGraphExecDestroy(S.graph);
+  S.graph = nullptr;
+}
 
 int require_ready() {
   if (!S.ready) return fail("library not initialised (call initialize first)");
   return 0;
 }
 
-int phi_step(hipStream_t st) {
+// Phi step of the next iteration: bump the device iteration counter, Poisson draw, alias rows,
+// and clear n_wk for the sweeps that follow (sample_phi, train.cu:137-156).
+int phi_step(hipStream_t st, hipEvent_t mid = nullptr) {
   S.iter += 1;
   S.doc_counter = 0;
-  CK(launch_phi_draw(S.n, S.phi, S.topic_sum, S.V, S.K, S.beta, S.iter, S.seed, S.pois, st));
+  k_bump_u32<<<1, 64, 0, st>>>(S.iter_dev);
+  HCK(hipGetLastError());
+  CK(launch_phi_draw(S.n, S.phi, S.topic_sum, S.V, S.K, S.beta, 0, S.iter_dev, S.seed, S.pois, st));
+  if (mid) HCK(hipEventRecord(mid, st));
   CK(launch_alias_rows(S.phi, S.V, S.K, S.topic_sum, S.sigma_a, S.alpha, S.alias, st));
   CK(fill_u32(S.n, (u64)S.V * S.K, 0, st));
   return 0;
@@ -375,7 +417,8 @@ SweepParams sweep_params(Segment* sg, u32 doc_base) {
   P.doc_order = nullptr;
   P.n_docs = sg->n_docs;
   P.doc_base = doc_base;
-  P.iter = S.iter;
+  P.iter = 0;
+  P.iter_ptr = S.iter_dev;
   P.k0 = (u32)S.seed;
   P.k1 = (u32)(S.seed >> 32);
   P.K = S.K;
@@ -384,6 +427,46 @@ SweepParams sweep_params(Segment* sg, u32 doc_base) {
   P.alias = S.alias;
   P.visited = S.visited;
   return P;
+}
+
+// LDS words one wavefront needs for any document of the segment (map + distinct-topic bitmap).
+u64 sweep_lds_bytes(const Segment* sg) { return ((u64)map_capacity(sg->max_len, S.K) + (S.K + 31) / 32) * 4; }
+
+// Resident sweep under the configured schedule:
+//   buckets              one launch per length bucket (k_sweep_lines<LMAX>), longest maps first;
+//   persistent           one launch of cu_count * waves_per_cu wavefronts taking documents from a
+//                        longest-first queue (k_sweep_persistent);
+//   persistent + block   documents of at least block_min_len tokens go to a block-per-document
+//                        persistent launch (k_sweep_block) on a second stream, the rest as above.
+// All schedules produce bit-identical z: a document's randomness depends on its global id only.
+int launch_sweep_schedule(const SweepParams& P0, Segment* sg, hipStream_t st) {
+  if (S.schedule == kSchedBuckets || sg->n_docs == 0) return launch_sweep(P0, sg, st);
+  SweepParams P = P0;
+  P.doc_order = sg->doc_lpt;
+  const u64 lds = sweep_lds_bytes(sg);
+  if (lds > (u64)max_lds_per_block()) return fail("doc-topic map exceeds LDS");
+  u32 n_long = 0;
+  if (S.schedule == kSchedPersistentBlock && S.block_min_len > 0)
+    while (n_long < sg->n_docs && sg->lens_desc[n_long] >= S.block_min_len) ++n_long;
+  CK(fill_u32(S.heads, 2, 0, st));
+  if (n_long > 0) {
+    HCK(hipEventRecord(S.fork, st));
+    HCK(hipStreamWaitEvent(S.stream2, S.fork, 0));
+    u32 blocks = std::min<u32>(n_long, (u32)S.cu_count * 2);
+    k_sweep_block<<<blocks, 256, lds, S.stream2>>>(P, S.heads + 1, n_long);
+    HCK(hipGetLastError());
+    HCK(hipEventRecord(S.join, S.stream2));
+  }
+  if (n_long < sg->n_docs) {
+    SweepParams Q = P;
+    Q.doc_order = sg->doc_lpt + n_long;
+    u32 per_cu = std::max<u32>(1, std::min<u32>(S.waves_per_cu, (u32)(lds_per_cu() / lds)));
+    u32 waves = std::min<u32>((u32)S.cu_count * per_cu, sg->n_docs - n_long);
+    k_sweep_persistent<8><<<waves, 64, lds, st>>>(Q, S.heads, sg->n_docs - n_long);  // lines <= 8 in registers
+    HCK(hipGetLastError());
+  }
+  if (n_long > 0) HCK(hipStreamWaitEvent(st, S.join, 0));
+  return 0;
 }
 
 int sweep_and_count(Segment* sg, u32 doc_base, hipStream_t st) {
@@ -451,6 +534,15 @@ void release_all() {
   dfree(S.topic_sum);
   dfree(S.visited);
   dfree(S.pois);
+  dfree(S.iter_dev);
+  dfree(S.heads);
+  if (S.graph) (void)hipGraphExecDestroy(S.graph);
+  S.graph = nullptr;
+  if (S.fork) (void)hipEventDestroy(S.fork);
+  if (S.join) (void)hipEventDestroy(S.join);
+  S.fork = S.join = nullptr;
+  if (S.stream2) (void)hipStreamDestroy(S.stream2);
+  S.stream2 = nullptr;
   for (auto& e : S.ev)
     if (e) {
       (void)hipEventDestroy(e);
@@ -463,6 +555,7 @@ void release_all() {
 
 int do_initialize(Args* args, Buffer* buffers, u32 n_buffers) {
   if (!args) return fail("initialize: args is NULL");
+  gpulda_serve_close();
   if (S.ready) release_all();
   if (args->K == 0 || args->K > 4096) return fail("K = %u outside 1..4096", args->K);
   if (args->V == 0) return fail("V must be positive");
@@ -478,7 +571,18 @@ int do_initialize(Args* args, Buffer* buffers, u32 n_buffers) {
   S.iter = 0;
   S.doc_counter = 0;
   HCK(hipStreamCreate(&S.stream));
+  HCK(hipStreamCreate(&S.stream2));
+  HCK(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
+  HCK(hipEventCreateWithFlags(&S.join, hipEventDisableTiming));
   for (auto& e : S.ev) HCK(hipEventCreate(&e));
+  {
+    int dev = 0;
+    HCK(hipGetDevice(&dev));
+    HCK(hipDeviceGetAttribute(&S.cu_count, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  CK(dalloc(&S.iter_dev, 1));
+  CK(dalloc(&S.heads, 2));
+  CK(fill_u32(S.iter_dev, 1, 0, S.stream));
   const u64 VK = (u64)S.V * S.K;
   CK(dalloc(&S.phi, VK));
   CK(dalloc(&S.alias, VK));
@@ -573,6 +677,7 @@ int do_sync_buffer(Buffer* B) {
 
 int do_cleanup(Buffer* buffers, u32 n_buffers) {
   if (!S.ready) return fail("cleanup: library not initialised");
+  gpulda_serve_close();
   HCK(hipDeviceSynchronize());
   for (u32 i = 0; i < n_buffers; ++i) {
     Buffer* B = &buffers[i];
@@ -743,6 +848,7 @@ extern "C" int gpulda_corpus_load(const uint32_t* w, const uint32_t* d_len, uint
   CK(require_ready());
   if (!w || !d_len) return fail("corpus_load: w and d_len are required");
   HCK(hipDeviceSynchronize());
+  drop_graph();
   free_segment(S.resident);
   S.resident = nullptr;
   u64 N = 0;
@@ -807,38 +913,166 @@ extern "C" int gpulda_corpus_copy_z(uint32_t* z) {
   return 0;
 }
 
+namespace {
+// One full iteration of the resident corpus on stream st; ev (5 events or null) brackets the
+// Poisson draw, alias rows (+ n_wk clear), sweep and reduction.
+int resident_iteration(Segment* sg, hipStream_t st, hipEvent_t* ev) {
+  if (ev) HCK(hipEventRecord(ev[0], st));
+  CK(phi_step(st, ev ? ev[1] : nullptr));
+  if (ev) HCK(hipEventRecord(ev[2], st));
+  CK(launch_sweep_schedule(sweep_params(sg, 0), sg, st));
+  if (ev) HCK(hipEventRecord(ev[3], st));
+  CK(launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, S.n, S.K, st));
+  if (ev) HCK(hipEventRecord(ev[4], st));
+  S.doc_counter = sg->n_docs;
+  return 0;
+}
+
+// Captures one iteration into a graph (the iteration number lives in device memory, so a replay
+// is the next iteration).
+int capture_graph(Segment* sg) {
+  drop_graph();
+  hipGraph_t g = nullptr;
+  HCK(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
+  const u32 iter_before = S.iter;
+  int rc = resident_iteration(sg, S.stream, nullptr);
+  hipError_t e = hipStreamEndCapture(S.stream, &g);
+  S.iter = iter_before;  // capture does not run anything
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HCK(e);
+  hipError_t ei = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HCK(ei);
+  return 0;
+}
+}  // namespace
+
 extern "C" int gpulda_corpus_iterate(uint32_t n_iters, float* stage_ms) {
   CK(require_ready());
   if (!S.resident) return fail("no resident corpus");
   Segment* sg = S.resident;
   hipStream_t st = S.stream;
   double acc[4] = {0, 0, 0, 0};
-  for (u32 it = 0; it < n_iters; ++it) {
-    S.iter += 1;
-    S.doc_counter = 0;
+  if (S.use_graph) {
+    if (!S.graph) CK(capture_graph(sg));
     HCK(hipEventRecord(S.ev[0], st));
-    CK(launch_phi_draw(S.n, S.phi, S.topic_sum, S.V, S.K, S.beta, S.iter, S.seed, S.pois, st));
-    HCK(hipEventRecord(S.ev[1], st));
-    CK(launch_alias_rows(S.phi, S.V, S.K, S.topic_sum, S.sigma_a, S.alpha, S.alias, st));
-    CK(fill_u32(S.n, (u64)S.V * S.K, 0, st));
-    HCK(hipEventRecord(S.ev[2], st));
-    CK(launch_sweep(sweep_params(sg, 0), sg, st));
-    HCK(hipEventRecord(S.ev[3], st));
-    CK(launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, S.n, S.K, st));
+    for (u32 it = 0; it < n_iters; ++it) {
+      HCK(hipGraphLaunch(S.graph, st));
+      S.iter += 1;
+    }
     HCK(hipEventRecord(S.ev[4], st));
-    if (stage_ms) {
-      HCK(hipEventSynchronize(S.ev[4]));
-      for (int s = 0; s < 4; ++s) {
-        float ms = 0;
-        HCK(hipEventElapsedTime(&ms, S.ev[s], S.ev[s + 1]));
-        acc[s] += ms;
-      }
+    HCK(hipEventSynchronize(S.ev[4]));
+    float ms = 0;
+    HCK(hipEventElapsedTime(&ms, S.ev[0], S.ev[4]));
+    if (stage_ms) {  // a graph replays the whole iteration: only the total is timed
+      stage_ms[0] = stage_ms[1] = stage_ms[3] = 0.0f;
+      stage_ms[2] = n_iters ? ms / n_iters : 0.0f;
     }
     S.doc_counter = sg->n_docs;
+    return 0;
+  }
+  for (u32 it = 0; it < n_iters; ++it) {
+    CK(resident_iteration(sg, st, stage_ms ? S.ev : nullptr));
+    if (stage_ms) {
+      HCK(hipEventSynchronize(S.ev[4]));
+      for (int k = 0; k < 4; ++k) {
+        float ms = 0;
+        HCK(hipEventElapsedTime(&ms, S.ev[k], S.ev[k + 1]));
+        acc[k] += ms;
+      }
+    }
   }
   HCK(hipStreamSynchronize(st));
   if (stage_ms)
-    for (int s = 0; s < 4; ++s) stage_ms[s] = n_iters ? (float)(acc[s] / n_iters) : 0.0f;
+    for (int k = 0; k < 4; ++k) stage_ms[k] = n_iters ? (float)(acc[k] / n_iters) : 0.0f;
+  return 0;
+}
+
+extern "C" int gpulda_corpus_schedule(int schedule, uint32_t waves_per_cu, uint32_t block_min_len) {
+  CK(require_ready());
+  if (schedule < kSchedBuckets || schedule > kSchedPersistentBlock) return fail("unknown schedule %d", schedule);
+  S.schedule = schedule;
+  S.waves_per_cu = waves_per_cu ? waves_per_cu : 16;
+  S.block_min_len = block_min_len;
+  drop_graph();
+  return 0;
+}
+
+extern "C" int gpulda_corpus_use_graph(int enabled) {
+  CK(require_ready());
+  S.use_graph = enabled != 0;
+  drop_graph();
+  return 0;
+}
+
+// Times one sweep of the resident corpus under each candidate schedule (z is saved before and
+// restored after every trial, so the chain does not move), then applies the fastest.
+// table: 4 floats per candidate {schedule, waves_per_cu, block_min_len, ms}.
+extern "C" int gpulda_corpus_autotune(float* table, uint32_t max_candidates, uint32_t* n_candidates, int* chosen) {
+  CK(require_ready());
+  if (!S.resident) return fail("no resident corpus");
+  if (S.iter == 0) return fail("autotune needs a Phi step first (iterate once)");
+  Segment* sg = S.resident;
+  struct Cand {
+    int sched;
+    u32 waves;
+    u32 minlen;
+  };
+  std::vector<Cand> cands = {{kSchedBuckets, 0, 0}};
+  for (u32 w : {4u, 8u, 16u, 32u}) cands.push_back({kSchedPersistent, w, 0});
+  const u32 longest = sg->lens_desc.empty() ? 0 : sg->lens_desc[0];
+  for (u32 m : {2048u, 8192u})
+    if (longest >= m) cands.push_back({kSchedPersistentBlock, 16, m});
+  u16* saved = nullptr;
+  CK(dalloc(&saved, sg->n_tokens));
+  unsigned long long vis0 = 0;
+  HCK(hipStreamSynchronize(S.stream));
+  HCK(hipMemcpy(&vis0, S.visited, 8, hipMemcpyDeviceToHost));
+  HCK(hipMemcpyAsync(saved, sg->z, sg->n_tokens * 2, hipMemcpyDeviceToDevice, S.stream));
+  const int old_sched = S.schedule;
+  const u32 old_waves = S.waves_per_cu, old_min = S.block_min_len;
+  float best = 1e30f;
+  int best_i = 0;
+  for (size_t i = 0; i < cands.size(); ++i) {
+    S.schedule = cands[i].sched;
+    S.waves_per_cu = cands[i].waves ? cands[i].waves : 16;
+    S.block_min_len = cands[i].minlen;
+    HCK(hipEventRecord(S.ev[0], S.stream));
+    int rc = launch_sweep_schedule(sweep_params(sg, 0), sg, S.stream);
+    HCK(hipEventRecord(S.ev[1], S.stream));
+    HCK(hipMemcpyAsync(sg->z, saved, sg->n_tokens * 2, hipMemcpyDeviceToDevice, S.stream));
+    HCK(hipStreamSynchronize(S.stream));
+    if (rc) {
+      dfree(saved);
+      return rc;
+    }
+    float ms = 0;
+    HCK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
+    if (table && i < max_candidates) {
+      table[4 * i + 0] = (float)cands[i].sched;
+      table[4 * i + 1] = (float)cands[i].waves;
+      table[4 * i + 2] = (float)cands[i].minlen;
+      table[4 * i + 3] = ms;
+    }
+    if (ms < best) {
+      best = ms;
+      best_i = (int)i;
+    }
+  }
+  dfree(saved);
+  HCK(hipMemcpy(S.visited, &vis0, 8, hipMemcpyHostToDevice));
+  (void)old_sched;
+  (void)old_waves;
+  (void)old_min;
+  S.schedule = cands[best_i].sched;
+  S.waves_per_cu = cands[best_i].waves ? cands[best_i].waves : 16;
+  S.block_min_len = cands[best_i].minlen;
+  drop_graph();
+  if (n_candidates) *n_candidates = (u32)cands.size();
+  if (chosen) *chosen = best_i;
   return 0;
 }
 
@@ -908,8 +1142,261 @@ extern "C" int gpulda_calibrate_copy(uint64_t bytes, uint32_t reps, float* ms_pe
 
 extern "C" int gpulda_corpus_release(void) {
   HCK(hipDeviceSynchronize());
+  drop_graph();
   free_segment(S.resident);
   S.resident = nullptr;
+  return 0;
+}
+
+// ---- fold-in server ----------------------------------------------------------------------------
+// Continuous batching against the resident model (phi, sigma_a and alias tables stay in HBM):
+//   submit   admits a request if a slot and a span of the token arena (bounded by the HBM budget)
+//            are free, copies its words to pinned staging; otherwise the caller retries;
+//   flush    launches every admitted, not yet launched request as one batch on the next of
+//            n_streams streams (batches on different streams overlap on the device);
+//   poll     reports requests whose completion flag the kernel raised (pinned, host-visible);
+//   result   returns a finished request's topics and frees its slot and span.
+// A request's randomness is keyed by its request id, so its result is the same whatever it is
+// batched with.
+namespace {
+struct ServeState {
+  bool open = false;
+  u32 sweeps = 50;
+  u32 n_slots = 0;
+  u64 arena = 0;            // tokens
+  u32* d_w = nullptr;       // arena word ids (device)
+  u16* d_z = nullptr;       // arena topics (device)
+  u32* h_w = nullptr;       // pinned staging of the arena word ids
+  u16* h_z = nullptr;       // pinned, host-visible final topics
+  u64* h_off = nullptr;     // pinned slot -> span offset
+  u32* h_len = nullptr;     // pinned slot -> span length
+  u32* h_rid = nullptr;     // pinned slot -> request id
+  u32* h_done = nullptr;    // pinned slot -> completion flag
+  u32* h_batch = nullptr;   // pinned batch lists (n_slots entries, used as a ring)
+  u32 batch_head = 0;
+  std::vector<u32> free_slots;
+  std::vector<std::pair<u64, u64>> free_spans;  // (offset, length), sorted by offset
+  std::vector<u32> pending;   // admitted, not launched
+  std::vector<u32> inflight;  // launched, not reported
+  std::vector<hipStream_t> streams;
+  u32 next_stream = 0;
+  u64 lds = 0;
+};
+ServeState SV;
+
+int serve_alloc_span(u64 len, u64* off) {
+  for (size_t i = 0; i < SV.free_spans.size(); ++i) {
+    if (SV.free_spans[i].second >= len) {
+      *off = SV.free_spans[i].first;
+      SV.free_spans[i].first += len;
+      SV.free_spans[i].second -= len;
+      if (SV.free_spans[i].second == 0) SV.free_spans.erase(SV.free_spans.begin() + (long)i);
+      return 0;
+    }
+  }
+  return 1;
+}
+
+void serve_free_span(u64 off, u64 len) {
+  if (len == 0) return;
+  auto it = std::lower_bound(SV.free_spans.begin(), SV.free_spans.end(), std::make_pair(off, (u64)0));
+  it = SV.free_spans.insert(it, {off, len});
+  // coalesce with the neighbours
+  size_t i = (size_t)(it - SV.free_spans.begin());
+  if (i + 1 < SV.free_spans.size() && SV.free_spans[i].first + SV.free_spans[i].second == SV.free_spans[i + 1].first) {
+    SV.free_spans[i].second += SV.free_spans[i + 1].second;
+    SV.free_spans.erase(SV.free_spans.begin() + (long)i + 1);
+  }
+  if (i > 0 && SV.free_spans[i - 1].first + SV.free_spans[i - 1].second == SV.free_spans[i].first) {
+    SV.free_spans[i - 1].second += SV.free_spans[i].second;
+    SV.free_spans.erase(SV.free_spans.begin() + (long)i);
+  }
+}
+
+template <class T>
+int host_alloc(T** p, u64 n) {
+  *p = nullptr;
+  HCK(hipHostMalloc((void**)p, std::max<u64>(n, 1) * sizeof(T), hipHostMallocMapped));
+  return 0;
+}
+template <class T>
+T* dev_view(T* h) {
+  void* d = nullptr;
+  if (hipHostGetDevicePointer(&d, (void*)h, 0) != hipSuccess) return nullptr;
+  return (T*)d;
+}
+
+void serve_release() {
+  for (hipStream_t st : SV.streams) (void)hipStreamSynchronize(st);
+  for (hipStream_t st : SV.streams) (void)hipStreamDestroy(st);
+  SV.streams.clear();
+  dfree(SV.d_w);
+  dfree(SV.d_z);
+  for (void* h : {(void*)SV.h_w, (void*)SV.h_z, (void*)SV.h_off, (void*)SV.h_len, (void*)SV.h_rid, (void*)SV.h_done,
+                  (void*)SV.h_batch})
+    if (h) (void)hipHostFree(h);
+  SV = ServeState();
+}
+
+SweepParams serve_params() {
+  SweepParams P;
+  P.w = SV.d_w;
+  P.z = SV.d_z;
+  P.doc_off = nullptr;
+  P.doc_order = nullptr;
+  P.n_docs = SV.n_slots;
+  P.doc_base = 0;
+  P.iter = 0;
+  P.iter_ptr = nullptr;
+  P.k0 = (u32)S.seed;
+  P.k1 = (u32)(S.seed >> 32);
+  P.K = S.K;
+  P.phi = S.phi;
+  P.sigma_a = S.sigma_a;
+  P.alias = S.alias;
+  P.visited = nullptr;
+  return P;
+}
+}  // namespace
+
+extern "C" int gpulda_serve_open(uint64_t hbm_budget_bytes, uint32_t sweeps, uint32_t n_streams, uint32_t max_slots) {
+  CK(require_ready());
+  if (S.iter == 0) return fail("serve_open needs a trained model (run at least one Phi step)");
+  if (SV.open) serve_release();
+  SV.sweeps = sweeps;
+  SV.n_slots = max_slots ? max_slots : 65536;
+  SV.arena = std::max<u64>(hbm_budget_bytes / (sizeof(u32) + sizeof(u16)), 1);
+  SV.lds = ((u64)map_capacity(S.K, S.K) + (S.K + 31) / 32) * 4;
+  if (SV.lds > (u64)max_lds_per_block()) return fail("doc-topic map exceeds LDS");
+  int rc = 0;
+  if ((rc = dalloc(&SV.d_w, SV.arena)) || (rc = dalloc(&SV.d_z, SV.arena)) || (rc = host_alloc(&SV.h_w, SV.arena)) ||
+      (rc = host_alloc(&SV.h_z, SV.arena)) || (rc = host_alloc(&SV.h_off, SV.n_slots)) ||
+      (rc = host_alloc(&SV.h_len, SV.n_slots)) || (rc = host_alloc(&SV.h_rid, SV.n_slots)) ||
+      (rc = host_alloc(&SV.h_done, SV.n_slots)) || (rc = host_alloc(&SV.h_batch, 2ull * SV.n_slots))) {
+    serve_release();
+    return rc;
+  }
+  for (u32 i = 0; i < std::max<u32>(1, n_streams); ++i) {
+    hipStream_t st;
+    HCK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    SV.streams.push_back(st);
+  }
+  SV.free_slots.resize(SV.n_slots);
+  for (u32 i = 0; i < SV.n_slots; ++i) SV.free_slots[i] = SV.n_slots - 1 - i;
+  SV.free_spans = {{0, SV.arena}};
+  memset(SV.h_done, 0, SV.n_slots * sizeof(u32));
+  SV.open = true;
+  return 0;
+}
+
+// Returns 0 when admitted (*slot set), 1 when the request must wait (no slot or arena space).
+extern "C" int gpulda_serve_submit(const uint32_t* w, uint32_t L, uint32_t request_id, uint32_t* slot) {
+  if (!SV.open) return fail("server not open");
+  if (L == 0 || L > kMaxDocLen) return fail("request length %u outside 1..%u", L, kMaxDocLen);
+  if (SV.free_slots.empty()) return 1;
+  u64 off = 0;
+  if (serve_alloc_span(L, &off)) return 1;
+  for (u32 i = 0; i < L; ++i)
+    if (w[i] >= S.V) {
+      serve_free_span(off, L);
+      return fail("request word id %u >= V", w[i]);
+    }
+  u32 s = SV.free_slots.back();
+  SV.free_slots.pop_back();
+  memcpy(SV.h_w + off, w, (u64)L * sizeof(u32));
+  SV.h_off[s] = off;
+  SV.h_len[s] = L;
+  SV.h_rid[s] = request_id;
+  __atomic_store_n(&SV.h_done[s], 0u, __ATOMIC_RELEASE);
+  SV.pending.push_back(s);
+  *slot = s;
+  return 0;
+}
+
+extern "C" int gpulda_serve_flush(void) {
+  if (!SV.open) return fail("server not open");
+  if (SV.pending.empty()) return 0;
+  hipStream_t st = SV.streams[SV.next_stream];
+  SV.next_stream = (SV.next_stream + 1) % (u32)SV.streams.size();
+  const u32 nb = (u32)SV.pending.size();
+  // ring of 2 * n_slots list entries: a wrap can only reuse entries of requests already finished,
+  // because at most n_slots requests are admitted at a time
+  if (SV.batch_head + nb > 2 * SV.n_slots) SV.batch_head = 0;
+  u32* list = SV.h_batch + SV.batch_head;
+  SV.batch_head += nb;
+  for (u32 i = 0; i < nb; ++i) {
+    const u32 sl = SV.pending[i];
+    list[i] = sl;
+    HCK(hipMemcpyAsync(SV.d_w + SV.h_off[sl], SV.h_w + SV.h_off[sl], (u64)SV.h_len[sl] * sizeof(u32),
+                       hipMemcpyHostToDevice, st));
+  }
+  FoldParams F;
+  F.slots = dev_view(list);
+  F.n_slots = SV.n_slots;
+  F.span_off = dev_view(SV.h_off);
+  F.span_len = dev_view(SV.h_len);
+  F.req_id = dev_view(SV.h_rid);
+  F.sweeps = SV.sweeps;
+  F.z_out = dev_view(SV.h_z);
+  F.done = dev_view(SV.h_done);
+  F.done_value = 1;
+  if (!F.slots || !F.span_off || !F.span_len || !F.req_id || !F.z_out || !F.done)
+    return fail("host-mapped pointers unavailable");
+  k_foldin<8><<<nb, 64, SV.lds, st>>>(serve_params(), F);
+  HCK(hipGetLastError());
+  SV.inflight.insert(SV.inflight.end(), SV.pending.begin(), SV.pending.end());
+  SV.pending.clear();
+  return 0;
+}
+
+// Slots whose fold-in finished since the last poll (at most max).
+extern "C" int gpulda_serve_poll(uint32_t* done_slots, uint32_t max, uint32_t* n_done) {
+  if (!SV.open) return fail("server not open");
+  u32 n = 0;
+  for (size_t i = 0; i < SV.inflight.size() && n < max;) {
+    const u32 sl = SV.inflight[i];
+    if (__atomic_load_n(&SV.h_done[sl], __ATOMIC_ACQUIRE)) {
+      done_slots[n++] = sl;
+      SV.inflight[i] = SV.inflight.back();
+      SV.inflight.pop_back();
+    } else {
+      ++i;
+    }
+  }
+  *n_done = n;
+  return 0;
+}
+
+// Copies a finished request's topics (u32) and frees its slot.
+extern "C" int gpulda_serve_result(uint32_t slot, uint32_t* z, uint32_t* request_id) {
+  if (!SV.open) return fail("server not open");
+  if (slot >= SV.n_slots || !__atomic_load_n(&SV.h_done[slot], __ATOMIC_ACQUIRE))
+    return fail("slot %u has no finished request", slot);
+  const u64 off = SV.h_off[slot];
+  const u32 L = SV.h_len[slot];
+  if (z)
+    for (u32 i = 0; i < L; ++i) z[i] = SV.h_z[off + i];
+  if (request_id) *request_id = SV.h_rid[slot];
+  __atomic_store_n(&SV.h_done[slot], 0u, __ATOMIC_RELEASE);
+  serve_free_span(off, L);
+  SV.free_slots.push_back(slot);
+  return 0;
+}
+
+extern "C" int gpulda_serve_stats(uint32_t* pending, uint32_t* inflight, uint64_t* free_tokens) {
+  if (!SV.open) return fail("server not open");
+  if (pending) *pending = (u32)SV.pending.size();
+  if (inflight) *inflight = (u32)SV.inflight.size();
+  if (free_tokens) {
+    u64 f = 0;
+    for (auto& sp : SV.free_spans) f += sp.second;
+    *free_tokens = f;
+  }
+  return 0;
+}
+
+extern "C" int gpulda_serve_close(void) {
+  if (SV.open) serve_release();
   return 0;
 }
 
@@ -1055,7 +1542,7 @@ extern "C" int gpulda_stage_phi_draw(const uint32_t* n, uint32_t V, uint32_t K, 
   CK(sc.upload(&dn, n, VK));
   CK(sc.alloc(&dphi, VK));
   CK(sc.alloc(&dts, K));
-  CK(launch_phi_draw(dn, dphi, dts, V, K, beta, iter, seed, tables, 0));
+  CK(launch_phi_draw(dn, dphi, dts, V, K, beta, iter, nullptr, seed, tables, 0));
   HCK(hipDeviceSynchronize());
   HCK(hipMemcpy(phi_tilde, dphi, VK * 4, hipMemcpyDeviceToHost));
   HCK(hipMemcpy(topic_sum, dts, (u64)K * 8, hipMemcpyDeviceToHost));
@@ -1113,6 +1600,7 @@ extern "C" int gpulda_stage_sweep(uint32_t* z, const uint32_t* w, const uint64_t
   P.n_docs = n_docs;
   P.doc_base = doc_base;
   P.iter = iter;
+  P.iter_ptr = nullptr;
   P.k0 = (u32)seed;
   P.k1 = (u32)(seed >> 32);
   P.K = K;
@@ -1167,4 +1655,78 @@ extern "C" int gpulda_stage_count(const uint32_t* w, const uint32_t* z, uint64_t
   if (!r2 && hipMemcpy(n_out, dn, VK * 4, hipMemcpyDeviceToHost) != hipSuccess) r2 = fail("copy failed");
   free_segment(sg);
   return r2;
+}
+
+extern "C" int gpulda_stage_foldin(const uint32_t* w, const uint64_t* doc_off, uint32_t n_req, const uint32_t* req_ids,
+                                   uint32_t sweeps, uint64_t seed, uint32_t K, uint32_t V, const float* phi,
+                                   const float* sigma_a, const float* prob, const uint32_t* alias, uint32_t* z_out,
+                                   uint64_t* visited) {
+  if (K == 0 || K > kMaxTopics) return fail("bad K");
+  const u64 N = doc_off[n_req];
+  std::vector<u32> lens(n_req);
+  for (u32 r = 0; r < n_req; ++r) {
+    u64 L = doc_off[r + 1] - doc_off[r];
+    if (L > kMaxDocLen) return fail("request %u longer than %u tokens", r, kMaxDocLen);
+    lens[r] = (u32)L;
+  }
+  for (u64 i = 0; i < N; ++i)
+    if (w[i] >= V) return fail("word id >= V");
+  const u64 VK = (u64)V * K;
+  Scratch sc;
+  u32 *dw, *dlen, *drid;
+  u16* dz;
+  u64* doff;
+  f32 *dphi, *dsig;
+  AliasSlot* dal;
+  unsigned long long* dvis;
+  CK(sc.upload(&dw, w, N));
+  CK(sc.alloc(&dz, N));
+  CK(sc.upload(&doff, reinterpret_cast<const u64*>(doc_off), n_req));
+  CK(sc.upload(&dlen, lens.data(), n_req));
+  CK(sc.upload(&drid, req_ids, n_req));
+  CK(sc.upload(&dphi, phi, VK));
+  CK(sc.upload(&dsig, sigma_a, V));
+  CK(pack_alias(sc, prob, alias, VK, &dal));
+  CK(sc.alloc(&dvis, 1));
+  CK(fill_u64(dvis, 1, 0, 0));
+  SweepParams P;
+  P.w = dw;
+  P.z = dz;
+  P.doc_off = nullptr;
+  P.doc_order = nullptr;
+  P.n_docs = n_req;
+  P.doc_base = 0;
+  P.iter = 0;
+  P.iter_ptr = nullptr;
+  P.k0 = (u32)seed;
+  P.k1 = (u32)(seed >> 32);
+  P.K = K;
+  P.phi = dphi;
+  P.sigma_a = dsig;
+  P.alias = dal;
+  P.visited = dvis;
+  FoldParams F;
+  F.slots = nullptr;
+  F.n_slots = n_req;
+  F.span_off = doff;
+  F.span_len = dlen;
+  F.req_id = drid;
+  F.sweeps = sweeps;
+  F.z_out = nullptr;
+  F.done = nullptr;
+  F.done_value = 0;
+  u32 maxL = 0;
+  for (u32 L : lens) maxL = std::max(maxL, L);
+  const u64 lds = ((u64)map_capacity(maxL, K) + (K + 31) / 32) * 4;
+  if (lds > (u64)max_lds_per_block()) return fail("doc-topic map exceeds LDS");
+  if (n_req) {
+    k_foldin<8><<<n_req, 64, lds>>>(P, F);
+    HCK(hipGetLastError());
+  }
+  std::vector<u16> hz(N);
+  HCK(hipDeviceSynchronize());
+  if (N) HCK(hipMemcpy(hz.data(), dz, N * 2, hipMemcpyDeviceToHost));
+  for (u64 i = 0; i < N; ++i) z_out[i] = hz[i];
+  if (visited) HCK(hipMemcpy(visited, dvis, 8, hipMemcpyDeviceToHost));
+  return 0;
 }

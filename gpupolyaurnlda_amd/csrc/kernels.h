@@ -20,7 +20,8 @@ struct SweepParams {
   const u32* doc_order;  // optional launch order (block -> doc); null = identity
   u32 n_docs;
   u32 doc_base;          // global id of document 0 (Philox counter word 1)
-  u32 iter;              // iteration (Philox counter word 2)
+  u32 iter;              // iteration (Philox counter word 2) ...
+  const u32* iter_ptr;   // ... or read from device memory when not null (replayable graphs)
   u32 k0, k1;            // Philox key = seed
   u32 K;
   const f32* phi;        // V x K word-major
@@ -29,19 +30,40 @@ struct SweepParams {
   unsigned long long* visited;  // optional: += map entries visited by the sparse sums
 };
 
+// Fold-in batch: request slots of a token arena.
+struct FoldParams {
+  const u32* slots;     // batch -> slot (null: block b is slot b)
+  u32 n_slots;
+  const u64* span_off;  // slot -> first token in the arena
+  const u32* span_len;  // slot -> tokens
+  const u32* req_id;    // slot -> request id (Philox counter word 1)
+  u32 sweeps;
+  u16* z_out;           // optional host-visible copy of the final topics (arena offsets)
+  u32* done;            // optional host-visible completion flags (slot)
+  u32 done_value;
+};
+
+template <int LMAX>
+__global__ void k_foldin(SweepParams P, FoldParams F);
+
 __global__ void k_fill_u32(u32* p, u64 n, u32 v);
 __global__ void k_fill_u64(unsigned long long* p, u64 n, unsigned long long v);
 __global__ void k_z_narrow(const u32* in, u16* out, u64 n, u32 K, u32* bad);
 __global__ void k_z_widen(const u16* in, u32* out, u64 n);
 __global__ void k_z_init(u16* z, const u64* doc_off, u32 n_docs, u32 doc_base, u32 K, u32 k0, u32 k1);
 __global__ void k_phi_draw(const u32* n, f32* phi, unsigned long long* topic_sum, u32 V, u32 K, f32 beta, u32 iter,
-                           u32 k0, u32 k1, const AliasSlot* tables, u32 rows_per_block);
+                           const u32* iter_ptr, u32 k0, u32 k1, const AliasSlot* tables, u32 rows_per_block);
+__global__ void k_bump_u32(u32* p);
 __global__ void k_init_counts(const u32* C, u32* n, u32 V, u32 K, f32 beta, u32 k0, u32 k1, const AliasSlot* tables,
                               u32 rows_per_block);
 __global__ void k_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
                              AliasSlot* out, u32 rows_per_block_waves);
 template <int LMAX>
 __global__ void k_sweep_lines(SweepParams P);
+template <int LMAX>
+__global__ void k_sweep_persistent(SweepParams P, u32* head, u32 n_items);
+// Block-per-document sweep (4 wavefronts per document); head == null: block b takes item b.
+__global__ void k_sweep_block(SweepParams P, u32* head, u32 n_items);
 // Register-resident line counts of the specialised sweep kernels (0 = any size).
 constexpr int kSweepBuckets[] = {1, 2, 4, 8, 16, 32};
 // Kernel family member for a map of `lines` 64-slot lines.

@@ -25,6 +25,11 @@ __global__ void k_fill_u32(u32* p, u64 n, u32 v) {
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// Iteration counter kept on the device, so that a captured iteration can be replayed.
+__global__ void k_bump_u32(u32* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) p[0] += 1u;
+}
+
 __global__ void k_fill_u64(unsigned long long* p, u64 n, unsigned long long v) {
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -63,10 +68,11 @@ __global__ void k_z_init(u16* z, const u64* doc_off, u32 n_docs, u32 doc_base, u
 // Phi step.  Thread = topic column k, block.y = a tile of words; topic sums are exact u64.
 // ---------------------------------------------------------------------------------------------
 __global__ void k_phi_draw(const u32* __restrict__ n, f32* __restrict__ phi, unsigned long long* topic_sum,
-                           u32 V, u32 K, f32 beta, u32 iter, u32 k0, u32 k1,
+                           u32 V, u32 K, f32 beta, u32 iter_arg, const u32* iter_ptr, u32 k0, u32 k1,
                            const AliasSlot* __restrict__ tables, u32 rows_per_block) {
   u32 k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K) return;
+  const u32 iter = iter_ptr ? *iter_ptr : iter_arg;
   u32 v0 = blockIdx.y * rows_per_block;
   u32 v1 = v0 + rows_per_block < V ? v0 + rows_per_block : V;
   u64 acc = 0;
@@ -327,31 +333,123 @@ __device__ __forceinline__ f32 line_total_before(f32 cw0, f32 cw1, u32 jsel) {
   return jsel - 1 < 64 ? readlane_f32(cw0, jsel - 1) : readlane_f32(cw1, jsel - 65);
 }
 
-// One document.  LMAX > 0: the map has at most LMAX lines and the whole line set is held in
-// registers (all phi gathers of a token are issued before the scans).  LMAX == 0: any size, one
-// line at a time.
-template <int LMAX>
-__device__ __forceinline__ void sweep_doc(const SweepParams& P, u32 doc, u32* T) {
-  const u32 lane = lane_id();
-  const u64 b = P.doc_off[doc];
-  const u32 L = (u32)(P.doc_off[doc + 1] - b);
-  const u32 K = P.K;
-  const u32 cap_max = map_capacity(L, K);  // never exceeded; LDS holds cap_max slots + a K-bit map
-  const u32 kmax = L < K ? L : K;
-  const u32 gdoc = P.doc_base + doc;
+// Per-document state of a sweep.
+struct DocState {
+  u64 b;          // first token
+  u32 L;          // tokens
+  u32 cap_max;    // capacity for min(L, K) topics: the map never grows beyond it
+  u32 kmax;       // min(L, K)
+  u32 cap;        // current capacity
+  u32 occupied;   // distinct topics in the map
+  u32 gdoc;       // global document id (Philox counter word 1)
+};
 
-  // Capacity rule: start sized for the n0 distinct topics; grow (rebuild) when a new topic would
-  // exceed the load limit.  The reference starts from the K_d hint (hashmap.cuh:98).
-  u32 occupied = distinct_topics(T + cap_max, K, P.z, b, L);
-  u32 cap = map_capacity(occupied, K);
-  build_doc_map(T, cap, P.z, b, L);
-  u64 visited = 0;
-  for (u32 c0 = 0; c0 < L; c0 += 64) {
+// Capacity rule: start sized for the n0 distinct topics; grow (rebuild) when a new topic would
+// exceed the load limit.  The reference starts from the K_d hint (hashmap.cuh:98).
+__device__ __forceinline__ DocState open_span(const SweepParams& P, u64 b, u32 L, u32 gdoc, u32* T) {
+  DocState ds;
+  ds.b = b;
+  ds.L = L;
+  ds.cap_max = map_capacity(ds.L, P.K);
+  ds.kmax = ds.L < P.K ? ds.L : P.K;
+  ds.gdoc = gdoc;
+  ds.occupied = distinct_topics(T + ds.cap_max, P.K, P.z, ds.b, ds.L);
+  ds.cap = map_capacity(ds.occupied, P.K);
+  build_doc_map(T, ds.cap, P.z, ds.b, ds.L);
+  return ds;
+}
+
+__device__ __forceinline__ DocState open_doc(const SweepParams& P, u32 doc, u32* T) {
+  const u64 b = P.doc_off[doc];
+  return open_span(P, b, (u32)(P.doc_off[doc + 1] - b), P.doc_base + doc, T);
+}
+
+// Sparse pick with every line of the map held in registers (NL >= lines): all phi gathers of the
+// token are issued before the scans.
+template <int NL>
+__device__ __forceinline__ u32 draw_topic_regs(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa,
+                                               u32 rx, u32 rw, u32 dense, u32 zt) {
+  const u32 lane = lane_id();
+  f32 carry = 0.0f;
+  f32 cw0 = 0.0f;  // running total after line j, held by lane j
+  u32 e[NL];
+  f32 ph[NL];
+#pragma unroll
+  for (int j = 0; j < NL; ++j) e[j] = (u32)j < lines ? T[j * 64 + lane] : kMapEmpty;
+#pragma unroll
+  for (int j = 0; j < NL; ++j) ph[j] = e[j] != kMapEmpty ? prow[map_key(e[j])] : 0.0f;
+#pragma unroll
+  for (int j = 0; j < NL; ++j) {
+    if ((u32)j < lines) {
+      f32 v = e[j] != kMapEmpty ? (f32)map_count(e[j]) * ph[j] : 0.0f;
+      carry = carry + readlane_f32(wave_scan_f32(v), 63);
+      if (lane == (u32)j) cw0 = carry;
+    }
+  }
+  const f32 sb = carry;
+  if (!(u24(rx) * (sa + sb) > sa)) return dense;
+  const f32 tgt = u24(rw) * sb;
+  const u32 jsel = pick_line(cw0, 0.0f, lines, tgt);
+  u32 es = kMapEmpty;
+  f32 ps = 0.0f;
+#pragma unroll
+  for (int j = 0; j < NL; ++j)
+    if ((u32)j == jsel) {
+      es = e[j];
+      ps = ph[j];
+    }
+  const f32 v = es != kMapEmpty ? (f32)map_count(es) * ps : 0.0f;
+  return pick_in_line(es, v, line_total_before(cw0, 0.0f, jsel), tgt, zt);
+}
+
+// Sparse pick for any number of lines, one line at a time.
+__device__ __forceinline__ u32 draw_topic_loop(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa,
+                                               u32 rx, u32 rw, u32 dense, u32 zt) {
+  const u32 lane = lane_id();
+  f32 carry = 0.0f;
+  f32 cw0 = 0.0f, cw1 = 0.0f;  // running total after line j, held by lane j (j < 128)
+  for (u32 j = 0; j < lines; ++j) {
+    f32 sc = wave_scan_f32(slot_mass(T[j * 64 + lane], prow));
+    carry = carry + readlane_f32(sc, 63);
+    if (j < 64) {
+      if (lane == j) cw0 = carry;
+    } else if (lane == j - 64) {
+      cw1 = carry;
+    }
+  }
+  const f32 sb = carry;
+  if (!(u24(rx) * (sa + sb) > sa)) return dense;
+  const f32 tgt = u24(rw) * sb;
+  const u32 jsel = pick_line(cw0, cw1, lines, tgt);
+  const u32 e = T[jsel * 64 + lane];
+  return pick_in_line(e, slot_mass(e, prow), line_total_before(cw0, cw1, jsel), tgt, zt);
+}
+
+// Sparse/dense pick for one token (sigma_a > 0).  The register variant is chosen by the map's
+// current line count, up to LMAX lines (LMAX caps the registers of a kernel); larger maps go
+// line by line.  All variants evaluate the same operations in the same order.
+template <int LMAX>
+__device__ __forceinline__ u32 draw_topic(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa, u32 rx,
+                                          u32 rw, u32 dense, u32 zt) {
+  if (LMAX >= 1 && lines <= 1) return draw_topic_regs<1>(T, lines, prow, sa, rx, rw, dense, zt);
+  if (LMAX >= 2 && lines <= 2) return draw_topic_regs<2>(T, lines, prow, sa, rx, rw, dense, zt);
+  if (LMAX >= 4 && lines <= 4) return draw_topic_regs<4>(T, lines, prow, sa, rx, rw, dense, zt);
+  if (LMAX >= 8 && lines <= 8) return draw_topic_regs<8>(T, lines, prow, sa, rx, rw, dense, zt);
+  return draw_topic_loop(T, lines, prow, sa, rx, rw, dense, zt);
+}
+
+// One pass over a document's tokens with Philox counters (token, gdoc, iter, tag).
+template <int LMAX>
+__device__ __forceinline__ void sweep_pass(const SweepParams& P, u32* T, DocState& ds, u32 iter, u32 tag,
+                                           u64& visited) {
+  const u32 lane = lane_id();
+  const u32 K = P.K;
+  for (u32 c0 = 0; c0 < ds.L; c0 += 64) {
     const u32 i = c0 + lane;
-    const bool valid = i < L;
-    const u32 ww = valid ? P.w[b + i] : 0;
-    const u32 zz = valid ? (u32)P.z[b + i] : 0;
-    const u32x4 r = philox_tagged(i, gdoc, P.iter, kTagSweep, 0, P.k0, P.k1);
+    const bool valid = i < ds.L;
+    const u32 ww = valid ? P.w[ds.b + i] : 0;
+    const u32 zz = valid ? (u32)P.z[ds.b + i] : 0;
+    const u32x4 r = philox_tagged(i, ds.gdoc, iter, tag, 0, P.k0, P.k1);
     // per-token values that do not depend on the map: prefetched for the whole chunk
     const f32 sa_l = valid ? P.sigma_a[ww] : 0.0f;
     const u32 slot_l = bounded(r.y, K);
@@ -361,88 +459,44 @@ __device__ __forceinline__ void sweep_doc(const SweepParams& P, u32 doc, u32* T)
     if (valid) al_l = P.alias[(u64)ww * K + slot_l];
     const u32 dense_l = u24(r.z) < al_l.prob ? slot_l : al_l.alias;
     u32 znew_lane = zz;
-    const u32 nt = L - c0 < 64 ? L - c0 : 64;
+    const u32 nt = ds.L - c0 < 64 ? ds.L - c0 : 64;
     for (u32 t = 0; t < nt; ++t) {
       const u32 wt = readlane_u32(ww, t);
       const u32 zt = readlane_u32(zz, t);
-      occupied -= map_dec(T, cap, zt);
+      ds.occupied -= map_dec(T, ds.cap, zt);
       const f32 sa = readlane_f32(sa_l, t);
-      const u32 lines = cap / kMapLine;
       u32 znew = zt;
       if (sa > 0.0f) {
-        const f32* prow = P.phi + (u64)wt * K;
         wave_sync();
-        f32 carry = 0.0f;
-        f32 cw0 = 0.0f, cw1 = 0.0f;
-        if constexpr (LMAX > 0) {
-          u32 e[LMAX];
-          f32 ph[LMAX];
-#pragma unroll
-          for (int j = 0; j < LMAX; ++j) e[j] = (u32)j < lines ? T[j * 64 + lane] : kMapEmpty;
-#pragma unroll
-          for (int j = 0; j < LMAX; ++j) ph[j] = e[j] != kMapEmpty ? prow[map_key(e[j])] : 0.0f;
-#pragma unroll
-          for (int j = 0; j < LMAX; ++j) {
-            if ((u32)j < lines) {
-              f32 v = e[j] != kMapEmpty ? (f32)map_count(e[j]) * ph[j] : 0.0f;
-              carry = carry + readlane_f32(wave_scan_f32(v), 63);
-              if (lane == (u32)j) cw0 = carry;
-            }
-          }
-          visited += occupied;
-          const f32 sb = carry;
-          if (u24(readlane_u32(r.x, t)) * (sa + sb) > sa) {
-            const f32 tgt = u24(readlane_u32(r.w, t)) * sb;
-            const u32 jsel = pick_line(cw0, cw1, lines, tgt);
-            u32 es = kMapEmpty;
-            f32 ps = 0.0f;
-#pragma unroll
-            for (int j = 0; j < LMAX; ++j)
-              if ((u32)j == jsel) {
-                es = e[j];
-                ps = ph[j];
-              }
-            const f32 v = es != kMapEmpty ? (f32)map_count(es) * ps : 0.0f;
-            znew = pick_in_line(es, v, line_total_before(cw0, cw1, jsel), tgt, zt);
-          } else {
-            znew = readlane_u32(dense_l, t);
-          }
-        } else {
-          for (u32 j = 0; j < lines; ++j) {
-            f32 sc = wave_scan_f32(slot_mass(T[j * 64 + lane], prow));
-            carry = carry + readlane_f32(sc, 63);
-            if (j < 64) {
-              if (lane == j) cw0 = carry;
-            } else if (lane == j - 64) {
-              cw1 = carry;
-            }
-          }
-          visited += occupied;
-          const f32 sb = carry;
-          if (u24(readlane_u32(r.x, t)) * (sa + sb) > sa) {
-            const f32 tgt = u24(readlane_u32(r.w, t)) * sb;
-            const u32 jsel = pick_line(cw0, cw1, lines, tgt);
-            const u32 e = T[jsel * 64 + lane];
-            znew = pick_in_line(e, slot_mass(e, prow), line_total_before(cw0, cw1, jsel), tgt, zt);
-          } else {
-            znew = readlane_u32(dense_l, t);
-          }
-        }
+        visited += ds.occupied;
+        znew = draw_topic<LMAX>(T, ds.cap / kMapLine, P.phi + (u64)wt * K, sa, readlane_u32(r.x, t),
+                                readlane_u32(r.w, t), readlane_u32(dense_l, t), zt);
       }
-      if (!map_probe(T, cap, znew).found) {
-        if (map_capacity(occupied + 1, K) > cap) {
-          const u32 want = 2 * (occupied + 1) < kmax ? 2 * (occupied + 1) : kmax;
-          cap = map_capacity(want, K);
-          regrow_doc_map(T, cap, P.z, b, L, c0, t, znew_lane, zz);
+      MapProbe pz = map_probe(T, ds.cap, znew);
+      if (!pz.found) {
+        if (map_capacity(ds.occupied + 1, K) > ds.cap) {
+          const u32 want = 2 * (ds.occupied + 1) < ds.kmax ? 2 * (ds.occupied + 1) : ds.kmax;
+          ds.cap = map_capacity(want, K);
+          regrow_doc_map(T, ds.cap, P.z, ds.b, ds.L, c0, t, znew_lane, zz);
+          pz = map_probe(T, ds.cap, znew);
         }
-        ++occupied;
+        ++ds.occupied;
       }
-      map_inc(T, cap, znew);
+      map_inc_at(T, ds.cap, znew, pz);
       if (lane == t) znew_lane = znew;
     }
-    if (valid) P.z[b + i] = (u16)znew_lane;
+    if (valid) P.z[ds.b + i] = (u16)znew_lane;
   }
-  if (lane == 0 && P.visited) atomicAdd(P.visited, (unsigned long long)visited);
+}
+
+__device__ __forceinline__ u32 sweep_iteration(const SweepParams& P) { return P.iter_ptr ? *P.iter_ptr : P.iter; }
+
+template <int LMAX>
+__device__ __forceinline__ void sweep_doc(const SweepParams& P, u32 doc, u32* T) {
+  DocState ds = open_doc(P, doc, T);
+  u64 visited = 0;
+  sweep_pass<LMAX>(P, T, ds, sweep_iteration(P), kTagSweep, visited);
+  if (lane_id() == 0 && P.visited) atomicAdd(P.visited, (unsigned long long)visited);
 }
 
 // One wavefront per document, documents taken from P.doc_order (a length bucket) or in order.
@@ -461,6 +515,183 @@ template __global__ void k_sweep_lines<4>(SweepParams);
 template __global__ void k_sweep_lines<8>(SweepParams);
 template __global__ void k_sweep_lines<16>(SweepParams);
 template __global__ void k_sweep_lines<32>(SweepParams);
+
+// Persistent form: a fixed grid of wavefronts takes documents from a queue (P.doc_order, longest
+// first) through an atomic head.  Every wavefront leaves once the head passes n_items, so the
+// grid always drains.  Results are identical to the per-bucket launches: the randomness of a
+// document depends on its global id only.
+template <int LMAX>
+__global__ __launch_bounds__(64) void k_sweep_persistent(SweepParams P, u32* head, u32 n_items) {
+  extern __shared__ u32 sweep_smem[];
+  for (;;) {
+    u32 item = 0;
+    if (lane_id() == 0) item = atomicAdd(head, 1u);
+    item = readlane_u32(item, 0);
+    if (item >= n_items) return;
+    sweep_doc<LMAX>(P, P.doc_order[item], sweep_smem);
+  }
+}
+template __global__ void k_sweep_persistent<8>(SweepParams, u32*, u32);
+
+// ---------------------------------------------------------------------------------------------
+// Block-per-document sweep for long documents: four wavefronts share one map.  Wave 0 owns the
+// map (decrement, growth, increment) and the pick; the line totals of the sparse sum are split
+// across the four waves (line j on wave j % 4) and chained in the canonical order, so the result
+// is bit-identical to the wavefront-per-document sweep.
+// ---------------------------------------------------------------------------------------------
+constexpr u32 kBlockWaves = 4;
+
+__global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u32 n_items) {
+  extern __shared__ u32 sweep_smem[];
+  __shared__ f32 line_tot[128];
+  __shared__ u32 sh_cap;
+  __shared__ u32 sh_item;
+  const u32 lane = lane_id();
+  const u32 wave = threadIdx.x >> 6;
+  const u32 K = P.K;
+  u32* T = sweep_smem;
+  for (;;) {
+    if (threadIdx.x == 0) sh_item = head ? atomicAdd(head, 1u) : (blockIdx.x < n_items ? blockIdx.x : n_items);
+    __syncthreads();
+    const u32 item = sh_item;
+    __syncthreads();
+    if (item >= n_items) return;
+    const u32 doc = P.doc_order ? P.doc_order[item] : item;
+    DocState ds;
+    if (wave == 0) {
+      ds = open_doc(P, doc, T);
+      if (lane == 0) sh_cap = ds.cap;
+    }
+    __syncthreads();
+    const u64 b = P.doc_off[doc];
+    const u32 L = (u32)(P.doc_off[doc + 1] - b);
+    const u32 gdoc = P.doc_base + doc;
+    const u32 iter = sweep_iteration(P);
+    u64 visited = 0;
+    for (u32 c0 = 0; c0 < L; c0 += 64) {
+      const u32 i = c0 + lane;
+      const bool valid = i < L;
+      const u32 ww = valid ? P.w[b + i] : 0;
+      const u32 zz = valid ? (u32)P.z[b + i] : 0;
+      const u32x4 r = philox_tagged(i, gdoc, iter, kTagSweep, 0, P.k0, P.k1);
+      const f32 sa_l = valid ? P.sigma_a[ww] : 0.0f;
+      u32 dense_l = 0;
+      if (wave == 0) {
+        const u32 slot_l = bounded(r.y, K);
+        AliasSlot al_l;
+        al_l.prob = 0.0f;
+        al_l.alias = 0;
+        if (valid) al_l = P.alias[(u64)ww * K + slot_l];
+        dense_l = u24(r.z) < al_l.prob ? slot_l : al_l.alias;
+      }
+      u32 znew_lane = zz;
+      const u32 nt = L - c0 < 64 ? L - c0 : 64;
+      for (u32 t = 0; t < nt; ++t) {
+        const u32 wt = readlane_u32(ww, t);
+        const u32 zt = readlane_u32(zz, t);
+        const f32 sa = readlane_f32(sa_l, t);
+        if (wave == 0) {
+          ds.occupied -= map_dec(T, ds.cap, zt);
+          if (lane == 0) sh_cap = ds.cap;
+        }
+        __syncthreads();
+        u32 znew = zt;
+        if (sa > 0.0f) {
+          const u32 lines = sh_cap / kMapLine;
+          const f32* prow = P.phi + (u64)wt * K;
+          for (u32 j = wave; j < lines; j += kBlockWaves) {
+            f32 sc = wave_scan_f32(slot_mass(T[j * 64 + lane], prow));
+            if (lane == 63) line_tot[j] = sc;
+          }
+          __syncthreads();
+          if (wave == 0) {
+            visited += ds.occupied;
+            f32 carry = 0.0f, cw0 = 0.0f, cw1 = 0.0f;
+            for (u32 j = 0; j < lines; ++j) {
+              carry = carry + line_tot[j];
+              if (j < 64) {
+                if (lane == j) cw0 = carry;
+              } else if (lane == j - 64) {
+                cw1 = carry;
+              }
+            }
+            const f32 sb = carry;
+            if (u24(readlane_u32(r.x, t)) * (sa + sb) > sa) {
+              const f32 tgt = u24(readlane_u32(r.w, t)) * sb;
+              const u32 jsel = pick_line(cw0, cw1, lines, tgt);
+              const u32 e = T[jsel * 64 + lane];
+              znew = pick_in_line(e, slot_mass(e, prow), line_total_before(cw0, cw1, jsel), tgt, zt);
+            } else {
+              znew = readlane_u32(dense_l, t);
+            }
+          }
+        }
+        if (wave == 0) {
+          MapProbe pz = map_probe(T, ds.cap, znew);
+          if (!pz.found) {
+            if (map_capacity(ds.occupied + 1, K) > ds.cap) {
+              const u32 want = 2 * (ds.occupied + 1) < ds.kmax ? 2 * (ds.occupied + 1) : ds.kmax;
+              ds.cap = map_capacity(want, K);
+              regrow_doc_map(T, ds.cap, P.z, b, L, c0, t, znew_lane, zz);
+              pz = map_probe(T, ds.cap, znew);
+            }
+            ++ds.occupied;
+          }
+          map_inc_at(T, ds.cap, znew, pz);
+          if (lane == t) znew_lane = znew;
+        }
+        __syncthreads();
+      }
+      if (wave == 0 && valid) P.z[b + i] = (u16)znew_lane;
+      __syncthreads();
+    }
+    if (wave == 0 && lane == 0 && P.visited) atomicAdd(P.visited, (unsigned long long)visited);
+    if (!head) return;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fold-in: topics of a new document against a fixed model.  Each request (a span of the token
+// arena) starts from its words' alias tables -- a draw from p(k | w) proportional to phi[w][k],
+// uniform for an all-zero row -- with counters (token, request id, 0, FOLD_INIT), then runs
+// `sweeps` passes with counters (token, request id, s, FOLD_SWEEP) and the map carried from pass
+// to pass.  A request's result depends only on its words and its id, never on its batch.
+// When done flags are given, the final topics go to the host-visible output and then the
+// request's flag is raised (system-scope fence in between).
+// ---------------------------------------------------------------------------------------------
+template <int LMAX>
+__global__ __launch_bounds__(64) void k_foldin(SweepParams P, FoldParams F) {
+  extern __shared__ u32 fold_smem[];
+  const u32 lane = lane_id();
+  const u32 slot = F.slots ? F.slots[blockIdx.x] : blockIdx.x;
+  if (slot >= F.n_slots) return;
+  const u64 b = F.span_off[slot];
+  const u32 L = F.span_len[slot];
+  const u32 rid = F.req_id[slot];
+  const u32 K = P.K;
+  for (u32 i = lane; i < L; i += 64) {
+    const u32 w = P.w[b + i];
+    const u32x4 r = philox_tagged(i, rid, 0, kTagFoldInit, 0, P.k0, P.k1);
+    const u32 z = P.sigma_a[w] > 0.0f ? draw_alias_slot(P.alias + (u64)w * K, K, r.y, r.z) : bounded(r.x, K);
+    P.z[b + i] = (u16)z;
+  }
+  __syncthreads();
+  DocState ds = open_span(P, b, L, rid, fold_smem);
+  u64 visited = 0;
+  for (u32 sweep = 1; sweep <= F.sweeps; ++sweep) sweep_pass<LMAX>(P, fold_smem, ds, sweep, kTagFoldSweep, visited);
+  if (F.z_out) {
+    __syncthreads();
+    for (u32 i = lane; i < L; i += 64) F.z_out[b + i] = P.z[b + i];
+  }
+  if (F.done) {
+    __threadfence_system();
+    if (lane == 0) {
+      __atomic_store_n(&F.done[slot], F.done_value, __ATOMIC_RELEASE);
+    }
+  }
+  if (lane == 0 && P.visited) atomicAdd(P.visited, (unsigned long long)visited);
+}
+template __global__ void k_foldin<8>(SweepParams, FoldParams);
 
 // ---------------------------------------------------------------------------------------------
 // n_wk reduction over the word-sorted token permutation.  One wavefront per task (a run of at

@@ -120,10 +120,9 @@ __device__ __forceinline__ MapProbe map_probe(const u32* T, u32 cap, u32 key) {
   return p;
 }
 
-// Count of `key` +1; inserts it (Robin Hood shift) when absent.  Returns 1 if a key was added.
-__device__ __forceinline__ u32 map_inc(u32* T, u32 cap, u32 key) {
+// Count of `key` +1 given its probe p; inserts it (Robin Hood shift) when absent.
+__device__ __forceinline__ u32 map_inc_at(u32* T, u32 cap, u32 key, MapProbe p) {
   const u32 lane = lane_id();
-  MapProbe p = map_probe(T, cap, key);
   if (p.found) {
     if (lane == 0) T[p.slot] = p.entry + 1;
     return 0;
@@ -144,6 +143,9 @@ __device__ __forceinline__ u32 map_inc(u32* T, u32 cap, u32 key) {
   }
   return 1;
 }
+
+// Count of `key` +1; inserts it (Robin Hood shift) when absent.  Returns 1 if a key was added.
+__device__ __forceinline__ u32 map_inc(u32* T, u32 cap, u32 key) { return map_inc_at(T, cap, key, map_probe(T, cap, key)); }
 
 // Count of `key` -1 (the key must be present); deletes it by backward shift at zero.
 // Returns 1 if the key was removed.

@@ -41,6 +41,28 @@ class ResidentCorpus:
         check(self._lib.gpulda_corpus_iterate(n, ptr(ms, C.c_float)), "corpus_iterate")
         return ms
 
+    BUCKETS, PERSISTENT, PERSISTENT_BLOCK = 0, 1, 2
+
+    def schedule(self, kind: int, waves_per_cu: int = 16, block_min_len: int = 0) -> None:
+        """Sweep schedule: BUCKETS (per-bucket launches), PERSISTENT (one queue-driven launch),
+        PERSISTENT_BLOCK (plus block-per-document for documents >= block_min_len tokens)."""
+        check(self._lib.gpulda_corpus_schedule(kind, waves_per_cu, block_min_len), "corpus_schedule")
+
+    def use_graph(self, enabled: bool = True) -> None:
+        check(self._lib.gpulda_corpus_use_graph(1 if enabled else 0), "corpus_use_graph")
+
+    def autotune(self):
+        """Times every candidate schedule on one sweep (chain unchanged) and applies the fastest.
+        Returns (table rows {schedule, waves_per_cu, block_min_len, ms}, chosen index)."""
+        table = np.zeros((16, 4), dtype=np.float32)
+        n = np.zeros(1, dtype=np.uint32)
+        chosen = C.c_int(-1)
+        check(self._lib.gpulda_corpus_autotune(ptr(table, C.c_float), 16, ptr(n, C.c_uint32), C.byref(chosen)),
+              "corpus_autotune")
+        rows = [dict(schedule=int(r[0]), waves_per_cu=int(r[1]), block_min_len=int(r[2]), ms=float(r[3]))
+                for r in table[:int(n[0])]]
+        return rows, int(chosen.value)
+
     def iterate_untimed(self, n: int = 1) -> None:
         check(self._lib.gpulda_corpus_iterate(n, None), "corpus_iterate")
 

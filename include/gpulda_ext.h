@@ -71,6 +71,19 @@ int gpulda_corpus_copy_z(uint32_t* z);
  * [Poisson draw, alias rows, z-sweep, n_wk reduction], timed with hipEvents on the library
  * stream. */
 int gpulda_corpus_iterate(uint32_t n_iters, float* stage_ms);
+/* Sweep schedule of the resident corpus (all produce bit-identical results):
+ *   0  one launch per length bucket (wavefront per document; default);
+ *   1  one persistent launch of cu_count * waves_per_cu wavefronts over a longest-first queue;
+ *   2  as 1, plus a block-per-document persistent launch (4 wavefronts share a document) for
+ *      documents of at least block_min_len tokens, on a second stream. */
+int gpulda_corpus_schedule(int schedule, uint32_t waves_per_cu, uint32_t block_min_len);
+/* 1 = capture one iteration into a hipGraph and replay it (the iteration number lives in device
+ * memory); stage_ms of gpulda_corpus_iterate then reports only the total, in stage_ms[2]. */
+int gpulda_corpus_use_graph(int enabled);
+/* Times one sweep under each candidate schedule on the resident corpus (z restored after each
+ * trial) and applies the fastest.  table: 4 floats per candidate
+ * {schedule, waves_per_cu, block_min_len, ms}.  Needs at least one iteration first. */
+int gpulda_corpus_autotune(float* table, uint32_t max_candidates, uint32_t* n_candidates, int* chosen);
 /* Collapsed joint log-likelihood log p(w, z | alpha, beta) of the current z and n_wk. */
 int gpulda_corpus_loglik(double* out);
 int gpulda_corpus_release(void);
@@ -78,6 +91,21 @@ int gpulda_corpus_release(void);
 /* Measurement aid: `reps` launches of a plain device-to-device copy kernel over `bytes` bytes
  * (reads `bytes`, writes `bytes`), used to scale hardware byte counters; mean ms per copy. */
 int gpulda_calibrate_copy(uint64_t bytes, uint32_t reps, float* ms_per_copy);
+
+/* ---- fold-in server (continuous batching against the resident model) ----------------------
+ * Keeps phi, sigma_a and the alias tables of the current model in HBM.  Requests are admitted
+ * while a slot and arena space (hbm_budget_bytes / 6 tokens) are free; flush launches every
+ * admitted request as one batch on the next of n_streams streams; each request runs `sweeps`
+ * passes, then its topics and completion flag become visible in pinned host memory.  A request's
+ * randomness depends on its id only: its result is the same alone or in any batch. */
+int gpulda_serve_open(uint64_t hbm_budget_bytes, uint32_t sweeps, uint32_t n_streams, uint32_t max_slots);
+/* 0 = admitted (slot set), 1 = no room now (retry after results are collected). */
+int gpulda_serve_submit(const uint32_t* w, uint32_t L, uint32_t request_id, uint32_t* slot);
+int gpulda_serve_flush(void);
+int gpulda_serve_poll(uint32_t* done_slots, uint32_t max, uint32_t* n_done);
+int gpulda_serve_result(uint32_t slot, uint32_t* z, uint32_t* request_id);
+int gpulda_serve_stats(uint32_t* pending, uint32_t* inflight, uint64_t* free_tokens);
+int gpulda_serve_close(void);
 
 /* ---- stage entry points (host arrays in and out; the library stages device copies) ------- */
 int gpulda_stage_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n);
@@ -99,6 +127,11 @@ int gpulda_stage_sweep(uint32_t* z, const uint32_t* w, const uint64_t* doc_off, 
                        const float* prob, const uint32_t* alias, uint64_t* visited);
 int gpulda_stage_count(const uint32_t* w, const uint32_t* z, uint64_t n_tokens, uint32_t V, uint32_t K,
                        uint32_t* n_out);
+/* Fold-in of requests (documents of w delimited by doc_off, ids req_ids) against the given model. */
+int gpulda_stage_foldin(const uint32_t* w, const uint64_t* doc_off, uint32_t n_req, const uint32_t* req_ids,
+                        uint32_t sweeps, uint64_t seed, uint32_t K, uint32_t V, const float* phi,
+                        const float* sigma_a, const float* prob, const uint32_t* alias, uint32_t* z_out,
+                        uint64_t* visited);
 
 #ifdef __cplusplus
 }

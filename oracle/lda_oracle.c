@@ -424,49 +424,68 @@ void orc_map_dec(u32* T, u32 cap, u32 key) {
  * (orc_map_capacity(n0, K)) and, when a new topic would exceed its load
  * (orc_map_capacity(keys + 1, K) > cap), grows to orc_map_capacity(min(2 (keys + 1), min(L, K)), K).
  * The reference starts from the K_d hint and regrows by 1.2x (hashmap.cuh:98, 159-207). */
-u64 orc_sweep_doc(u32* z, const u32* w, u32 L, u32 doc, u32 iter, u64 seed, u32 K, const float* phi,
-                  const float* sigma_a, const OrcAlias* alias, u32* T /* scratch >= 5*capacity(L,K) */) {
+typedef struct {
+  u32* T;
+  float *mass, *pre, *after;
+  u32* scratch;
+  u32 L, K, cap, keys, kmax;
+} OrcDoc;
+
+/* Opens a document: scratch holds >= 5 * orc_map_capacity(L, K) words. */
+static OrcDoc doc_open(const u32* z, u32 L, u32 K, u32* scratch) {
+  OrcDoc d;
   const u32 cap_max = orc_map_capacity(L, K);
-  float* mass = (float*)(T + cap_max);   /* count * phi per slot */
-  float* pre = mass + cap_max;           /* carry of the line + inclusive scan inside the line */
-  float* after = pre + cap_max;          /* running total after each line */
-  u32* scratch = (u32*)(after + cap_max);
+  d.T = scratch;
+  d.mass = (float*)(d.T + cap_max);
+  d.pre = d.mass + cap_max;
+  d.after = d.pre + cap_max;
+  d.scratch = (u32*)(d.after + cap_max);
+  d.L = L;
+  d.K = K;
+  d.kmax = L < K ? L : K;
   u32 n0 = 0;
-  for (u32 s = 0; s < cap_max; ++s) T[s] = MAP_EMPTY;
+  for (u32 s = 0; s < cap_max; ++s) d.T[s] = MAP_EMPTY;
   for (u32 i = 0; i < L; ++i) {
-    if (orc_map_find(T, cap_max, z[i]) < 0) ++n0;
-    orc_map_inc(T, cap_max, z[i]);
+    if (orc_map_find(d.T, cap_max, z[i]) < 0) ++n0;
+    orc_map_inc(d.T, cap_max, z[i]);
   }
-  u32 cap = orc_map_capacity(n0, K);
-  map_regrow(T, cap_max, cap, scratch);  /* the same key set at the starting capacity */
-  u32 keys = n0;
-  const u32 kmax = L < K ? L : K;
+  d.cap = orc_map_capacity(n0, K);
+  map_regrow(d.T, cap_max, d.cap, d.scratch); /* the same key set at the starting capacity */
+  d.keys = n0;
+  return d;
+}
+
+/* One pass over the document's tokens with counters (token, doc, iter, tag); returns the number
+ * of map entries the sparse sums visited. */
+static u64 doc_pass(OrcDoc* d, u32* z, const u32* w, u32 doc, u32 iter, u32 tag, u64 seed, const float* phi,
+                    const float* sigma_a, const OrcAlias* alias) {
+  const u32 K = d->K;
   u64 visited = 0;
   float P[64];
-  for (u32 i = 0; i < L; ++i) {
+  for (u32 i = 0; i < d->L; ++i) {
     u32 wi = w[i], zi = z[i];
     u32 r[4];
-    draw4(i, doc, iter, TAG_SWEEP, 0, seed, r);
-    long at = orc_map_find(T, cap, zi);
-    if ((T[at] & 0xFFFFu) == 1) --keys;
-    orc_map_dec(T, cap, zi);
+    draw4(i, doc, iter, tag, 0, seed, r);
+    long at = orc_map_find(d->T, d->cap, zi);
+    if ((d->T[at] & 0xFFFFu) == 1) --d->keys;
+    orc_map_dec(d->T, d->cap, zi);
     float sa = sigma_a[wi];
     u32 znew = zi;
     if (sa > 0.0f) {
       const float* prow = phi + (u64)wi * K;
-      const u32 lines = cap / 64;
+      const u32 lines = d->cap / 64;
       float carry = 0.0f;
       for (u32 j = 0; j < lines; ++j) {
         for (u32 l = 0; l < 64; ++l) {
-          u32 e = T[j * 64 + l];
+          u32 e = d->T[j * 64 + l];
           visited += e != MAP_EMPTY;
-          mass[j * 64 + l] = (e == MAP_EMPTY) ? 0.0f : (float)(e & 0xFFFFu) * prow[e >> 16];
-          P[l] = mass[j * 64 + l];
+          d->mass[j * 64 + l] = (e == MAP_EMPTY) ? 0.0f : (float)(e & 0xFFFFu) * prow[e >> 16];
+          P[l] = d->mass[j * 64 + l];
         }
         orc_scan64(P);
-        for (u32 l = 0; l < 64; ++l) pre[j * 64 + l] = carry + P[l];
+        for (u32 l = 0; l < 64; ++l) d->pre[j * 64 + l] = carry + P[l];
         carry = carry + P[63];
-        after[j] = carry;
+        d->after[j] = carry;
       }
       float sb = carry;
       float u1 = unif24(r[0]);
@@ -474,33 +493,82 @@ u64 orc_sweep_doc(u32* z, const u32* w, u32 L, u32 doc, u32 iter, u64 seed, u32 
         /* draw_wary_search: the line holding the target, then the slot inside the line */
         float t = unif24(r[3]) * sb;
         u32 j = 0;
-        while (j + 1 < lines && !(after[j] > t)) ++j;
+        while (j + 1 < lines && !(d->after[j] > t)) ++j;
         long pick = -1, last = -1;
         for (u32 l = 0; l < 64; ++l) {
-          float m = mass[j * 64 + l];
+          float m = d->mass[j * 64 + l];
           if (m > 0.0f) {
             last = l;
-            if (pick < 0 && pre[j * 64 + l] > t) pick = l;
+            if (pick < 0 && d->pre[j * 64 + l] > t) pick = l;
           }
         }
         if (pick < 0) pick = last;
-        if (pick >= 0) znew = T[j * 64 + (u32)pick] >> 16;
+        if (pick >= 0) znew = d->T[j * 64 + (u32)pick] >> 16;
       } else {
         znew = draw_alias(alias + (u64)wi * K, K, r[1], r[2]);
       }
     }
-    if (orc_map_find(T, cap, znew) < 0) {
-      if (orc_map_capacity(keys + 1, K) > cap) {
-        u32 want = 2 * (keys + 1) < kmax ? 2 * (keys + 1) : kmax;
+    if (orc_map_find(d->T, d->cap, znew) < 0) {
+      if (orc_map_capacity(d->keys + 1, K) > d->cap) {
+        u32 want = 2 * (d->keys + 1) < d->kmax ? 2 * (d->keys + 1) : d->kmax;
         u32 ncap = orc_map_capacity(want, K);
-        map_regrow(T, cap, ncap, scratch);
-        cap = ncap;
+        map_regrow(d->T, d->cap, ncap, d->scratch);
+        d->cap = ncap;
       }
-      ++keys;
+      ++d->keys;
     }
-    orc_map_inc(T, cap, znew);
+    orc_map_inc(d->T, d->cap, znew);
     z[i] = znew;
   }
+  return visited;
+}
+
+u64 orc_sweep_doc(u32* z, const u32* w, u32 L, u32 doc, u32 iter, u64 seed, u32 K, const float* phi,
+                  const float* sigma_a, const OrcAlias* alias, u32* T /* scratch >= 5*capacity(L,K) */) {
+  OrcDoc d = doc_open(z, L, K, T);
+  return doc_pass(&d, z, w, doc, iter, TAG_SWEEP, seed, phi, sigma_a, alias);
+}
+
+/* Fold-in of one request against a fixed model: topics start from the word's alias table
+ * (a draw from p(k | w) proportional to phi[w][k]; uniform for an all-zero row), then `sweeps`
+ * passes with counters (token, request, sweep s = 1..sweeps, FOLD_SWEEP), the map carried from
+ * pass to pass. */
+u64 orc_foldin_doc(u32* z, const u32* w, u32 L, u32 req, u32 sweeps, u64 seed, u32 K, const float* phi,
+                   const float* sigma_a, const OrcAlias* alias, u32* scratch) {
+  for (u32 i = 0; i < L; ++i) {
+    u32 r[4];
+    draw4(i, req, 0, TAG_FOLD_INIT, 0, seed, r);
+    z[i] = sigma_a[w[i]] > 0.0f ? draw_alias(alias + (u64)w[i] * K, K, r[1], r[2]) : below(r[0], K);
+  }
+  OrcDoc d = doc_open(z, L, K, scratch);
+  u64 visited = 0;
+  for (u32 s = 1; s <= sweeps; ++s) visited += doc_pass(&d, z, w, req, s, TAG_FOLD_SWEEP, seed, phi, sigma_a, alias);
+  return visited;
+}
+
+u64 orc_foldin(u32* z, const u32* w, const u64* off, u32 n_req, const u32* req_ids, u32 sweeps, u64 seed, u32 K,
+               const float* phi, const float* sigma_a, const OrcAlias* alias, int threads) {
+  u32 maxL = 0;
+  for (u32 r = 0; r < n_req; ++r) {
+    u32 L = (u32)(off[r + 1] - off[r]);
+    if (L > maxL) maxL = L;
+  }
+  u32 need = 5 * orc_map_capacity(maxL, K);
+  u64 visited = 0;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(threads > 0 ? threads : 1) reduction(+ : visited)
+#endif
+  {
+    u32* T = (u32*)malloc(sizeof(u32) * need);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (long r = 0; r < (long)n_req; ++r)
+      visited += orc_foldin_doc(z + off[r], w + off[r], (u32)(off[r + 1] - off[r]), req_ids[r], sweeps, seed, K, phi,
+                                sigma_a, alias, T);
+    free(T);
+  }
+  (void)threads;
   return visited;
 }
 

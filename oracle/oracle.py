@@ -42,6 +42,8 @@ _SIGS = {
     "orc_map_apply": (None, [u32p, C.c_uint32, u32p, C.c_uint32]),
     "orc_sweep": (C.c_uint64, [u32p, u32p, u64p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, f32p,
                                f32p, C.POINTER(OrcAlias), C.c_int]),
+    "orc_foldin": (C.c_uint64, [u32p, u32p, u64p, C.c_uint32, u32p, C.c_uint32, C.c_uint64, C.c_uint32, f32p,
+                                f32p, C.POINTER(OrcAlias), C.c_int]),
     "orc_count": (None, [u32p, u32p, C.c_uint64, C.c_uint32, u32p, C.c_uint64]),
     "orc_threads": (C.c_int, []),
     "orc_loglik": (C.c_double, [u32p, C.c_uint32, C.c_uint32, u32p, u64p, C.c_uint32, C.c_double, C.c_double]),
@@ -183,6 +185,21 @@ def sweep(z, w, doc_off, doc_base, iteration, seed, K, phi, sigma_a, prob, al, t
     visited = lib().orc_sweep(_p(z, C.c_uint32), _p(w, C.c_uint32), _p(doc_off, C.c_uint64), len(doc_off) - 1,
                               doc_base, iteration, seed, K, _p(phi, C.c_float), _p(sigma_a, C.c_float),
                               packed.arr, threads)
+    return z, int(visited)
+
+
+def foldin(w, doc_off, req_ids, sweeps, seed, K, phi, sigma_a, prob, al, threads=1):
+    """Fold-in of each request (documents of w delimited by doc_off) against a fixed model."""
+    w = np.ascontiguousarray(w, dtype=np.uint32)
+    doc_off = np.ascontiguousarray(doc_off, dtype=np.uint64)
+    req_ids = np.ascontiguousarray(req_ids, dtype=np.uint32)
+    z = np.zeros(len(w), dtype=np.uint32)
+    phi = np.ascontiguousarray(phi, dtype=np.float32)
+    sigma_a = np.ascontiguousarray(sigma_a, dtype=np.float32)
+    packed = _AliasPacked(np.asarray(prob, dtype=np.float32), np.asarray(al, dtype=np.uint32))
+    visited = lib().orc_foldin(_p(z, C.c_uint32), _p(w, C.c_uint32), _p(doc_off, C.c_uint64), len(req_ids),
+                               _p(req_ids, C.c_uint32), sweeps, seed, K, _p(phi, C.c_float),
+                               _p(sigma_a, C.c_float), packed.arr, threads)
     return z, int(visited)
 
 

@@ -275,3 +275,50 @@ def test_sweep_growth_bit_exact(S, orc):
     w1 = np.arange(300, dtype=np.uint32)
     z1, _ = S.sweep(np.zeros(300, np.uint32), w1, np.array([0, 300], np.uint64), 0, 1, 3, K, phi1, sig1, prob1, al1)
     assert np.array_equal(z1, (3 * w1 + 1) % K)
+
+
+def _schedules_corpus():
+    from gpupolyaurnlda_amd import corpus
+    spec = corpus.BATCHED.scaled(3000)
+    return spec, corpus.generate(spec)
+
+
+def test_schedules_and_graph_bit_identical():
+    """Per-bucket launches, the persistent queue, persistent + block-per-document and a replayed
+    hipGraph give the same chain (same Philox streams) on a power-law corpus."""
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+    spec, c = _schedules_corpus()
+    say(f"schedules corpus N={len(c['w'])} max L={int(c['d_len'].max())}")
+    results = {}
+    for name, sched, graph in [("buckets", 0, False), ("persistent", 1, False), ("block", 2, False),
+                               ("graph", 0, True), ("graph-block", 2, True)]:
+        with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=4, z=c["z"]) as rc:
+            rc.schedule(sched, 8, 1000 if sched == 2 else 0)
+            rc.use_graph(graph)
+            zs = []
+            for _ in range(3):  # replays with eager reads in between
+                rc.iterate_untimed(1)
+                zs.append(rc.z())
+            results[name] = (zs, rc.counts(), rc.visited())
+    ref = results["buckets"]
+    for name, (zs, n, vis) in results.items():
+        for it in range(3):
+            assert np.array_equal(zs[it], ref[0][it]), (name, it)
+        assert np.array_equal(n, ref[1]) and vis == ref[2], name
+
+
+def test_autotune_keeps_chain():
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+    spec, c = _schedules_corpus()
+    with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=4, z=c["z"]) as rc:
+        rc.iterate_untimed(1)
+        z1 = rc.z()
+        rows, chosen = rc.autotune()
+        say(f"autotune: {rows} chosen={chosen}")
+        assert len(rows) >= 5 and 0 <= chosen < len(rows)
+        assert np.array_equal(rc.z(), z1)
+        rc.iterate_untimed(1)
+        z_tuned = rc.z()
+    with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=4, z=c["z"]) as rc:
+        rc.iterate_untimed(2)
+        assert np.array_equal(rc.z(), z_tuned)
