@@ -43,6 +43,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedule", default="auto", choices=["auto", "buckets", "persistent", "block"],
                     help="sweep schedule; auto = the autotuner's choice after warmup")
+    ap.add_argument("--no-serve", action="store_true", help="skip the fold-in serving measurement")
+    ap.add_argument("--serve-requests", type=int, default=4000)
+    ap.add_argument("--serve-serial", type=int, default=100, help="requests in the serial baseline sample")
     ap.add_argument("--compare-graph", action="store_true",
                     help="also time the iteration as a replayed hipGraph against plain per-bucket launches "
                          "(identical Philox streams, z compared)")
@@ -113,6 +116,10 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(rc, spec, c, n_before, z_before, it0, args.steps, args.cpu_docs)
 
+    serving = None
+    if not args.no_serve:
+        serving = serve_bench(rc, spec, args.serve_requests, args.serve_serial)
+
     graph = None
     if args.compare_graph:  # last: it releases rc (the library holds one model at a time)
         graph = compare_graph(rc, c, spec, args.steps)
@@ -139,9 +146,45 @@ def main():
                    "graph_vs_plain": graph},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     rc.close()
     print(json.dumps(out), flush=True)
+
+
+def serve_bench(rc, spec, n_req, n_serial):
+    """Fold-in serving against the model just trained (phi and alias tables resident): requests
+    with lognormal(300, 1.2) lengths clipped to [10, 5000], words from a random mixture of 1-5
+    topics, Poisson arrivals at 2,000 docs/s, 50 sweeps each, admission bounded by 16 GB of HBM.
+    Baseline: the first n_serial requests run one at a time through the same sampler."""
+    from gpupolyaurnlda_amd import corpus
+    from gpupolyaurnlda_amd.serve import FoldInServer, percentiles, run_serial, run_stream
+
+    phi = rc.phi()
+    d_len, arrival, w = corpus.requests(phi, n_req, median=300.0, sigma=1.2, lo=10, hi=5000, rate=2000.0, seed=4)
+    del phi
+    with FoldInServer(hbm_budget_bytes=16 << 30, sweeps=50, n_streams=8, max_slots=16384) as srv:
+        # warm the kernels once outside the measurement
+        run_serial(srv, d_len[:2], w[:int(d_len[:2].sum())], id_base=1 << 30)
+        mixed = run_stream(srv, d_len, arrival, w, window=0.0005)
+        n_s = min(n_serial, n_req)
+        serial = run_serial(srv, d_len[:n_s], w[:int(d_len[:n_s].sum())])
+    lat = mixed["latency_s"]
+    short = d_len <= 1000
+    docs_s = n_req / mixed["wall_s"]
+    serial_docs_s = n_s / serial["service_s"].sum()
+    same = all(np.array_equal(mixed["results"][i], serial["results"][i]) for i in range(n_s))
+    out = {"requests": n_req, "offered_docs_per_s": 2000.0, "sweeps": 50,
+           "docs_per_s": round(docs_s, 1),
+           "latency_ms": {k: round(v * 1e3, 3) for k, v in percentiles(lat).items()},
+           "latency_ms_le1000_tokens": {k: round(v * 1e3, 3) for k, v in percentiles(lat[short]).items()},
+           "serial_docs_per_s": round(serial_docs_s, 2),
+           "serial_sample": n_s,
+           "speedup_vs_serial": round(docs_s / serial_docs_s, 1),
+           "results_identical_alone_vs_mixed": bool(same),
+           "mean_tokens": round(float(d_len.mean()), 1)}
+    log(f"serving: {out}")
+    return out
 
 
 def compare_graph(rc, c, spec, steps):

@@ -1322,7 +1322,8 @@ extern "C" int gpulda_serve_flush(void) {
   // ring of 2 * n_slots list entries: a wrap can only reuse entries of requests already finished,
   // because at most n_slots requests are admitted at a time
   if (SV.batch_head + nb > 2 * SV.n_slots) SV.batch_head = 0;
-  u32* list = SV.h_batch + SV.batch_head;
+  const u32 list_at = SV.batch_head;
+  u32* list = SV.h_batch + list_at;
   SV.batch_head += nb;
   for (u32 i = 0; i < nb; ++i) {
     const u32 sl = SV.pending[i];
@@ -1331,7 +1332,8 @@ extern "C" int gpulda_serve_flush(void) {
                        hipMemcpyHostToDevice, st));
   }
   FoldParams F;
-  F.slots = dev_view(list);
+  u32* batch_dev = dev_view(SV.h_batch);  // device view of the pinned base, then the offset
+  F.slots = batch_dev ? batch_dev + list_at : nullptr;
   F.n_slots = SV.n_slots;
   F.span_off = dev_view(SV.h_off);
   F.span_len = dev_view(SV.h_len);

@@ -46,6 +46,19 @@ struct FoldParams {
 template <int LMAX>
 __global__ void k_foldin(SweepParams P, FoldParams F);
 
+// Request ring of the serving worker pool (ring, tail and shutdown host-visible; head on the
+// device).
+struct ServeRing {
+  const u32* ring;
+  u32 ring_size;
+  const u32* tail;
+  u32* head;
+  const u32* shutdown;
+  u64 lifetime_ticks;  // s_memrealtime ticks (100 MHz) a wave may keep claiming
+  u64 idle_ticks;      // ticks an idle wave waits for work before leaving
+};
+__global__ void k_serve_worker(SweepParams P, FoldParams F, ServeRing R);
+
 __global__ void k_fill_u32(u32* p, u64 n, u32 v);
 __global__ void k_fill_u64(unsigned long long* p, u64 n, unsigned long long v);
 __global__ void k_z_narrow(const u32* in, u16* out, u64 n, u32 K, u32* bad);

@@ -694,6 +694,70 @@ __global__ __launch_bounds__(64) void k_foldin(SweepParams P, FoldParams F) {
 template __global__ void k_foldin<8>(SweepParams, FoldParams);
 
 // ---------------------------------------------------------------------------------------------
+// Serving worker: a pool of wavefronts that take requests from a ring written by the host
+// (continuous batching).  A wave claims ring entry `head` with a CAS while head < tail (tail and
+// the ring live in host-visible memory), runs the request's fold-in (the same steps as k_foldin),
+// publishes the result and claims the next one.  Every wave exits when the ring is empty and it
+// has been idle for idle_ticks, when lifetime_ticks have passed since it started, or when the
+// host raises *shutdown -- so the grid always drains.  Ticks are s_memrealtime (100 MHz).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ u64 rt_ticks() { return __builtin_amdgcn_s_memrealtime(); }
+
+__global__ __launch_bounds__(64) void k_serve_worker(SweepParams P, FoldParams F, ServeRing R) {
+  extern __shared__ u32 serve_smem[];
+  const u32 lane = lane_id();
+  const u32 K = P.K;
+  const u64 start = rt_ticks();
+  for (;;) {
+    u32 item = 0xFFFFFFFFu;
+    if (lane == 0) {
+      u64 idle_from = rt_ticks();
+      u32 nap = 1;
+      for (;;) {
+        const u32 t = __atomic_load_n(R.tail, __ATOMIC_ACQUIRE);
+        const u32 h = __hip_atomic_load(R.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (h != t) {  // tail - head counts unclaimed entries (both only grow)
+          u32 expect = h;
+          if (__hip_atomic_compare_exchange_strong(R.head, &expect, h + 1, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)) {
+            item = h;
+            break;
+          }
+          continue;
+        }
+        const u64 now = rt_ticks();
+        if (__atomic_load_n(R.shutdown, __ATOMIC_ACQUIRE) || now - start > R.lifetime_ticks ||
+            now - idle_from > R.idle_ticks)
+          break;
+        for (u32 k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(32);
+        nap = nap < 16 ? nap * 2 : 16;
+      }
+    }
+    item = readlane_u32(item, 0);
+    if (item == 0xFFFFFFFFu) return;
+    const u32 slot = __atomic_load_n(&R.ring[item % R.ring_size], __ATOMIC_ACQUIRE);
+    const u64 b = F.span_off[slot];
+    const u32 L = F.span_len[slot];
+    const u32 rid = F.req_id[slot];
+    for (u32 i = lane; i < L; i += 64) {
+      const u32 w = P.w[b + i];
+      const u32x4 r = philox_tagged(i, rid, 0, kTagFoldInit, 0, P.k0, P.k1);
+      const u32 z = P.sigma_a[w] > 0.0f ? draw_alias_slot(P.alias + (u64)w * K, K, r.y, r.z) : bounded(r.x, K);
+      P.z[b + i] = (u16)z;
+    }
+    __syncthreads();
+    DocState ds = open_span(P, b, L, rid, serve_smem);
+    u64 visited = 0;
+    for (u32 sweep = 1; sweep <= F.sweeps; ++sweep) sweep_pass<8>(P, serve_smem, ds, sweep, kTagFoldSweep, visited);
+    __syncthreads();
+    for (u32 i = lane; i < L; i += 64) F.z_out[b + i] = P.z[b + i];
+    __threadfence_system();
+    if (lane == 0) __atomic_store_n(&F.done[slot], F.done_value, __ATOMIC_RELEASE);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // n_wk reduction over the word-sorted token permutation.  One wavefront per task (a run of at
 // most kCountTask tokens of one word): LDS histogram over K, then nonzero bins added to n_wk.
 // ---------------------------------------------------------------------------------------------

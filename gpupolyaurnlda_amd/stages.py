@@ -124,3 +124,21 @@ def count(w, z, V: int, K: int) -> np.ndarray:
     check(lib().gpulda_stage_count(ptr(w, C.c_uint32), ptr(z, C.c_uint32), len(w), V, K, ptr(out, C.c_uint32)),
           "stage_count")
     return out
+
+
+def foldin(w, doc_off, req_ids, sweeps: int, seed: int, K: int, phi, sigma_a, prob, al):
+    """Fold-in of each request (documents of w delimited by doc_off) against a fixed model;
+    returns (topics, map entries visited)."""
+    w = _u32(w)
+    doc_off = np.ascontiguousarray(doc_off, dtype=np.uint64)
+    req_ids = _u32(req_ids)
+    phi = _f32(phi)
+    V = phi.shape[0]
+    z = np.empty(len(w), dtype=np.uint32)
+    visited = np.zeros(1, dtype=np.uint64)
+    check(lib().gpulda_stage_foldin(ptr(w, C.c_uint32), ptr(doc_off, C.c_uint64), len(req_ids),
+                                    ptr(req_ids, C.c_uint32), sweeps, seed, K, V, ptr(phi, C.c_float),
+                                    ptr(_f32(sigma_a), C.c_float), ptr(_f32(prob), C.c_float),
+                                    ptr(_u32(al), C.c_uint32), ptr(z, C.c_uint32), ptr(visited, C.c_uint64)),
+          "stage_foldin")
+    return z, int(visited[0])

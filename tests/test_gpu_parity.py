@@ -322,3 +322,74 @@ def test_autotune_keeps_chain():
     with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=4, z=c["z"]) as rc:
         rc.iterate_untimed(2)
         assert np.array_equal(rc.z(), z_tuned)
+
+
+def test_foldin_bit_exact(S, orc):
+    rng = np.random.default_rng(21)
+    V, K = 500, 256
+    phi, sig, prob, al = _tiny_phi(orc, V, K, seed=3, alpha=0.1)
+    L = np.concatenate([[1, 5], rng.integers(10, 800, 60)]).astype(np.uint32)
+    off = np.concatenate([[0], np.cumsum(L)]).astype(np.uint64)
+    w = rng.integers(0, V, int(off[-1])).astype(np.uint32)
+    ids = (np.arange(len(L)) * 7 + 100).astype(np.uint32)
+    zg, vg = S.foldin(w, off, ids, 12, 5, K, phi, sig, prob, al)
+    zc, vc = orc.foldin(w, off, ids, 12, 5, K, phi, sig, prob, al, threads=8)
+    assert vg == vc and np.array_equal(zg, zc)
+
+
+def test_server_alone_equals_mixed(orc):
+    """Continuous batching: each request's result is bit-identical whether it runs alone or in
+    a mixed stream, and equals the stage fold-in of the same request id."""
+    from gpupolyaurnlda_amd import corpus
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+    from gpupolyaurnlda_amd.serve import FoldInServer, run_serial, run_stream
+    from gpupolyaurnlda_amd import stages
+    spec = corpus.MEDIUM.scaled(2000)
+    c = corpus.generate(spec)
+    with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=1, z=c["z"]) as rc:
+        rc.iterate_untimed(3)
+        phi = rc.phi()
+        sig = rc.sigma_a()
+        prob, al = rc.alias()
+        d_len, arr, w = corpus.requests(phi, 60, median=120, sigma=1.0, lo=10, hi=1500, rate=3000.0, seed=4)
+        with FoldInServer(hbm_budget_bytes=64 << 20, sweeps=20, n_streams=4, max_slots=256) as srv:
+            mixed = run_stream(srv, d_len, arr, w, window=0.0005)
+            alone = run_serial(srv, d_len[:12], w[:int(d_len[:12].sum())])
+        say(f"server p50={np.median(mixed['latency_s']) * 1e3:.2f} ms serial mean={alone['service_s'].mean() * 1e3:.2f} ms")
+    for i in range(12):
+        assert np.array_equal(mixed["results"][i], alone["results"][i]), i
+    off = np.concatenate([[0], np.cumsum(d_len)]).astype(np.uint64)
+    zs, _ = stages.foldin(w, off, np.arange(len(d_len), dtype=np.uint32), 20, 1, spec.K, phi, sig, prob, al)
+    assert np.array_equal(np.concatenate(mixed["results"]), zs)
+    zc, _ = orc.foldin(w, off, np.arange(len(d_len), dtype=np.uint32), 20, 1, spec.K, phi, sig, prob, al, threads=8)
+    assert np.array_equal(zs, zc)
+
+
+def test_medium_config_parity_and_ll_trace(orc):
+    """Medium workload shape (V=50k Zipf, K=256, lognormal lengths) on a 20k-document slice:
+    three iterations bit-exact against the oracle, then a 60-iteration log-likelihood trace on
+    the GPU that must rise from the random start and settle (last 10 within 0.5%)."""
+    from gpupolyaurnlda_amd import corpus
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+    spec = corpus.MEDIUM.scaled(20000)
+    c = corpus.generate(spec)
+    say(f"medium slice N={len(c['w'])}")
+    model = orc.Model(c["w"], c["doc_off"], c["z"], spec.V, spec.K, spec.alpha, spec.beta, seed=1, counts="z",
+                      threads=16)
+    with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=1, z=c["z"]) as rc:
+        lls = [rc.loglik()]
+        for it in range(3):
+            rc.iterate_untimed(1)
+            model.iterate()
+            assert np.array_equal(rc.z(), model.z), it
+            assert np.array_equal(rc.counts(), model.n), it
+            lls.append(rc.loglik())
+        ll_c = orc.loglik(model.n, model.z, c["doc_off"], float(np.float32(spec.alpha)), float(np.float32(spec.beta)))
+        assert abs(lls[-1] - ll_c) <= 1e-9 * abs(ll_c)
+        for _ in range(57):
+            rc.iterate_untimed(1)
+            lls.append(rc.loglik())
+    say(f"LL trace: start {lls[0]:.4e} it3 {lls[3]:.4e} it60 {lls[-1]:.4e}")
+    assert lls[-1] > lls[0]
+    tail = np.array(lls[-10:])
+    assert np.abs(tail - tail.mean()).max() <= 0.005 * abs(tail.mean())

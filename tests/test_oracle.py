@@ -392,3 +392,46 @@ def test_sweep_growth_known_answer(orc):
     znew, visited = orc.sweep(z, w, np.array([0, L], np.uint64), 0, 1, 3, K, phi, sig, prob, al)
     assert np.array_equal(znew, (3 * w + 1) % K)
     assert visited > 0
+
+
+def test_foldin_exact_conditional(orc):
+    """One fold-in sweep of two-token requests: z0, z1 start independently from p(k | w) (the
+    alias tables), then z0' | z1 and z1' | z0' follow the full conditionals.  The joint law of
+    (z0', z1') is enumerated exactly and compared by chi-square."""
+    phi, sig, prob, al, alpha = _tiny_model(orc)
+    K = phi.shape[1]
+    ph = phi.astype(np.float64)
+    w = np.array([1, 2], np.uint32)
+    n_req = 40000
+    W = np.tile(w, n_req)
+    off = np.arange(0, 2 * n_req + 1, 2, dtype=np.uint64)
+    z, _ = orc.foldin(W, off, np.arange(n_req, dtype=np.uint32), 1, 7, K, phi, sig, prob, al, threads=4)
+    pairs = z.reshape(-1, 2)
+    obs = np.bincount(pairs[:, 0] * K + pairs[:, 1], minlength=K * K).astype(float)
+    init1 = ph[w[1]] / ph[w[1]].sum()
+    ex = np.zeros((K, K))
+    for z1 in range(K):
+        p0 = ph[w[0]] * (float(alpha) + (np.arange(K) == z1))
+        p0 /= p0.sum()
+        for a in range(K):
+            p1 = ph[w[1]] * (float(alpha) + (np.arange(K) == a))
+            ex[a] += init1[z1] * p0[a] * p1 / p1.sum()
+    ex = ex.reshape(-1) * n_req
+    keep = ex > 0
+    assert obs[~keep].sum() == 0
+    assert stats.chisquare(obs[keep], ex[keep]).pvalue > 1e-4
+
+
+def test_foldin_recovers_topic(orc):
+    """Requests drawn from one topic's vocabulary fold in onto that topic."""
+    V, K = 40, 4
+    phi_t = np.zeros((V, K), np.float32)
+    for k in range(K):
+        phi_t[k * 10:(k + 1) * 10, k] = 3
+    phi_t[:, :] += (np.random.default_rng(0).random((V, K)) < 0.05) * 1
+    phi, sig, prob, al = orc.phi_finish(phi_t, phi_t.sum(0).astype(np.uint64), 0.1)
+    rng = np.random.default_rng(1)
+    reqs = [rng.integers(20, 30, 200).astype(np.uint32) for _ in range(20)]  # topic 2's words
+    off = np.concatenate([[0], np.cumsum([len(r) for r in reqs])]).astype(np.uint64)
+    z, _ = orc.foldin(np.concatenate(reqs), off, np.arange(20, dtype=np.uint32), 20, 3, K, phi, sig, prob, al)
+    assert (z == 2).mean() > 0.9
