@@ -248,15 +248,29 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
 // z-sweep, one wavefront per document, the doc-topic map resident in LDS.
 // Per token i (word w, old topic z), with r = Philox(i, doc, iter, SWEEP):
 //   remove z; if sigma_a[w] == 0 (phi_w. is all zero) keep z; else
-//   sigma_b = canonical scan of count*phi[w][key] over the map's 64-slot lines, carried line by
-//   line; sparse if u(r.x) * (sigma_a + sigma_b) > sigma_a: target t = u(r.w) * sigma_b, first
-//   line whose running total exceeds t, then inside it the first occupied slot whose prefix
-//   exceeds t (the last occupied one if rounding leaves none); otherwise the alias table of w
-//   (slot from r.y, keep test on r.z); add the new topic.
+//   m[j][l] = count * phi[w][key] of map slot j*64 + l (line j, lane l; 0 when empty);
+//   G_g = column sums of line group g = j % 4, lines left to right; A = ((G_0 + G_1) + G_2) + G_3;
+//   P = canonical scan of A; sigma_b = P[63];
+//   sparse if u(r.x) * (sigma_a + sigma_b) > sigma_a: target t = u(r.w) * sigma_b, lane l* = the
+//   first with A > 0 and P > t (else the last with A > 0), E = P[l* - 1]; in column l* the first
+//   slot in (group, line) order with m > 0 and E + (base_g + run) > t, base_g = G_0 + .. + G_{g-1}
+//   and run the group's masses up to that line (else the last slot with m > 0);
+//   otherwise the alias table of w (slot from r.y, keep test on r.z); add the new topic.
+// Each lane sums its own column, so a token costs one wave scan however many lines the map has.
 // ---------------------------------------------------------------------------------------------
+constexpr u32 kNoSlot = 0xFFFFFFFFu;
+
+// Mass of one map entry for row prow.  An empty slot counts 0 at key 0, so the gather and the
+// product are unconditional (no branch, all gathers of a token in flight together) and its mass
+// is 0 * phi[w][0] = +0, the value the definition gives it.
 __device__ __forceinline__ f32 slot_mass(u32 e, const f32* __restrict__ prow) {
-  return e == kMapEmpty ? 0.0f : (f32)map_count(e) * prow[map_key(e)];
+  const bool empty = e == kMapEmpty;
+  const u32 cnt = empty ? 0u : map_count(e);
+  const u32 key = empty ? 0u : map_key(e);
+  return (f32)cnt * prow[key];
 }
+
+__device__ __forceinline__ f32 combine_groups(f32 g0, f32 g1, f32 g2, f32 g3) { return ((g0 + g1) + g2) + g3; }
 
 // Number of distinct topics among z[b .. b+L): a K-bit LDS bitmap, then a popcount reduction.
 __device__ __forceinline__ u32 distinct_topics(u32* bits, u32 K, const u16* __restrict__ z, u64 b, u32 L) {
@@ -307,32 +321,6 @@ __device__ __noinline__ void regrow_doc_map(u32* T, u32 ncap, const u16* __restr
   }
 }
 
-// Picks the sparse topic inside line `jsel` (prefix of the line = cprev + canonical scan).
-__device__ __forceinline__ u32 pick_in_line(u32 e, f32 v, f32 cprev, f32 tgt, u32 fallback) {
-  const f32 pv = cprev + wave_scan_f32(v);
-  const u64 cand = __ballot(v > 0.0f && pv > tgt);
-  const u64 pos = __ballot(v > 0.0f);
-  if (!pos) return fallback;  // unreachable: the selected line holds positive mass
-  const u32 l = cand ? first_lane(cand) : 63u - (u32)__clzll((long long)pos);
-  return map_key(readlane_u32(e, l));
-}
-
-// Line index whose running total first exceeds tgt (lane j of cw0 / cw1 holds the total after
-// line j / j + 64).
-__device__ __forceinline__ u32 pick_line(f32 cw0, f32 cw1, u32 lines, f32 tgt) {
-  const u32 lane = lane_id();
-  u64 bw = __ballot(lane < lines && cw0 > tgt);
-  if (bw) return first_lane(bw);
-  bw = __ballot(lane + 64 < lines && cw1 > tgt);
-  if (bw) return 64 + first_lane(bw);
-  return lines - 1;  // unreachable: the last line's total is sigma_b > tgt
-}
-
-__device__ __forceinline__ f32 line_total_before(f32 cw0, f32 cw1, u32 jsel) {
-  if (jsel == 0) return 0.0f;
-  return jsel - 1 < 64 ? readlane_f32(cw0, jsel - 1) : readlane_f32(cw1, jsel - 65);
-}
-
 // Per-document state of a sweep.
 struct DocState {
   u64 b;          // first token
@@ -348,12 +336,12 @@ struct DocState {
 // exceed the load limit.  The reference starts from the K_d hint (hashmap.cuh:98).
 __device__ __forceinline__ DocState open_span(const SweepParams& P, u64 b, u32 L, u32 gdoc, u32* T) {
   DocState ds;
-  ds.b = b;
-  ds.L = L;
+  ds.b = uniform_u64(b);
+  ds.L = uniform_u32(L);
   ds.cap_max = map_capacity(ds.L, P.K);
   ds.kmax = ds.L < P.K ? ds.L : P.K;
-  ds.gdoc = gdoc;
-  ds.occupied = distinct_topics(T + ds.cap_max, P.K, P.z, ds.b, ds.L);
+  ds.gdoc = uniform_u32(gdoc);
+  ds.occupied = uniform_u32(distinct_topics(T + ds.cap_max, P.K, P.z, ds.b, ds.L));
   ds.cap = map_capacity(ds.occupied, P.K);
   build_doc_map(T, ds.cap, P.z, ds.b, ds.L);
   return ds;
@@ -364,78 +352,168 @@ __device__ __forceinline__ DocState open_doc(const SweepParams& P, u32 doc, u32*
   return open_span(P, b, (u32)(P.doc_off[doc + 1] - b), P.doc_base + doc, T);
 }
 
-// Sparse pick with every line of the map held in registers (NL >= lines): all phi gathers of the
-// token are issued before the scans.
+// Outcome of the draw for one token: the new topic, and for a sparse pick the map slot (and its
+// entry) that holds it, so the increment needs no probe.
+struct Pick {
+  u32 topic;
+  u32 slot;   // kNoSlot unless sparse
+  u32 entry;
+};
+
+// Lane l* of the sparse pick and its exclusive prefix E.
+__device__ __forceinline__ u32 pick_lane(f32 A, f32 P, f32 tgt, f32& E) {
+  const u64 c = __ballot(A > 0.0f && P > tgt);
+  const u64 pos = __ballot(A > 0.0f);
+  const u32 ls = c ? first_lane(c) : 63u - (u32)__clzll((long long)pos);
+  E = ls ? readlane_f32(P, ls - 1) : 0.0f;
+  return ls;
+}
+
+// Running state of the in-column search (every lane searches its own column; lane l*'s result is
+// the one used).
+struct ColumnSearch {
+  u32 hit;   // first line with m > 0 and E + (base + run) > t, or kNoSlot
+  u32 last;  // last line with m > 0, or kNoSlot
+  __device__ __forceinline__ void step(u32 j, f32 m, f32 cum, f32 E, f32 tgt) {
+    const bool pj = m > 0.0f;
+    if (pj && hit == kNoSlot && E + cum > tgt) hit = j;
+    if (pj) last = j;
+  }
+  __device__ __forceinline__ u32 line() const { return hit != kNoSlot ? hit : last; }
+};
+
+// Sparse/dense pick with every line of the map held in registers (NL >= lines): all phi gathers
+// of the token are issued before any arithmetic on them.
 template <int NL>
-__device__ __forceinline__ u32 draw_topic_regs(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa,
-                                               u32 rx, u32 rw, u32 dense, u32 zt) {
+__device__ __forceinline__ Pick draw_topic_regs(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa,
+                                                u32 rx, u32 rw, u32 dense) {
   const u32 lane = lane_id();
-  f32 carry = 0.0f;
-  f32 cw0 = 0.0f;  // running total after line j, held by lane j
   u32 e[NL];
-  f32 ph[NL];
+  f32 m[NL];
 #pragma unroll
   for (int j = 0; j < NL; ++j) e[j] = (u32)j < lines ? T[j * 64 + lane] : kMapEmpty;
 #pragma unroll
-  for (int j = 0; j < NL; ++j) ph[j] = e[j] != kMapEmpty ? prow[map_key(e[j])] : 0.0f;
+  for (int j = 0; j < NL; ++j) m[j] = slot_mass(e[j], prow);
+  // G_g starts at line g's mass (0 + m == m, masses are never -0)
+  f32 G[4];
 #pragma unroll
-  for (int j = 0; j < NL; ++j) {
-    if ((u32)j < lines) {
-      f32 v = e[j] != kMapEmpty ? (f32)map_count(e[j]) * ph[j] : 0.0f;
-      carry = carry + readlane_f32(wave_scan_f32(v), 63);
-      if (lane == (u32)j) cw0 = carry;
+  for (int g = 0; g < 4; ++g) G[g] = g < NL ? m[g] : 0.0f;
+#pragma unroll
+  for (int j = 4; j < NL; ++j) G[j & 3] = G[j & 3] + m[j];
+  const f32 A = combine_groups(G[0], G[1], G[2], G[3]);
+  const f32 P = wave_scan_f32(A);
+  const f32 sb = readlane_f32(P, 63);
+  Pick pk;
+  pk.topic = dense;
+  pk.slot = kNoSlot;
+  pk.entry = 0;
+  if (!(u24(rx) * (sa + sb) > sa)) return pk;
+  const f32 tgt = u24(rw) * sb;
+  f32 E;
+  const u32 ls = pick_lane(A, P, tgt, E);
+  const f32 base[4] = {0.0f, G[0], G[0] + G[1], (G[0] + G[1]) + G[2]};
+  ColumnSearch cs{kNoSlot, kNoSlot};
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    f32 run = 0.0f;
+#pragma unroll
+    for (int j = g; j < NL; j += 4) {
+      run = run + m[j];
+      cs.step((u32)j, m[j], base[g] + run, E, tgt);
     }
   }
-  const f32 sb = carry;
-  if (!(u24(rx) * (sa + sb) > sa)) return dense;
-  const f32 tgt = u24(rw) * sb;
-  const u32 jsel = pick_line(cw0, 0.0f, lines, tgt);
-  u32 es = kMapEmpty;
-  f32 ps = 0.0f;
+  const u32 js = readlane_u32(cs.line(), ls);
+  u32 es = e[0];
 #pragma unroll
-  for (int j = 0; j < NL; ++j)
-    if ((u32)j == jsel) {
-      es = e[j];
-      ps = ph[j];
-    }
-  const f32 v = es != kMapEmpty ? (f32)map_count(es) * ps : 0.0f;
-  return pick_in_line(es, v, line_total_before(cw0, 0.0f, jsel), tgt, zt);
+  for (int j = 1; j < NL; ++j)
+    if ((u32)j == js) es = e[j];
+  pk.entry = readlane_u32(es, ls);
+  pk.topic = map_key(pk.entry);
+  pk.slot = js * 64 + ls;
+  return pk;
 }
 
-// Sparse pick for any number of lines, one line at a time.
-__device__ __forceinline__ u32 draw_topic_loop(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa,
-                                               u32 rx, u32 rw, u32 dense, u32 zt) {
+// Sparse/dense pick for any number of lines: a first pass for the column sums, and for a sparse
+// draw a second pass that recomputes the same masses (same operations, so the same values)
+// for the in-column search.
+__device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa,
+                                                u32 rx, u32 rw, u32 dense) {
   const u32 lane = lane_id();
-  f32 carry = 0.0f;
-  f32 cw0 = 0.0f, cw1 = 0.0f;  // running total after line j, held by lane j (j < 128)
-  for (u32 j = 0; j < lines; ++j) {
-    f32 sc = wave_scan_f32(slot_mass(T[j * 64 + lane], prow));
-    carry = carry + readlane_f32(sc, 63);
-    if (j < 64) {
-      if (lane == j) cw0 = carry;
-    } else if (lane == j - 64) {
-      cw1 = carry;
+  f32 G0 = 0.0f, G1 = 0.0f, G2 = 0.0f, G3 = 0.0f;
+  for (u32 j = 0; j < lines; j += 4) {
+    const u32 e0 = T[j * 64 + lane];
+    const u32 e1 = j + 1 < lines ? T[(j + 1) * 64 + lane] : kMapEmpty;
+    const u32 e2 = j + 2 < lines ? T[(j + 2) * 64 + lane] : kMapEmpty;
+    const u32 e3 = j + 3 < lines ? T[(j + 3) * 64 + lane] : kMapEmpty;
+    const f32 m0 = slot_mass(e0, prow), m1 = slot_mass(e1, prow), m2 = slot_mass(e2, prow),
+              m3 = slot_mass(e3, prow);
+    G0 = G0 + m0;
+    G1 = G1 + m1;
+    G2 = G2 + m2;
+    G3 = G3 + m3;
+  }
+  const f32 A = combine_groups(G0, G1, G2, G3);
+  const f32 P = wave_scan_f32(A);
+  const f32 sb = readlane_f32(P, 63);
+  Pick pk;
+  pk.topic = dense;
+  pk.slot = kNoSlot;
+  pk.entry = 0;
+  if (!(u24(rx) * (sa + sb) > sa)) return pk;
+  const f32 tgt = u24(rw) * sb;
+  f32 E;
+  const u32 ls = pick_lane(A, P, tgt, E);
+  const f32 base[4] = {0.0f, G0, G0 + G1, (G0 + G1) + G2};
+  ColumnSearch cs{kNoSlot, kNoSlot};
+#pragma unroll
+  for (u32 g = 0; g < 4; ++g) {
+    f32 run = 0.0f;
+    for (u32 j = g; j < lines; j += 4) {
+      const f32 m = slot_mass(T[j * 64 + lane], prow);
+      run = run + m;
+      cs.step(j, m, base[g] + run, E, tgt);
     }
   }
-  const f32 sb = carry;
-  if (!(u24(rx) * (sa + sb) > sa)) return dense;
-  const f32 tgt = u24(rw) * sb;
-  const u32 jsel = pick_line(cw0, cw1, lines, tgt);
-  const u32 e = T[jsel * 64 + lane];
-  return pick_in_line(e, slot_mass(e, prow), line_total_before(cw0, cw1, jsel), tgt, zt);
+  const u32 js = readlane_u32(cs.line(), ls);
+  pk.slot = js * 64 + ls;
+  pk.entry = readlane_u32(T[js * 64 + lane], ls);
+  pk.topic = map_key(pk.entry);
+  return pk;
 }
 
 // Sparse/dense pick for one token (sigma_a > 0).  The register variant is chosen by the map's
-// current line count, up to LMAX lines (LMAX caps the registers of a kernel); larger maps go
-// line by line.  All variants evaluate the same operations in the same order.
+// current line count, up to LMAX lines (LMAX caps the registers of a kernel); larger maps use
+// the two-pass form.  All variants evaluate the same operations in the same order.
 template <int LMAX>
-__device__ __forceinline__ u32 draw_topic(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa, u32 rx,
-                                          u32 rw, u32 dense, u32 zt) {
-  if (LMAX >= 1 && lines <= 1) return draw_topic_regs<1>(T, lines, prow, sa, rx, rw, dense, zt);
-  if (LMAX >= 2 && lines <= 2) return draw_topic_regs<2>(T, lines, prow, sa, rx, rw, dense, zt);
-  if (LMAX >= 4 && lines <= 4) return draw_topic_regs<4>(T, lines, prow, sa, rx, rw, dense, zt);
-  if (LMAX >= 8 && lines <= 8) return draw_topic_regs<8>(T, lines, prow, sa, rx, rw, dense, zt);
-  return draw_topic_loop(T, lines, prow, sa, rx, rw, dense, zt);
+__device__ __forceinline__ Pick draw_topic(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa, u32 rx,
+                                           u32 rw, u32 dense) {
+  if (LMAX >= 1 && lines <= 1) return draw_topic_regs<1>(T, lines, prow, sa, rx, rw, dense);
+  if (LMAX >= 2 && lines <= 2) return draw_topic_regs<2>(T, lines, prow, sa, rx, rw, dense);
+  if (LMAX >= 4 && lines <= 4) return draw_topic_regs<4>(T, lines, prow, sa, rx, rw, dense);
+  if (LMAX >= 8 && lines <= 8) return draw_topic_regs<8>(T, lines, prow, sa, rx, rw, dense);
+  return draw_topic_loop(T, lines, prow, sa, rx, rw, dense);
+}
+
+// Adds the drawn topic to the map: a sparse pick increments the slot it came from; otherwise the
+// topic is probed for and inserted when absent, the map growing first if the new key would
+// exceed its load limit.
+__device__ __forceinline__ void add_topic(const SweepParams& P, u32* T, DocState& ds, const Pick& pk, u32 c0, u32 t,
+                                          u32 znew_lane, u32 zz) {
+  if (pk.slot != kNoSlot) {
+    if (lane_id() == 0) T[pk.slot] = pk.entry + 1;
+    return;
+  }
+  MapProbe pz = map_probe(T, ds.cap, pk.topic);
+  if (!pz.found) {
+    if (map_capacity(ds.occupied + 1, P.K) > ds.cap) {
+      const u32 want = 2 * (ds.occupied + 1) < ds.kmax ? 2 * (ds.occupied + 1) : ds.kmax;
+      ds.cap = map_capacity(want, P.K);
+      regrow_doc_map(T, ds.cap, P.z, ds.b, ds.L, c0, t, znew_lane, zz);
+      pz = map_probe(T, ds.cap, pk.topic);
+    }
+    ++ds.occupied;
+  }
+  map_inc_at(T, ds.cap, pk.topic, pz);
 }
 
 // One pass over a document's tokens with Philox counters (token, gdoc, iter, tag).
@@ -465,24 +543,17 @@ __device__ __forceinline__ void sweep_pass(const SweepParams& P, u32* T, DocStat
       const u32 zt = readlane_u32(zz, t);
       ds.occupied -= map_dec(T, ds.cap, zt);
       const f32 sa = readlane_f32(sa_l, t);
-      u32 znew = zt;
+      Pick pk;
+      pk.topic = zt;
+      pk.slot = kNoSlot;
       if (sa > 0.0f) {
         wave_sync();
         visited += ds.occupied;
-        znew = draw_topic<LMAX>(T, ds.cap / kMapLine, P.phi + (u64)wt * K, sa, readlane_u32(r.x, t),
-                                readlane_u32(r.w, t), readlane_u32(dense_l, t), zt);
+        pk = draw_topic<LMAX>(T, ds.cap / kMapLine, P.phi + (u64)wt * K, sa, readlane_u32(r.x, t),
+                              readlane_u32(r.w, t), readlane_u32(dense_l, t));
       }
-      MapProbe pz = map_probe(T, ds.cap, znew);
-      if (!pz.found) {
-        if (map_capacity(ds.occupied + 1, K) > ds.cap) {
-          const u32 want = 2 * (ds.occupied + 1) < ds.kmax ? 2 * (ds.occupied + 1) : ds.kmax;
-          ds.cap = map_capacity(want, K);
-          regrow_doc_map(T, ds.cap, P.z, ds.b, ds.L, c0, t, znew_lane, zz);
-          pz = map_probe(T, ds.cap, znew);
-        }
-        ++ds.occupied;
-      }
-      map_inc_at(T, ds.cap, znew, pz);
+      add_topic(P, T, ds, pk, c0, t, znew_lane, zz);
+      const u32 znew = pk.topic;
       if (lane == t) znew_lane = znew;
     }
     if (valid) P.z[ds.b + i] = (u16)znew_lane;
@@ -535,15 +606,16 @@ template __global__ void k_sweep_persistent<8>(SweepParams, u32*, u32);
 
 // ---------------------------------------------------------------------------------------------
 // Block-per-document sweep for long documents: four wavefronts share one map.  Wave 0 owns the
-// map (decrement, growth, increment) and the pick; the line totals of the sparse sum are split
-// across the four waves (line j on wave j % 4) and chained in the canonical order, so the result
-// is bit-identical to the wavefront-per-document sweep.
+// map (decrement, growth, increment); wave g sums and searches line group g (lines j = g mod 4),
+// which is exactly the grouping of the canonical order, so the result is bit-identical to the
+// wavefront-per-document sweep.
 // ---------------------------------------------------------------------------------------------
 constexpr u32 kBlockWaves = 4;
 
 __global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u32 n_items) {
   extern __shared__ u32 sweep_smem[];
-  __shared__ f32 line_tot[128];
+  __shared__ f32 grp[kBlockWaves][64];
+  __shared__ u32 hit_sh[kBlockWaves], last_sh[kBlockWaves];
   __shared__ u32 sh_cap;
   __shared__ u32 sh_item;
   const u32 lane = lane_id();
@@ -595,50 +667,56 @@ __global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u
           if (lane == 0) sh_cap = ds.cap;
         }
         __syncthreads();
-        u32 znew = zt;
+        Pick pk;
+        pk.topic = zt;
+        pk.slot = kNoSlot;
+        pk.entry = 0;
         if (sa > 0.0f) {
           const u32 lines = sh_cap / kMapLine;
           const f32* prow = P.phi + (u64)wt * K;
-          for (u32 j = wave; j < lines; j += kBlockWaves) {
-            f32 sc = wave_scan_f32(slot_mass(T[j * 64 + lane], prow));
-            if (lane == 63) line_tot[j] = sc;
-          }
+          // wave g sums line group g of every column
+          f32 G = 0.0f;
+          for (u32 j = wave; j < lines; j += kBlockWaves) G = G + slot_mass(T[j * 64 + lane], prow);
+          grp[wave][lane] = G;
           __syncthreads();
-          if (wave == 0) {
-            visited += ds.occupied;
-            f32 carry = 0.0f, cw0 = 0.0f, cw1 = 0.0f;
-            for (u32 j = 0; j < lines; ++j) {
-              carry = carry + line_tot[j];
-              if (j < 64) {
-                if (lane == j) cw0 = carry;
-              } else if (lane == j - 64) {
-                cw1 = carry;
-              }
+          // every wave combines the same values, so the decision below is uniform over the block
+          const f32 g0 = grp[0][lane], g1 = grp[1][lane], g2 = grp[2][lane], g3 = grp[3][lane];
+          const f32 A = combine_groups(g0, g1, g2, g3);
+          const f32 Ps = wave_scan_f32(A);
+          const f32 sb = readlane_f32(Ps, 63);
+          if (wave == 0) visited += ds.occupied;
+          if (u24(readlane_u32(r.x, t)) * (sa + sb) > sa) {
+            const f32 tgt = u24(readlane_u32(r.w, t)) * sb;
+            f32 E;
+            const u32 ls = pick_lane(A, Ps, tgt, E);
+            const f32 base = wave == 0 ? 0.0f : (wave == 1 ? g0 : (wave == 2 ? g0 + g1 : (g0 + g1) + g2));
+            ColumnSearch cs{kNoSlot, kNoSlot};
+            f32 run = 0.0f;
+            for (u32 j = wave; j < lines; j += kBlockWaves) {
+              const f32 m = slot_mass(T[j * 64 + lane], prow);
+              run = run + m;
+              cs.step(j, m, base + run, E, tgt);
             }
-            const f32 sb = carry;
-            if (u24(readlane_u32(r.x, t)) * (sa + sb) > sa) {
-              const f32 tgt = u24(readlane_u32(r.w, t)) * sb;
-              const u32 jsel = pick_line(cw0, cw1, lines, tgt);
-              const u32 e = T[jsel * 64 + lane];
-              znew = pick_in_line(e, slot_mass(e, prow), line_total_before(cw0, cw1, jsel), tgt, zt);
-            } else {
-              znew = readlane_u32(dense_l, t);
+            if (lane == ls) {
+              hit_sh[wave] = cs.hit;
+              last_sh[wave] = cs.last;
             }
+            __syncthreads();
+            if (wave == 0) {
+              u32 js = kNoSlot;
+              for (u32 g = 0; g < kBlockWaves && js == kNoSlot; ++g) js = hit_sh[g];
+              for (u32 g = kBlockWaves; g-- > 0 && js == kNoSlot;) js = last_sh[g];
+              pk.slot = js * 64 + ls;
+              pk.entry = T[pk.slot];
+              pk.topic = map_key(pk.entry);
+            }
+          } else {
+            pk.topic = readlane_u32(dense_l, t);
           }
         }
         if (wave == 0) {
-          MapProbe pz = map_probe(T, ds.cap, znew);
-          if (!pz.found) {
-            if (map_capacity(ds.occupied + 1, K) > ds.cap) {
-              const u32 want = 2 * (ds.occupied + 1) < ds.kmax ? 2 * (ds.occupied + 1) : ds.kmax;
-              ds.cap = map_capacity(want, K);
-              regrow_doc_map(T, ds.cap, P.z, b, L, c0, t, znew_lane, zz);
-              pz = map_probe(T, ds.cap, znew);
-            }
-            ++ds.occupied;
-          }
-          map_inc_at(T, ds.cap, znew, pz);
-          if (lane == t) znew_lane = znew;
+          add_topic(P, T, ds, pk, c0, t, znew_lane, zz);
+          if (lane == t) znew_lane = pk.topic;
         }
         __syncthreads();
       }

@@ -22,6 +22,13 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// Values the whole wave agrees on, moved to scalar registers: loops and branches on them are
+// then uniform (no exec-mask bookkeeping) and their arithmetic runs on the scalar unit.
+__device__ __forceinline__ u32 uniform_u32(u32 v) { return (u32)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ u64 uniform_u64(u64 v) {
+  return (u64)uniform_u32((u32)v) | ((u64)uniform_u32((u32)(v >> 32)) << 32);
+}
+
 __device__ __forceinline__ u64 lanemask_lt() {
   u32 l = lane_id();
   return l == 0 ? 0ull : (~0ull >> (64 - l));

@@ -19,8 +19,9 @@
  *                                                                          spalias.cu:88-174
  *   z sweep      per document, sequential over tokens:                    topics.cu:36-105
  *                remove z; r = Philox(i, doc, t, SWEEP); if T_w == 0 keep z; else
- *                sigma_b via canonical scan of count*Phi_wk over map slots; sparse if
- *                u(r.x)*(sigma_a+sigma_b) > sigma_a: first slot with prefix > u(r.w)*sigma_b,
+ *                sigma_b = grouped column sums of count*Phi_wk over map slots, then one
+ *                canonical scan; sparse if u(r.x)*(sigma_a+sigma_b) > sigma_a: the slot whose
+ *                prefix interval holds u(r.w)*sigma_b (column, then slot in the column),
  *                else alias draw (r.y slot, r.z keep); add the new z.
  *   counts       n_vk = #tokens with (w = v, z = k)                         topics.cu:95, polyaurn.cu:133-141
  *
@@ -418,6 +419,15 @@ void orc_map_dec(u32* T, u32 cap, u32 key) {
 /* (topics.cuh:142-158), compute_product_cumsum (topics.cuh:163-192, with Phi[w][k] -- the     */
 /* reference reads Phi[k] only), draw_wary_search (topics.cuh:88-137: the slot whose prefix    */
 /* interval holds the target), draw_alias (topics.cuh:63-83).                                  */
+/* Summation order of the sparse mass (the reference's is its warp scan; this one is fixed so   */
+/* the device can evaluate it with one wave scan per token):                                   */
+/*   m[j][l] = count * phi[w][key] of slot j*64+l (0 when empty);                              */
+/*   G_g[l]  = m[g][l] + m[g+4][l] + ... left to right (line group g = j % 4);                 */
+/*   A[l]    = ((G_0 + G_1) + G_2) + G_3;  P = scan64(A);  sigma_b = P[63].                    */
+/* The sparse pick for target t: lane l* = first l with A[l] > 0 and P[l] > t (else the last   */
+/* with A[l] > 0); E = P[l*-1] (0 for l* = 0); base_g = G_0 + .. + G_{g-1} of column l*; then  */
+/* the first slot in (group, line) order with m > 0 and E + (base_g + run) > t, run the sum of */
+/* the group's masses up to that line (else the last slot with m > 0).                         */
 /* Returns the number of map entries that the sparse sums visited (sum over tokens).           */
 /* ------------------------------------------------------------------------------------------ */
 /* Capacity rule: the map starts sized for the document's distinct topics n0
@@ -426,7 +436,7 @@ void orc_map_dec(u32* T, u32 cap, u32 key) {
  * The reference starts from the K_d hint and regrows by 1.2x (hashmap.cuh:98, 159-207). */
 typedef struct {
   u32* T;
-  float *mass, *pre, *after;
+  float* mass;
   u32* scratch;
   u32 L, K, cap, keys, kmax;
 } OrcDoc;
@@ -437,9 +447,7 @@ static OrcDoc doc_open(const u32* z, u32 L, u32 K, u32* scratch) {
   const u32 cap_max = orc_map_capacity(L, K);
   d.T = scratch;
   d.mass = (float*)(d.T + cap_max);
-  d.pre = d.mass + cap_max;
-  d.after = d.pre + cap_max;
-  d.scratch = (u32*)(d.after + cap_max);
+  d.scratch = (u32*)(d.mass + cap_max);
   d.L = L;
   d.K = K;
   d.kmax = L < K ? L : K;
@@ -474,36 +482,61 @@ static u64 doc_pass(OrcDoc* d, u32* z, const u32* w, u32 doc, u32 iter, u32 tag,
     if (sa > 0.0f) {
       const float* prow = phi + (u64)wi * K;
       const u32 lines = d->cap / 64;
-      float carry = 0.0f;
+      /* column sums in four interleaved line groups (line j in group j % 4, lines in order) */
+      float G[4][64];
+      for (int g = 0; g < 4; ++g)
+        for (int l = 0; l < 64; ++l) G[g][l] = 0.0f;
       for (u32 j = 0; j < lines; ++j) {
         for (u32 l = 0; l < 64; ++l) {
           u32 e = d->T[j * 64 + l];
           visited += e != MAP_EMPTY;
-          d->mass[j * 64 + l] = (e == MAP_EMPTY) ? 0.0f : (float)(e & 0xFFFFu) * prow[e >> 16];
-          P[l] = d->mass[j * 64 + l];
+          float m = (e == MAP_EMPTY) ? 0.0f : (float)(e & 0xFFFFu) * prow[e >> 16];
+          d->mass[j * 64 + l] = m;
+          G[j % 4][l] = G[j % 4][l] + m;
         }
-        orc_scan64(P);
-        for (u32 l = 0; l < 64; ++l) d->pre[j * 64 + l] = carry + P[l];
-        carry = carry + P[63];
-        d->after[j] = carry;
       }
-      float sb = carry;
+      float A[64];
+      for (int l = 0; l < 64; ++l) {
+        A[l] = ((G[0][l] + G[1][l]) + G[2][l]) + G[3][l];
+        P[l] = A[l];
+      }
+      orc_scan64(P);
+      float sb = P[63];
       float u1 = unif24(r[0]);
       if (u1 * (sa + sb) > sa) {
-        /* draw_wary_search: the line holding the target, then the slot inside the line */
+        /* draw_wary_search restated two-level: the lane (column) whose prefix holds the target,
+         * then the slot inside that column */
         float t = unif24(r[3]) * sb;
-        u32 j = 0;
-        while (j + 1 < lines && !(d->after[j] > t)) ++j;
-        long pick = -1, last = -1;
-        for (u32 l = 0; l < 64; ++l) {
-          float m = d->mass[j * 64 + l];
-          if (m > 0.0f) {
-            last = l;
-            if (pick < 0 && d->pre[j * 64 + l] > t) pick = l;
+        long ls = -1, lpos = -1;
+        for (int l = 0; l < 64; ++l) {
+          if (A[l] > 0.0f) {
+            lpos = l;
+            if (ls < 0 && P[l] > t) ls = l;
           }
         }
-        if (pick < 0) pick = last;
-        if (pick >= 0) znew = d->T[j * 64 + (u32)pick] >> 16;
+        if (ls < 0) ls = lpos;
+        if (ls >= 0) {
+          float E = ls > 0 ? P[ls - 1] : 0.0f;
+          float base[4];
+          base[0] = 0.0f;
+          base[1] = G[0][ls];
+          base[2] = G[0][ls] + G[1][ls];
+          base[3] = (G[0][ls] + G[1][ls]) + G[2][ls];
+          long hit = -1, last = -1;
+          for (u32 g = 0; g < 4; ++g) {
+            float run = 0.0f;
+            for (u32 j = g; j < lines; j += 4) {
+              float m = d->mass[j * 64 + (u32)ls];
+              run = run + m;
+              if (m > 0.0f) {
+                if (hit < 0 && E + (base[g] + run) > t) hit = (long)j;
+                last = (long)j;
+              }
+            }
+          }
+          long js = hit >= 0 ? hit : last;
+          znew = d->T[(u32)js * 64 + (u32)ls] >> 16;
+        }
       } else {
         znew = draw_alias(alias + (u64)wi * K, K, r[1], r[2]);
       }
