@@ -252,9 +252,9 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
 //   G_g = column sums of line group g = j % 4, lines left to right; A = ((G_0 + G_1) + G_2) + G_3;
 //   P = canonical scan of A; sigma_b = P[63];
 //   sparse if u(r.x) * (sigma_a + sigma_b) > sigma_a: target t = u(r.w) * sigma_b, lane l* = the
-//   first with A > 0 and P > t (else the last with A > 0), E = P[l* - 1]; in column l* the first
-//   slot in (group, line) order with m > 0 and E + (base_g + run) > t, base_g = G_0 + .. + G_{g-1}
-//   and run the group's masses up to that line (else the last slot with m > 0);
+//   first with A > 0 and P > t (else the last with A > 0), E = P[l* - 1]; Q = canonical scan of
+//   column l*'s masses in line order (64 lines per scan, carry = total of the earlier scans);
+//   the line is the first with m > 0 and E + (carry + Q_j) > t (else the last with m > 0);
 //   otherwise the alias table of w (slot from r.y, keep test on r.z); add the new topic.
 // Each lane sums its own column, so a token costs one wave scan however many lines the map has.
 // ---------------------------------------------------------------------------------------------
@@ -369,18 +369,25 @@ __device__ __forceinline__ u32 pick_lane(f32 A, f32 P, f32 tgt, f32& E) {
   return ls;
 }
 
-// Running state of the in-column search (every lane searches its own column; lane l*'s result is
-// the one used).
-struct ColumnSearch {
-  u32 hit;   // first line with m > 0 and E + (base + run) > t, or kNoSlot
-  u32 last;  // last line with m > 0, or kNoSlot
-  __device__ __forceinline__ void step(u32 j, f32 m, f32 cum, f32 E, f32 tgt) {
-    const bool pj = m > 0.0f;
-    if (pj && hit == kNoSlot && E + cum > tgt) hit = j;
-    if (pj) last = j;
+// In-column search over lines read transposed (lane k holds line c0 + k of column ls), one scan
+// per 64 lines.  Returns the picked line.
+__device__ __forceinline__ u32 column_pick(const u32* T, u32 lines, u32 ls, const f32* __restrict__ prow, f32 E,
+                                           f32 tgt) {
+  const u32 lane = lane_id();
+  f32 carry = 0.0f;
+  u32 last = 0;
+  for (u32 c0 = 0; c0 < lines; c0 += 64) {
+    const u32 j = c0 + lane;
+    const f32 m = slot_mass(j < lines ? T[j * 64 + ls] : kMapEmpty, prow);
+    const f32 Q = wave_scan_f32(m);
+    const u64 hit = __ballot(m > 0.0f && E + (carry + Q) > tgt);
+    if (hit) return c0 + first_lane(hit);
+    const u64 pos = __ballot(m > 0.0f);
+    if (pos) last = c0 + 63u - (u32)__clzll((long long)pos);
+    carry = carry + readlane_f32(Q, 63);
   }
-  __device__ __forceinline__ u32 line() const { return hit != kNoSlot ? hit : last; }
-};
+  return last;
+}
 
 // Sparse/dense pick with every line of the map held in registers (NL >= lines): all phi gathers
 // of the token are issued before any arithmetic on them.
@@ -411,18 +418,25 @@ __device__ __forceinline__ Pick draw_topic_regs(const u32* T, u32 lines, const f
   const f32 tgt = u24(rw) * sb;
   f32 E;
   const u32 ls = pick_lane(A, P, tgt, E);
-  const f32 base[4] = {0.0f, G[0], G[0] + G[1], (G[0] + G[1]) + G[2]};
-  ColumnSearch cs{kNoSlot, kNoSlot};
+  // every lane scans its own column in registers with the canonical network (NL <= 16 lines lie
+  // in the network's first row: offsets 1, 2, 4, 8, entries below the offset kept); lane ls's
+  // result is the one used
+  f32 q[NL];
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    f32 run = 0.0f;
+  for (int j = 0; j < NL; ++j) q[j] = m[j];
 #pragma unroll
-    for (int j = g; j < NL; j += 4) {
-      run = run + m[j];
-      cs.step((u32)j, m[j], base[g] + run, E, tgt);
-    }
+  for (int off = 1; off < 16; off <<= 1)
+#pragma unroll
+    for (int j = NL - 1; j >= off; --j) q[j] = q[j] + q[j - off];
+  u32 hit = kNoSlot, last = 0;
+#pragma unroll
+  for (int j = NL - 1; j >= 0; --j) {
+    if (m[j] > 0.0f && E + q[j] > tgt) hit = (u32)j;  // carry 0: E + (0 + q) == E + q
   }
-  const u32 js = readlane_u32(cs.line(), ls);
+#pragma unroll
+  for (int j = 0; j < NL; ++j)
+    if (m[j] > 0.0f) last = (u32)j;
+  const u32 js = readlane_u32(hit != kNoSlot ? hit : last, ls);
   u32 es = e[0];
 #pragma unroll
   for (int j = 1; j < NL; ++j)
@@ -433,24 +447,28 @@ __device__ __forceinline__ Pick draw_topic_regs(const u32* T, u32 lines, const f
   return pk;
 }
 
-// Sparse/dense pick for any number of lines: a first pass for the column sums, and for a sparse
-// draw a second pass that recomputes the same masses (same operations, so the same values)
-// for the in-column search.
+// Sparse/dense pick for any number of lines: one pass for the column sums, then for a sparse draw
+// the in-column search over column l* read transposed.
 __device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa,
                                                 u32 rx, u32 rw, u32 dense) {
   const u32 lane = lane_id();
   f32 G0 = 0.0f, G1 = 0.0f, G2 = 0.0f, G3 = 0.0f;
-  for (u32 j = 0; j < lines; j += 4) {
-    const u32 e0 = T[j * 64 + lane];
-    const u32 e1 = j + 1 < lines ? T[(j + 1) * 64 + lane] : kMapEmpty;
-    const u32 e2 = j + 2 < lines ? T[(j + 2) * 64 + lane] : kMapEmpty;
-    const u32 e3 = j + 3 < lines ? T[(j + 3) * 64 + lane] : kMapEmpty;
-    const f32 m0 = slot_mass(e0, prow), m1 = slot_mass(e1, prow), m2 = slot_mass(e2, prow),
-              m3 = slot_mass(e3, prow);
-    G0 = G0 + m0;
-    G1 = G1 + m1;
-    G2 = G2 + m2;
-    G3 = G3 + m3;
+  // eight lines per step, so eight gathers are in flight at once
+  for (u32 j = 0; j < lines; j += 8) {
+    u32 e[8];
+#pragma unroll
+    for (u32 k = 0; k < 8; ++k) e[k] = j + k < lines ? T[(j + k) * 64 + lane] : kMapEmpty;
+    f32 m[8];
+#pragma unroll
+    for (u32 k = 0; k < 8; ++k) m[k] = slot_mass(e[k], prow);
+    G0 = G0 + m[0];
+    G1 = G1 + m[1];
+    G2 = G2 + m[2];
+    G3 = G3 + m[3];
+    G0 = G0 + m[4];
+    G1 = G1 + m[5];
+    G2 = G2 + m[6];
+    G3 = G3 + m[7];
   }
   const f32 A = combine_groups(G0, G1, G2, G3);
   const f32 P = wave_scan_f32(A);
@@ -463,27 +481,15 @@ __device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const f
   const f32 tgt = u24(rw) * sb;
   f32 E;
   const u32 ls = pick_lane(A, P, tgt, E);
-  const f32 base[4] = {0.0f, G0, G0 + G1, (G0 + G1) + G2};
-  ColumnSearch cs{kNoSlot, kNoSlot};
-#pragma unroll
-  for (u32 g = 0; g < 4; ++g) {
-    f32 run = 0.0f;
-    for (u32 j = g; j < lines; j += 4) {
-      const f32 m = slot_mass(T[j * 64 + lane], prow);
-      run = run + m;
-      cs.step(j, m, base[g] + run, E, tgt);
-    }
-  }
-  const u32 js = readlane_u32(cs.line(), ls);
-  pk.slot = js * 64 + ls;
-  pk.entry = readlane_u32(T[js * 64 + lane], ls);
+  pk.slot = column_pick(T, lines, ls, prow, E, tgt) * 64 + ls;
+  pk.entry = T[pk.slot];
   pk.topic = map_key(pk.entry);
   return pk;
 }
 
 // Sparse/dense pick for one token (sigma_a > 0).  The register variant is chosen by the map's
 // current line count, up to LMAX lines (LMAX caps the registers of a kernel); larger maps use
-// the two-pass form.  All variants evaluate the same operations in the same order.
+// the line-loop form.  All variants evaluate the same operations in the same order.
 template <int LMAX>
 __device__ __forceinline__ Pick draw_topic(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa, u32 rx,
                                            u32 rw, u32 dense) {
@@ -606,16 +612,15 @@ template __global__ void k_sweep_persistent<8>(SweepParams, u32*, u32);
 
 // ---------------------------------------------------------------------------------------------
 // Block-per-document sweep for long documents: four wavefronts share one map.  Wave 0 owns the
-// map (decrement, growth, increment); wave g sums and searches line group g (lines j = g mod 4),
-// which is exactly the grouping of the canonical order, so the result is bit-identical to the
-// wavefront-per-document sweep.
+// map (decrement, growth, increment) and the in-column search; wave g sums line group g (lines
+// j = g mod 4), which is exactly the grouping of the canonical order, so the result is
+// bit-identical to the wavefront-per-document sweep.
 // ---------------------------------------------------------------------------------------------
 constexpr u32 kBlockWaves = 4;
 
 __global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u32 n_items) {
   extern __shared__ u32 sweep_smem[];
   __shared__ f32 grp[kBlockWaves][64];
-  __shared__ u32 hit_sh[kBlockWaves], last_sh[kBlockWaves];
   __shared__ u32 sh_cap;
   __shared__ u32 sh_item;
   const u32 lane = lane_id();
@@ -674,44 +679,39 @@ __global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u
         if (sa > 0.0f) {
           const u32 lines = sh_cap / kMapLine;
           const f32* prow = P.phi + (u64)wt * K;
-          // wave g sums line group g of every column
+          // wave g sums line group g of every column (four of its lines per step, gathers in
+          // flight together, added in line order)
           f32 G = 0.0f;
-          for (u32 j = wave; j < lines; j += kBlockWaves) G = G + slot_mass(T[j * 64 + lane], prow);
+          for (u32 j = wave; j < lines; j += 4 * kBlockWaves) {
+            u32 e[4];
+#pragma unroll
+            for (u32 k = 0; k < 4; ++k) {
+              const u32 jj = j + k * kBlockWaves;
+              e[k] = jj < lines ? T[jj * 64 + lane] : kMapEmpty;
+            }
+            f32 m[4];
+#pragma unroll
+            for (u32 k = 0; k < 4; ++k) m[k] = slot_mass(e[k], prow);
+#pragma unroll
+            for (u32 k = 0; k < 4; ++k) G = G + m[k];
+          }
           grp[wave][lane] = G;
           __syncthreads();
-          // every wave combines the same values, so the decision below is uniform over the block
-          const f32 g0 = grp[0][lane], g1 = grp[1][lane], g2 = grp[2][lane], g3 = grp[3][lane];
-          const f32 A = combine_groups(g0, g1, g2, g3);
-          const f32 Ps = wave_scan_f32(A);
-          const f32 sb = readlane_f32(Ps, 63);
-          if (wave == 0) visited += ds.occupied;
-          if (u24(readlane_u32(r.x, t)) * (sa + sb) > sa) {
-            const f32 tgt = u24(readlane_u32(r.w, t)) * sb;
-            f32 E;
-            const u32 ls = pick_lane(A, Ps, tgt, E);
-            const f32 base = wave == 0 ? 0.0f : (wave == 1 ? g0 : (wave == 2 ? g0 + g1 : (g0 + g1) + g2));
-            ColumnSearch cs{kNoSlot, kNoSlot};
-            f32 run = 0.0f;
-            for (u32 j = wave; j < lines; j += kBlockWaves) {
-              const f32 m = slot_mass(T[j * 64 + lane], prow);
-              run = run + m;
-              cs.step(j, m, base + run, E, tgt);
-            }
-            if (lane == ls) {
-              hit_sh[wave] = cs.hit;
-              last_sh[wave] = cs.last;
-            }
-            __syncthreads();
-            if (wave == 0) {
-              u32 js = kNoSlot;
-              for (u32 g = 0; g < kBlockWaves && js == kNoSlot; ++g) js = hit_sh[g];
-              for (u32 g = kBlockWaves; g-- > 0 && js == kNoSlot;) js = last_sh[g];
-              pk.slot = js * 64 + ls;
+          if (wave == 0) {
+            const f32 A = combine_groups(grp[0][lane], grp[1][lane], grp[2][lane], grp[3][lane]);
+            const f32 Ps = wave_scan_f32(A);
+            const f32 sb = readlane_f32(Ps, 63);
+            visited += ds.occupied;
+            if (u24(readlane_u32(r.x, t)) * (sa + sb) > sa) {
+              const f32 tgt = u24(readlane_u32(r.w, t)) * sb;
+              f32 E;
+              const u32 ls = pick_lane(A, Ps, tgt, E);
+              pk.slot = column_pick(T, lines, ls, prow, E, tgt) * 64 + ls;
               pk.entry = T[pk.slot];
               pk.topic = map_key(pk.entry);
+            } else {
+              pk.topic = readlane_u32(dense_l, t);
             }
-          } else {
-            pk.topic = readlane_u32(dense_l, t);
           }
         }
         if (wave == 0) {

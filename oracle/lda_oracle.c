@@ -425,9 +425,9 @@ void orc_map_dec(u32* T, u32 cap, u32 key) {
 /*   G_g[l]  = m[g][l] + m[g+4][l] + ... left to right (line group g = j % 4);                 */
 /*   A[l]    = ((G_0 + G_1) + G_2) + G_3;  P = scan64(A);  sigma_b = P[63].                    */
 /* The sparse pick for target t: lane l* = first l with A[l] > 0 and P[l] > t (else the last   */
-/* with A[l] > 0); E = P[l*-1] (0 for l* = 0); base_g = G_0 + .. + G_{g-1} of column l*; then  */
-/* the first slot in (group, line) order with m > 0 and E + (base_g + run) > t, run the sum of */
-/* the group's masses up to that line (else the last slot with m > 0).                         */
+/* with A[l] > 0); E = P[l*-1] (0 for l* = 0); Q = scan64 of column l*'s masses m[j][l*] in  */
+/* line order (64 lines per scan, carry = running total of earlier scans); the line is the     */
+/* first with m > 0 and E + (carry + Q_j) > t (else the last line with m > 0).                 */
 /* Returns the number of map entries that the sparse sums visited (sum over tokens).           */
 /* ------------------------------------------------------------------------------------------ */
 /* Capacity rule: the map starts sized for the document's distinct topics n0
@@ -516,23 +516,23 @@ static u64 doc_pass(OrcDoc* d, u32* z, const u32* w, u32 doc, u32 iter, u32 tag,
         }
         if (ls < 0) ls = lpos;
         if (ls >= 0) {
+          /* in column l*: canonical scans of its masses in line order, 64 lines at a time */
           float E = ls > 0 ? P[ls - 1] : 0.0f;
-          float base[4];
-          base[0] = 0.0f;
-          base[1] = G[0][ls];
-          base[2] = G[0][ls] + G[1][ls];
-          base[3] = (G[0][ls] + G[1][ls]) + G[2][ls];
+          float carry = 0.0f;
           long hit = -1, last = -1;
-          for (u32 g = 0; g < 4; ++g) {
-            float run = 0.0f;
-            for (u32 j = g; j < lines; j += 4) {
-              float m = d->mass[j * 64 + (u32)ls];
-              run = run + m;
-              if (m > 0.0f) {
-                if (hit < 0 && E + (base[g] + run) > t) hit = (long)j;
-                last = (long)j;
+          for (u32 c0 = 0; c0 < lines && hit < 0; c0 += 64) {
+            float Q[64];
+            for (u32 k = 0; k < 64; ++k) Q[k] = c0 + k < lines ? d->mass[(c0 + k) * 64 + (u32)ls] : 0.0f;
+            float mc[64];
+            memcpy(mc, Q, sizeof(Q));
+            orc_scan64(Q);
+            for (u32 k = 0; k < 64 && c0 + k < lines; ++k) {
+              if (mc[k] > 0.0f) {
+                if (hit < 0 && E + (carry + Q[k]) > t) hit = (long)(c0 + k);
+                last = (long)(c0 + k);
               }
             }
+            carry = carry + Q[63];
           }
           long js = hit >= 0 ? hit : last;
           znew = d->T[(u32)js * 64 + (u32)ls] >> 16;
