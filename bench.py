@@ -163,10 +163,10 @@ def serve_bench(rc, spec, n_req, n_serial):
     phi = rc.phi()
     d_len, arrival, w = corpus.requests(phi, n_req, median=300.0, sigma=1.2, lo=10, hi=5000, rate=2000.0, seed=4)
     del phi
-    with FoldInServer(hbm_budget_bytes=16 << 30, sweeps=50, n_streams=8, max_slots=16384) as srv:
+    with FoldInServer(hbm_budget_bytes=16 << 30, sweeps=50, waves_per_cu=2, max_slots=16384) as srv:
         # warm the kernels once outside the measurement
         run_serial(srv, d_len[:2], w[:int(d_len[:2].sum())], id_base=1 << 30)
-        mixed = run_stream(srv, d_len, arrival, w, window=0.0005)
+        mixed = run_stream(srv, d_len, arrival, w)
         n_s = min(n_serial, n_req)
         serial = run_serial(srv, d_len[:n_s], w[:int(d_len[:n_s].sum())])
     lat = mixed["latency_s"]

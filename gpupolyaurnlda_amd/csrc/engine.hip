@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <string>
 #include <vector>
 
@@ -1151,38 +1152,70 @@ extern "C" int gpulda_corpus_release(void) {
 // ---- fold-in server ----------------------------------------------------------------------------
 // Continuous batching against the resident model (phi, sigma_a and alias tables stay in HBM):
 //   submit   admits a request if a slot and a span of the token arena (bounded by the HBM budget)
-//            are free, copies its words to pinned staging; otherwise the caller retries;
-//   flush    launches every admitted, not yet launched request as one batch on the next of
-//            n_streams streams (batches on different streams overlap on the device);
-//   poll     reports requests whose completion flag the kernel raised (pinned, host-visible);
+//            are free and stages its words in host-visible memory; otherwise the caller retries;
+//   flush    publishes every admitted, not yet published request to the request ring;
+//   poll     reports requests whose completion flag the device raised;
 //   result   returns a finished request's topics and frees its slot and span.
-// A request's randomness is keyed by its request id, so its result is the same whatever it is
-// batched with.
+// A pool of worker wavefronts (k_serve_worker, waves_per_cu per CU) stays resident and takes
+// requests from the ring as soon as they are published, one wavefront per request, so requests
+// that arrive while others run start at once and overlap on the device.  A pool stops claiming
+// after kPoolLifetime seconds; the host starts a younger pool on a second stream once the running
+// one is half way through, and again whenever none is running while requests are outstanding.
+// A request's randomness is keyed by its request id, so its result is the same whatever runs
+// beside it.
 namespace {
+constexpr double kPoolLifetime = 0.5;  // seconds a pool keeps claiming requests
+
+struct ServePool {
+  hipStream_t st = nullptr;
+  double launched = 0.0;  // host seconds
+  bool active = false;
+};
+
 struct ServeState {
   bool open = false;
   u32 sweeps = 50;
   u32 n_slots = 0;
+  u32 waves_per_cu = 2;
   u64 arena = 0;            // tokens
-  u32* d_w = nullptr;       // arena word ids (device)
-  u16* d_z = nullptr;       // arena topics (device)
+  u32* d_w = nullptr;       // arena word ids (copied in by DMA at each flush)
+  u16* d_z = nullptr;       // arena topics
+  u32* d_rec = nullptr;     // per slot request record (kServeRecWords, one line each)
+  u32* d_ring = nullptr;    // request ring (n_slots entries)
+  u32* d_ctl = nullptr;     // [0] tail, [1] head, [2] shutdown
+  u32* d_meta = nullptr;    // records as the host sends them (2 * n_slots, a ring)
   u32* h_w = nullptr;       // pinned staging of the arena word ids
-  u16* h_z = nullptr;       // pinned, host-visible final topics
-  u64* h_off = nullptr;     // pinned slot -> span offset
-  u32* h_len = nullptr;     // pinned slot -> span length
-  u32* h_rid = nullptr;     // pinned slot -> request id
-  u32* h_done = nullptr;    // pinned slot -> completion flag
-  u32* h_batch = nullptr;   // pinned batch lists (n_slots entries, used as a ring)
-  u32 batch_head = 0;
+  u16* h_z = nullptr;       // host-visible final topics (arena offsets), written by the device
+  u64* h_off = nullptr;     // slot -> span offset
+  u32* h_len = nullptr;     // slot -> span length
+  u32* h_rid = nullptr;     // slot -> request id
+  u32* h_done = nullptr;    // host-visible slot -> completion flag (the device's fingerprint)
+  u32* h_drec = nullptr;    // host-visible slot -> what the serving wave read and wrote
+  u32* h_meta = nullptr;    // pinned request records of the flushes (2 * n_slots records, a ring)
+  u32 meta_head = 0;        // next free record
+  std::vector<u32> expect;  // slot -> fingerprint of the staged request
+  std::vector<u32> wsum;    // slot -> word sum of the staged request
+  std::vector<u32> pub_at;  // slot -> ring index it was published at
+  u32 tail = 0;
+  u64 lifetime_ticks = 0;
   std::vector<u32> free_slots;
   std::vector<std::pair<u64, u64>> free_spans;  // (offset, length), sorted by offset
-  std::vector<u32> pending;   // admitted, not launched
-  std::vector<u32> inflight;  // launched, not reported
-  std::vector<hipStream_t> streams;
-  u32 next_stream = 0;
+  std::vector<u32> pending;   // admitted, not published
+  std::vector<u32> inflight;  // published, not reported
+  ServePool pools[2];
+  hipStream_t feed = nullptr;
   u64 lds = 0;
 };
 ServeState SV;
+
+double host_seconds() {
+  timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+// Spans start on 32-token (128-byte) boundaries, so no two requests share a cache line of words.
+u64 serve_span_tokens(u32 L) { return ((u64)L + 31) & ~(u64)31; }
 
 int serve_alloc_span(u64 len, u64* off) {
   for (size_t i = 0; i < SV.free_spans.size(); ++i) {
@@ -1213,6 +1246,9 @@ void serve_free_span(u64 off, u64 len) {
   }
 }
 
+// Pinned host memory mapped into the device address space.  Kernels read it only at their start
+// (a launch makes the host's earlier stores visible); the device writes it only with system-scope
+// release stores, which reach host memory before the store that follows them.
 template <class T>
 int host_alloc(T** p, u64 n) {
   *p = nullptr;
@@ -1227,15 +1263,37 @@ T* dev_view(T* h) {
 }
 
 void serve_release() {
-  for (hipStream_t st : SV.streams) (void)hipStreamSynchronize(st);
-  for (hipStream_t st : SV.streams) (void)hipStreamDestroy(st);
-  SV.streams.clear();
-  dfree(SV.d_w);
-  dfree(SV.d_z);
+  if (SV.d_ctl && SV.feed) {
+    k_fill_u32<<<1, 64, 0, SV.feed>>>(SV.d_ctl + 2, 1, 1u);  // shutdown: every pool wave leaves
+    (void)hipStreamSynchronize(SV.feed);
+  }
+  for (ServePool& pl : SV.pools)
+    if (pl.st) {
+      (void)hipStreamSynchronize(pl.st);
+      (void)hipStreamDestroy(pl.st);
+    }
+  if (SV.feed) (void)hipStreamDestroy(SV.feed);
+  for (void* d : {(void*)SV.d_w, (void*)SV.d_z, (void*)SV.d_rec, (void*)SV.d_ring, (void*)SV.d_ctl, (void*)SV.d_meta})
+    if (d) (void)hipFree(d);
   for (void* h : {(void*)SV.h_w, (void*)SV.h_z, (void*)SV.h_off, (void*)SV.h_len, (void*)SV.h_rid, (void*)SV.h_done,
-                  (void*)SV.h_batch})
+                  (void*)SV.h_meta, (void*)SV.h_drec})
     if (h) (void)hipHostFree(h);
   SV = ServeState();
+}
+
+ServeRing serve_ring() {
+  ServeRing R;
+  R.ring = SV.d_ring;
+  R.ring_size = SV.n_slots;
+  R.tail = SV.d_ctl;
+  R.head = SV.d_ctl + 1;
+  R.shutdown = SV.d_ctl + 2;
+  R.lifetime_ticks = SV.lifetime_ticks;
+  R.rec = SV.d_rec;
+  R.arena = SV.arena;
+  R.V = S.V;
+  R.done_rec = dev_view(SV.h_drec);
+  return R;
 }
 
 SweepParams serve_params() {
@@ -1257,34 +1315,86 @@ SweepParams serve_params() {
   P.visited = nullptr;
   return P;
 }
+
+int launch_pool(ServePool& pl) {
+  FoldParams F;
+  F.slots = nullptr;
+  F.n_slots = SV.n_slots;
+  F.span_off = nullptr;  // the worker reads the request records instead
+  F.span_len = nullptr;
+  F.req_id = nullptr;
+  F.sweeps = SV.sweeps;
+  F.z_out = dev_view(SV.h_z);
+  F.done = dev_view(SV.h_done);
+  F.done_value = 0;
+  if (!F.done || !F.z_out || !dev_view(SV.h_drec)) return fail("host-mapped result buffers unavailable");
+  const u32 waves = std::max<u32>(1, (u32)S.cu_count * SV.waves_per_cu);
+  k_serve_worker<<<waves, 64, SV.lds, pl.st>>>(serve_params(), F, serve_ring());
+  HCK(hipGetLastError());
+  pl.launched = host_seconds();
+  pl.active = true;
+  return 0;
+}
+
+// Keeps a claiming pool on the device while requests are outstanding.
+int serve_keep_pool() {
+  const double now = host_seconds();
+  double newest = -1.0;
+  for (ServePool& pl : SV.pools) {
+    if (pl.active && hipStreamQuery(pl.st) == hipSuccess) pl.active = false;
+    if (pl.active) newest = std::max(newest, pl.launched);
+  }
+  if (SV.inflight.empty()) return 0;
+  const bool none = newest < 0.0;
+  const bool ageing = !none && now - newest > 0.5 * kPoolLifetime;
+  if (!none && !ageing) return 0;
+  for (ServePool& pl : SV.pools)
+    if (!pl.active) return launch_pool(pl);
+  return 0;  // both pools still running: the younger one is claiming
+}
 }  // namespace
 
-extern "C" int gpulda_serve_open(uint64_t hbm_budget_bytes, uint32_t sweeps, uint32_t n_streams, uint32_t max_slots) {
+extern "C" int gpulda_serve_open(uint64_t hbm_budget_bytes, uint32_t sweeps, uint32_t waves_per_cu, uint32_t max_slots) {
   CK(require_ready());
   if (S.iter == 0) return fail("serve_open needs a trained model (run at least one Phi step)");
   if (SV.open) serve_release();
   SV.sweeps = sweeps;
   SV.n_slots = max_slots ? max_slots : 65536;
-  SV.arena = std::max<u64>(hbm_budget_bytes / (sizeof(u32) + sizeof(u16)), 1);
+  SV.waves_per_cu = std::max<u32>(1, waves_per_cu);
+  SV.arena = std::max<u64>(hbm_budget_bytes / (sizeof(u32) + sizeof(u16)), 1);  // words and topics per token
   SV.lds = ((u64)map_capacity(S.K, S.K) + (S.K + 31) / 32) * 4;
   if (SV.lds > (u64)max_lds_per_block()) return fail("doc-topic map exceeds LDS");
+  int dev = 0, rate_khz = 0;
+  HCK(hipGetDevice(&dev));
+  HCK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+  if (rate_khz <= 0) return fail("device wall clock rate unavailable");
+  SV.lifetime_ticks = (u64)(kPoolLifetime * 1000.0 * (double)rate_khz);
   int rc = 0;
-  if ((rc = dalloc(&SV.d_w, SV.arena)) || (rc = dalloc(&SV.d_z, SV.arena)) || (rc = host_alloc(&SV.h_w, SV.arena)) ||
+  if ((rc = dalloc(&SV.d_w, SV.arena)) || (rc = dalloc(&SV.d_z, SV.arena)) ||
+      (rc = dalloc(&SV.d_rec, (u64)kServeRecWords * SV.n_slots)) || (rc = dalloc(&SV.d_ring, SV.n_slots)) ||
+      (rc = dalloc(&SV.d_ctl, 4)) || (rc = dalloc(&SV.d_meta, 2ull * kServeMetaWords * SV.n_slots)) ||
+      (rc = host_alloc(&SV.h_w, SV.arena)) ||
       (rc = host_alloc(&SV.h_z, SV.arena)) || (rc = host_alloc(&SV.h_off, SV.n_slots)) ||
       (rc = host_alloc(&SV.h_len, SV.n_slots)) || (rc = host_alloc(&SV.h_rid, SV.n_slots)) ||
-      (rc = host_alloc(&SV.h_done, SV.n_slots)) || (rc = host_alloc(&SV.h_batch, 2ull * SV.n_slots))) {
+      (rc = host_alloc(&SV.h_done, SV.n_slots)) || (rc = host_alloc(&SV.h_meta, 2ull * kServeMetaWords * SV.n_slots)) ||
+      (rc = host_alloc(&SV.h_drec, (u64)kServeRecWords * SV.n_slots))) {
     serve_release();
     return rc;
   }
-  for (u32 i = 0; i < std::max<u32>(1, n_streams); ++i) {
-    hipStream_t st;
-    HCK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    SV.streams.push_back(st);
-  }
+  HCK(hipMemset(SV.d_ring, 0, SV.n_slots * sizeof(u32)));
+  HCK(hipMemset(SV.d_rec, 0, (u64)kServeRecWords * SV.n_slots * sizeof(u32)));
+  HCK(hipMemset(SV.d_ctl, 0, 4 * sizeof(u32)));
+  for (ServePool& pl : SV.pools) HCK(hipStreamCreateWithFlags(&pl.st, hipStreamNonBlocking));
+  HCK(hipStreamCreateWithFlags(&SV.feed, hipStreamNonBlocking));
+  HCK(hipDeviceSynchronize());
   SV.free_slots.resize(SV.n_slots);
   for (u32 i = 0; i < SV.n_slots; ++i) SV.free_slots[i] = SV.n_slots - 1 - i;
+  SV.expect.assign(SV.n_slots, 0);
+  SV.wsum.assign(SV.n_slots, 0);
+  SV.pub_at.assign(SV.n_slots, 0);
   SV.free_spans = {{0, SV.arena}};
   memset(SV.h_done, 0, SV.n_slots * sizeof(u32));
+  memset(SV.h_drec, 0, (u64)kServeRecWords * SV.n_slots * sizeof(u32));
   SV.open = true;
   return 0;
 }
@@ -1293,17 +1403,18 @@ extern "C" int gpulda_serve_open(uint64_t hbm_budget_bytes, uint32_t sweeps, uin
 extern "C" int gpulda_serve_submit(const uint32_t* w, uint32_t L, uint32_t request_id, uint32_t* slot) {
   if (!SV.open) return fail("server not open");
   if (L == 0 || L > kMaxDocLen) return fail("request length %u outside 1..%u", L, kMaxDocLen);
+  for (u32 i = 0; i < L; ++i)
+    if (w[i] >= S.V) return fail("request word id %u >= V", w[i]);
   if (SV.free_slots.empty()) return 1;
   u64 off = 0;
-  if (serve_alloc_span(L, &off)) return 1;
-  for (u32 i = 0; i < L; ++i)
-    if (w[i] >= S.V) {
-      serve_free_span(off, L);
-      return fail("request word id %u >= V", w[i]);
-    }
+  if (serve_alloc_span(serve_span_tokens(L), &off)) return 1;
   u32 s = SV.free_slots.back();
   SV.free_slots.pop_back();
   memcpy(SV.h_w + off, w, (u64)L * sizeof(u32));
+  u32 word_sum = 0;
+  for (u32 i = 0; i < L; ++i) word_sum += w[i] * (2u * i + 1u);
+  SV.expect[s] = serve_fingerprint(word_sum, L, request_id);
+  SV.wsum[s] = word_sum;
   SV.h_off[s] = off;
   SV.h_len[s] = L;
   SV.h_rid[s] = request_id;
@@ -1315,40 +1426,44 @@ extern "C" int gpulda_serve_submit(const uint32_t* w, uint32_t L, uint32_t reque
 
 extern "C" int gpulda_serve_flush(void) {
   if (!SV.open) return fail("server not open");
-  if (SV.pending.empty()) return 0;
-  hipStream_t st = SV.streams[SV.next_stream];
-  SV.next_stream = (SV.next_stream + 1) % (u32)SV.streams.size();
-  const u32 nb = (u32)SV.pending.size();
-  // ring of 2 * n_slots list entries: a wrap can only reuse entries of requests already finished,
-  // because at most n_slots requests are admitted at a time
-  if (SV.batch_head + nb > 2 * SV.n_slots) SV.batch_head = 0;
-  const u32 list_at = SV.batch_head;
-  u32* list = SV.h_batch + list_at;
-  SV.batch_head += nb;
-  for (u32 i = 0; i < nb; ++i) {
-    const u32 sl = SV.pending[i];
-    list[i] = sl;
-    HCK(hipMemcpyAsync(SV.d_w + SV.h_off[sl], SV.h_w + SV.h_off[sl], (u64)SV.h_len[sl] * sizeof(u32),
-                       hipMemcpyHostToDevice, st));
+  if (!SV.pending.empty()) {
+    const u32 n = (u32)SV.pending.size();
+    // records live in a ring of 2 * n_slots; before it wraps onto records whose copies may still
+    // be pending, wait for the feeder stream
+    if (SV.meta_head + n > 2 * SV.n_slots) {
+      HCK(hipStreamSynchronize(SV.feed));
+      SV.meta_head = 0;
+    }
+    u32* rec = SV.h_meta + (u64)SV.meta_head * kServeMetaWords;
+    for (u32 i = 0; i < n; ++i) {
+      const u32 sl = SV.pending[i];
+      const u64 off = SV.h_off[sl];
+      const u32 L = SV.h_len[sl];
+      rec[i * kServeMetaWords + 0] = sl;
+      rec[i * kServeMetaWords + 1] = L;
+      rec[i * kServeMetaWords + 2] = SV.h_rid[sl];
+      rec[i * kServeMetaWords + 3] = (u32)off;
+      rec[i * kServeMetaWords + 4] = (u32)(off >> 32);
+      rec[i * kServeMetaWords + 5] = SV.wsum[sl];
+      SV.pub_at[sl] = SV.tail + i;
+      HCK(hipMemcpyAsync(SV.d_w + off, SV.h_w + off, (u64)L * sizeof(u32), hipMemcpyHostToDevice, SV.feed));
+    }
+    u32* drec = SV.d_meta + (u64)SV.meta_head * kServeMetaWords;
+    HCK(hipMemcpyAsync(drec, rec, (u64)n * kServeMetaWords * sizeof(u32), hipMemcpyHostToDevice, SV.feed));
+    ServeFeed Fd;
+    Fd.meta = drec;
+    Fd.base = SV.tail;
+    Fd.count = n;
+    // at most n_slots requests are admitted at a time, so a ring entry is taken before the ring
+    // wraps onto it
+    k_serve_feed<<<(n + 63) / 64, 64, 0, SV.feed>>>(Fd, serve_ring());
+    HCK(hipGetLastError());
+    SV.meta_head += n;
+    SV.tail += n;
+    SV.inflight.insert(SV.inflight.end(), SV.pending.begin(), SV.pending.end());
+    SV.pending.clear();
   }
-  FoldParams F;
-  u32* batch_dev = dev_view(SV.h_batch);  // device view of the pinned base, then the offset
-  F.slots = batch_dev ? batch_dev + list_at : nullptr;
-  F.n_slots = SV.n_slots;
-  F.span_off = dev_view(SV.h_off);
-  F.span_len = dev_view(SV.h_len);
-  F.req_id = dev_view(SV.h_rid);
-  F.sweeps = SV.sweeps;
-  F.z_out = dev_view(SV.h_z);
-  F.done = dev_view(SV.h_done);
-  F.done_value = 1;
-  if (!F.slots || !F.span_off || !F.span_len || !F.req_id || !F.z_out || !F.done)
-    return fail("host-mapped pointers unavailable");
-  k_foldin<8><<<nb, 64, SV.lds, st>>>(serve_params(), F);
-  HCK(hipGetLastError());
-  SV.inflight.insert(SV.inflight.end(), SV.pending.begin(), SV.pending.end());
-  SV.pending.clear();
-  return 0;
+  return serve_keep_pool();
 }
 
 // Slots whose fold-in finished since the last poll (at most max).
@@ -1366,7 +1481,7 @@ extern "C" int gpulda_serve_poll(uint32_t* done_slots, uint32_t max, uint32_t* n
     }
   }
   *n_done = n;
-  return 0;
+  return serve_keep_pool();
 }
 
 // Copies a finished request's topics (u32) and frees its slot.
@@ -1376,13 +1491,36 @@ extern "C" int gpulda_serve_result(uint32_t slot, uint32_t* z, uint32_t* request
     return fail("slot %u has no finished request", slot);
   const u64 off = SV.h_off[slot];
   const u32 L = SV.h_len[slot];
-  if (z)
-    for (u32 i = 0; i < L; ++i) z[i] = SV.h_z[off + i];
+  const u32 flag = __atomic_load_n(&SV.h_done[slot], __ATOMIC_ACQUIRE);
+  const u32* dr = SV.h_drec + (u64)kServeRecWords * slot;
+  int rc = 0;
+  if (flag != SV.expect[slot]) {
+    rc = fail("slot %u: the device processed different request data (fingerprint %08x, staged %08x); it read ring "
+              "index %d, L %u, id %u, offset %u, word sum %08x; the host published ring index %u with L %u, id %u, "
+              "offset %u, word sum %08x",
+              slot, flag, SV.expect[slot], (int)dr[0] - 1, dr[1], dr[2], dr[3], dr[5], SV.pub_at[slot], L,
+              SV.h_rid[slot], (u32)off, SV.wsum[slot]);
+  } else {
+    // the topics arrive by system-scope stores ordered before the flag; their checksum is in the
+    // record, and the copy is read again (briefly) until it matches
+    const double t0 = host_seconds();
+    for (;;) {
+      u32 zs = 0;
+      for (u32 i = 0; i < L; ++i) zs += ((u32)__atomic_load_n(&SV.h_z[off + i], __ATOMIC_ACQUIRE) + 1u) * (2u * i + 1u);
+      if (zs == __atomic_load_n(&dr[6], __ATOMIC_ACQUIRE)) break;
+      if (host_seconds() - t0 > 1.0) {
+        rc = fail("slot %u: the topics did not arrive intact", slot);
+        break;
+      }
+    }
+    if (rc == 0 && z)
+      for (u32 i = 0; i < L; ++i) z[i] = SV.h_z[off + i];
+  }
   if (request_id) *request_id = SV.h_rid[slot];
   __atomic_store_n(&SV.h_done[slot], 0u, __ATOMIC_RELEASE);
-  serve_free_span(off, L);
+  serve_free_span(off, serve_span_tokens(L));
   SV.free_slots.push_back(slot);
-  return 0;
+  return rc;
 }
 
 extern "C" int gpulda_serve_stats(uint32_t* pending, uint32_t* inflight, uint64_t* free_tokens) {

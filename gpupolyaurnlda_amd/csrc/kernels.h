@@ -38,7 +38,7 @@ struct FoldParams {
   const u32* span_len;  // slot -> tokens
   const u32* req_id;    // slot -> request id (Philox counter word 1)
   u32 sweeps;
-  u16* z_out;           // optional host-visible copy of the final topics (arena offsets)
+  u16* z_out;           // optional copy of the final topics (arena offsets) the host reads
   u32* done;            // optional host-visible completion flags (slot)
   u32 done_value;
 };
@@ -46,17 +46,38 @@ struct FoldParams {
 template <int LMAX>
 __global__ void k_foldin(SweepParams P, FoldParams F);
 
-// Request ring of the serving worker pool (ring, tail and shutdown host-visible; head on the
-// device).
+// Request ring of the serving worker pool (device memory).
 struct ServeRing {
-  const u32* ring;
-  u32 ring_size;
-  const u32* tail;
-  u32* head;
-  const u32* shutdown;
-  u64 lifetime_ticks;  // s_memrealtime ticks (100 MHz) a wave may keep claiming
-  u64 idle_ticks;      // ticks an idle wave waits for work before leaving
+  u32* ring;            // entry i % ring_size: slot + 1 of the i-th request once published, else 0
+  u32 ring_size;        // = number of slots
+  u32* tail;            // requests announced (an announced entry may still be being filled)
+  u32* head;            // requests claimed
+  u32* shutdown;        // nonzero asks the pool to leave
+  u64 lifetime_ticks;   // wall-clock ticks after which a wave stops claiming or waiting
+  u32* rec;             // per slot, kServeRecWords: the request record (see below)
+  u64 arena;            // tokens in the arena (P.w / P.z)
+  u32 V;                // word ids must lie below V
+  u32* done_rec;        // host-visible, per slot kServeRecWords: what the serving wave read and wrote
 };
+
+// Request record, one 128-byte line per slot:
+//   [0] ring index + 1 it was published at   [1] L   [2] request id   [3] offset low
+//   [4] offset high   [5] sum of w_i * (2i + 1)
+// A worker uses a record only once [0] names the entry it claimed and the words in the arena
+// sum to [5] (both re-read for a bounded time); otherwise it reports the request as failed.
+constexpr u32 kServeRecWords = 32;
+constexpr u32 kServeMetaWords = 6;  // the part of a record the host sends ([0] holds the slot)
+constexpr u32 kServeFailed = 2;     // completion flag of a request whose data never validated
+
+// One flush.  The host has copied (by DMA, on the feeder's stream) each request's words into the
+// arena and its record into meta; the feeder writes the records into the per-slot lines and
+// publishes the ring entries.
+struct ServeFeed {
+  const u32* meta;      // kServeMetaWords per request of this flush, in ring order
+  u32 base;             // ring index of the first record
+  u32 count;
+};
+__global__ void k_serve_feed(ServeFeed Fd, ServeRing R);
 __global__ void k_serve_worker(SweepParams P, FoldParams F, ServeRing R);
 
 __global__ void k_fill_u32(u32* p, u64 n, u32 v);

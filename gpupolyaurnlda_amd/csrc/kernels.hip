@@ -772,66 +772,161 @@ __global__ __launch_bounds__(64) void k_foldin(SweepParams P, FoldParams F) {
 template __global__ void k_foldin<8>(SweepParams, FoldParams);
 
 // ---------------------------------------------------------------------------------------------
-// Serving worker: a pool of wavefronts that take requests from a ring written by the host
-// (continuous batching).  A wave claims ring entry `head` with a CAS while head < tail (tail and
-// the ring live in host-visible memory), runs the request's fold-in (the same steps as k_foldin),
-// publishes the result and claims the next one.  Every wave exits when the ring is empty and it
-// has been idle for idle_ticks, when lifetime_ticks have passed since it started, or when the
-// host raises *shutdown -- so the grid always drains.  Ticks are s_memrealtime (100 MHz).
+// Serving (continuous batching).
+//   k_serve_feed   one launch per flush, after the host's copies of the flush's words (into the
+//                  arena) and records on the same stream: writes each record into its slot's line,
+//                  then publishes ring entry base + i (slot + 1) with an agent-scope release and
+//                  raises the announced count `tail`;
+//   k_serve_worker a resident pool: each wavefront claims the next announced entry with a
+//                  compare-and-swap on `head`, takes the entry once published, checks the record
+//                  (it names the claimed entry) and the words (their sum is the recorded one, all
+//                  ids below V), runs the request's fold-in (the steps of k_foldin), writes the
+//                  topics and their checksum to host memory and raises the completion flag.
+// Control flow in the worker is uniform: every loop and branch tests a scalar, and lane 0 alone
+// only ever issues a single atomic, so no part of the wave can be left behind in a loop.  A
+// worker stops claiming once the pool's lifetime has passed or `shutdown` is raised and waits for
+// an entry, a record or words at most that long, so every wave reaches its exit.  Data that does
+// not validate in time is never used: the request's flag reports the failure.
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ u64 rt_ticks() { return __builtin_amdgcn_s_memrealtime(); }
+__device__ __forceinline__ u32 scalar_load(const u32* p) {
+  return uniform_u32(__hip_atomic_load(const_cast<u32*>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+}
+// A load that bypasses the vector and L2 caches (system scope), for data another agent wrote.
+__device__ __forceinline__ u32 fresh_load(const u32* p) {
+  return __hip_atomic_load(const_cast<u32*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__global__ __launch_bounds__(64) void k_serve_feed(ServeFeed Fd, ServeRing R) {
+  const u32 k = blockIdx.x * 64 + threadIdx.x;
+  if (k >= Fd.count) return;
+  const u32* m = Fd.meta + (u64)k * kServeMetaWords;
+  const u32 slot = fresh_load(&m[0]);
+  if (slot >= R.ring_size) return;  // never from the host; the request would then time out
+  const u32 idx = Fd.base + k;
+  u32* rec = R.rec + (u64)slot * kServeRecWords;
+  for (u32 q = 1; q < kServeMetaWords; ++q) rec[q] = fresh_load(&m[q]);
+  __threadfence();
+  rec[0] = idx + 1u;  // the record is complete once it names this entry
+  __threadfence();    // record before the entry
+  __hip_atomic_exchange(&R.ring[idx % R.ring_size], slot + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_max(R.tail, idx + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __global__ __launch_bounds__(64) void k_serve_worker(SweepParams P, FoldParams F, ServeRing R) {
   extern __shared__ u32 serve_smem[];
   const u32 lane = lane_id();
   const u32 K = P.K;
-  const u64 start = rt_ticks();
+  const u64 start = wall_clock64();
   for (;;) {
-    u32 item = 0xFFFFFFFFu;
-    if (lane == 0) {
-      u64 idle_from = rt_ticks();
-      u32 nap = 1;
-      for (;;) {
-        const u32 t = __atomic_load_n(R.tail, __ATOMIC_ACQUIRE);
-        const u32 h = __hip_atomic_load(R.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (h != t) {  // tail - head counts unclaimed entries (both only grow)
-          u32 expect = h;
-          if (__hip_atomic_compare_exchange_strong(R.head, &expect, h + 1, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT)) {
-            item = h;
-            break;
-          }
-          continue;
+    // claim the next announced entry
+    u32 item = kNoSlot;
+    for (u32 nap = 1;;) {
+      if (scalar_load(R.shutdown) != 0u || wall_clock64() - start > R.lifetime_ticks) break;
+      const u32 t = scalar_load(R.tail);
+      const u32 h = scalar_load(R.head);
+      // head is read after tail, so other waves may have moved it past this tail: only an entry
+      // below the tail read here is announced (signed difference, the counters wrap)
+      if ((int)(t - h) > 0) {
+        u32 won = 0;
+        if (lane == 0) {
+          u32 e = h;
+          won = __hip_atomic_compare_exchange_strong(R.head, &e, h + 1, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)
+                    ? 1u
+                    : 0u;
         }
-        const u64 now = rt_ticks();
-        if (__atomic_load_n(R.shutdown, __ATOMIC_ACQUIRE) || now - start > R.lifetime_ticks ||
-            now - idle_from > R.idle_ticks)
+        if (readlane_u32(won, 0)) {
+          item = h;
           break;
-        for (u32 k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(32);
-        nap = nap < 16 ? nap * 2 : 16;
+        }
+        continue;  // another wave took it: retry at once
+      }
+      for (u32 q = 0; q < nap; ++q) __builtin_amdgcn_s_sleep(16);
+      nap = nap < 32 ? 2 * nap : 32;
+    }
+    if (item == kNoSlot) return;
+    // take the entry (announced before its feeder has published it); taking it clears it for the
+    // next lap of the ring
+    u32 slot = kNoSlot;
+    for (const u64 t0 = wall_clock64();;) {
+      u32 v = 0;
+      if (lane == 0) v = __hip_atomic_exchange(&R.ring[item % R.ring_size], 0u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      v = readlane_u32(v, 0);
+      if (v != 0u) {
+        slot = v - 1u;
+        break;
+      }
+      if (scalar_load(R.shutdown) != 0u || wall_clock64() - t0 > R.lifetime_ticks) break;
+      __builtin_amdgcn_s_sleep(8);
+    }
+    if (slot >= R.ring_size) return;  // no entry in time: the host reports the request as lost
+    // the record, until it names the claimed entry
+    const u32* rec = R.rec + (u64)slot * kServeRecWords;
+    u32 rv = 0;
+    bool ok = false;
+    for (const u64 t0 = wall_clock64();;) {
+      rv = lane < kServeMetaWords ? fresh_load(&rec[lane]) : 0u;
+      ok = readlane_u32(rv, 0) == item + 1u;
+      if (ok || wall_clock64() - t0 > R.lifetime_ticks) break;
+      __builtin_amdgcn_s_sleep(4);
+    }
+    const u32 L = readlane_u32(rv, 1);
+    const u32 rid = readlane_u32(rv, 2);
+    const u64 b = (u64)readlane_u32(rv, 3) | ((u64)readlane_u32(rv, 4) << 32);
+    const u32 want_sum = readlane_u32(rv, 5);
+    ok = ok && L >= 1 && L <= kMaxDocLen && b + L <= R.arena;
+    // the words, until they sum to the recorded value (and all ids lie below V)
+    u32 fp = 0;
+    if (ok) {
+      ok = false;
+      for (const u64 t0 = wall_clock64();;) {
+        fp = 0;
+        u32 out_of_range = 0;
+        for (u32 i = lane; i < L; i += 64) {
+          const u32 w = fresh_load(&P.w[b + i]);
+          fp += w * (2u * i + 1u);
+          out_of_range |= w >= R.V ? 1u : 0u;
+        }
+        for (u32 off = 32; off > 0; off >>= 1) fp += __shfl_xor(fp, off, 64);
+        const bool range_ok = __ballot(out_of_range != 0u) == 0ull;
+        ok = fp == want_sum && range_ok;
+        if (ok || wall_clock64() - t0 > R.lifetime_ticks) break;
+        __builtin_amdgcn_s_sleep(4);
       }
     }
-    item = readlane_u32(item, 0);
-    if (item == 0xFFFFFFFFu) return;
-    const u32 slot = __atomic_load_n(&R.ring[item % R.ring_size], __ATOMIC_ACQUIRE);
-    const u64 b = F.span_off[slot];
-    const u32 L = F.span_len[slot];
-    const u32 rid = F.req_id[slot];
-    for (u32 i = lane; i < L; i += 64) {
-      const u32 w = P.w[b + i];
-      const u32x4 r = philox_tagged(i, rid, 0, kTagFoldInit, 0, P.k0, P.k1);
-      const u32 z = P.sigma_a[w] > 0.0f ? draw_alias_slot(P.alias + (u64)w * K, K, r.y, r.z) : bounded(r.x, K);
-      P.z[b + i] = (u16)z;
+    __threadfence();  // the plain reads of the arena below come after the validated ones
+    u32 flag = kServeFailed, zsum = 0;
+    if (ok) {
+      for (u32 i = lane; i < L; i += 64) {
+        const u32 w = P.w[b + i];
+        const u32x4 r = philox_tagged(i, rid, 0, kTagFoldInit, 0, P.k0, P.k1);
+        const u32 z = P.sigma_a[w] > 0.0f ? draw_alias_slot(P.alias + (u64)w * K, K, r.y, r.z) : bounded(r.x, K);
+        P.z[b + i] = (u16)z;
+      }
+      wave_sync();
+      DocState ds = open_span(P, b, L, rid, serve_smem);
+      u64 visited = 0;
+      for (u32 sweep = 1; sweep <= F.sweeps; ++sweep) sweep_pass<8>(P, serve_smem, ds, sweep, kTagFoldSweep, visited);
+      wave_sync();
+      // the topics go to host memory with system-scope stores and their checksum in the record
+      for (u32 i = lane; i < L; i += 64) {
+        const u32 z = P.z[b + i];
+        zsum += (z + 1u) * (2u * i + 1u);
+        __hip_atomic_store(&F.z_out[b + i], (u16)z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+      for (u32 off = 32; off > 0; off >>= 1) zsum += __shfl_xor(zsum, off, 64);
+      flag = serve_fingerprint(fp, L, rid);
     }
-    __syncthreads();
-    DocState ds = open_span(P, b, L, rid, serve_smem);
-    u64 visited = 0;
-    for (u32 sweep = 1; sweep <= F.sweeps; ++sweep) sweep_pass<8>(P, serve_smem, ds, sweep, kTagFoldSweep, visited);
-    __syncthreads();
-    for (u32 i = lane; i < L; i += 64) F.z_out[b + i] = P.z[b + i];
+    // what this wave read and wrote, then the completion flag (the fingerprint of what it read)
+    u32* dr = R.done_rec + (u64)slot * kServeRecWords;
+    const u32 out_rec[8] = {item + 1u, L, rid, (u32)b, (u32)(b >> 32), fp, zsum, ok ? 1u : 0u};
+    u32 v = 0;
+#pragma unroll
+    for (u32 q = 0; q < 8; ++q)
+      if (lane == q) v = out_rec[q];
+    if (lane < 8) __hip_atomic_store(&dr[lane], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __threadfence_system();
-    if (lane == 0) __atomic_store_n(&F.done[slot], F.done_value, __ATOMIC_RELEASE);
-    __syncthreads();
+    if (lane == 0) __hip_atomic_store(&F.done[slot], flag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 

@@ -113,6 +113,12 @@ constexpr u32 kMapLine = 64;          // one wavefront probes 64 consecutive slo
 constexpr u32 kMaxDocLen = 65535;     // counts are 16-bit
 constexpr u32 kMaxTopics = 65535;     // keys are 16-bit, 0xFFFF marks empty
 
+// Completion flag of a served request: a fingerprint of its words (sum of w_i * (2i + 1), any
+// order), length and request id, never 0 (0 means not finished).
+LDA_HD u32 serve_fingerprint(u32 word_sum, u32 L, u32 rid) {
+  return (word_sum ^ (L * 0x9E3779B1u) ^ (rid * 0x85EBCA6Bu)) | 1u;
+}
+
 LDA_HD u32 map_key(u32 e) { return e >> 16; }
 LDA_HD u32 map_count(u32 e) { return e & 0xFFFFu; }
 LDA_HD u32 map_entry(u32 key, u32 count) { return (key << 16) | count; }

@@ -2,11 +2,11 @@
 
 ``FoldInServer`` wraps the library's continuous-batching server (gpulda_serve_*).  ``run_stream``
 plays an open-loop request stream against it: requests are admitted as their arrival time passes
-(and as slots and HBM arena space free up), admitted requests are launched together every
-``window`` seconds -- on rotating streams, so a batch starts while earlier ones are still running
--- and each result is collected as soon as its completion flag is raised.  Latency is completion
-minus arrival.  ``run_serial`` is the baseline: the same requests one at a time, each launched
-alone and waited for before the next.
+(and as slots and HBM arena space free up), admitted requests are published to the device's
+request ring every ``window`` seconds -- where the resident worker pool picks each up at once,
+beside the requests already running -- and each result is collected as soon as its completion
+flag is raised.  Latency is completion minus arrival.  ``run_serial`` is the baseline: the same
+requests one at a time, each submitted alone and waited for before the next.
 """
 from __future__ import annotations
 
@@ -19,10 +19,10 @@ from ._lib import check, lib, ptr
 
 
 class FoldInServer:
-    def __init__(self, hbm_budget_bytes: int = 16 << 30, sweeps: int = 50, n_streams: int = 8,
+    def __init__(self, hbm_budget_bytes: int = 16 << 30, sweeps: int = 50, waves_per_cu: int = 2,
                  max_slots: int = 16384):
         self._lib = lib()
-        check(self._lib.gpulda_serve_open(hbm_budget_bytes, sweeps, n_streams, max_slots), "serve_open")
+        check(self._lib.gpulda_serve_open(hbm_budget_bytes, sweeps, waves_per_cu, max_slots), "serve_open")
         self._done = np.zeros(max_slots, dtype=np.uint32)
         self._n = np.zeros(1, dtype=np.uint32)
         self.open = True
@@ -69,8 +69,13 @@ def _split(d_len, w):
     return [w[off[i]:off[i + 1]] for i in range(len(d_len))]
 
 
-def run_stream(server: FoldInServer, d_len, arrival, w, window: float = 0.001, id_base: int = 0,
-               max_batch: int = 4096):
+def _stalled(last_progress: float, limit: float) -> None:
+    if time.perf_counter() - last_progress > limit:
+        raise RuntimeError(f"fold-in server made no progress for {limit:.0f} s")
+
+
+def run_stream(server: FoldInServer, d_len, arrival, w, window: float = 0.0002, id_base: int = 0,
+               max_batch: int = 4096, stall_s: float = 60.0):
     """Open-loop replay of a request stream.  Returns dict(latency_s, results, wall_s)."""
     docs = _split(d_len, w)
     n = len(docs)
@@ -82,7 +87,9 @@ def run_stream(server: FoldInServer, d_len, arrival, w, window: float = 0.001, i
     pending = 0
     last_flush = 0.0
     t0 = time.perf_counter()
+    progress = t0
     while done < n:
+        _stalled(progress, stall_s)
         now = time.perf_counter() - t0
         while nxt < n and arrival[nxt] <= now and pending < max_batch:
             slot = server.submit(docs[nxt], id_base + nxt)
@@ -96,6 +103,7 @@ def run_stream(server: FoldInServer, d_len, arrival, w, window: float = 0.001, i
             pending = 0
             last_flush = now
         for slot in server.poll():
+            progress = time.perf_counter()
             i = slot_of.pop(slot)
             latency[i] = (time.perf_counter() - t0) - arrival[i]
             results[i], _ = server.result(slot, len(docs[i]))
@@ -108,8 +116,8 @@ def run_stream(server: FoldInServer, d_len, arrival, w, window: float = 0.001, i
     return {"latency_s": latency, "results": results, "wall_s": time.perf_counter() - t0}
 
 
-def run_serial(server: FoldInServer, d_len, w, id_base: int = 0):
-    """Each request launched alone and awaited before the next.  Returns dict(service_s, results)."""
+def run_serial(server: FoldInServer, d_len, w, id_base: int = 0, stall_s: float = 60.0):
+    """Each request submitted alone and awaited before the next.  Returns dict(service_s, results)."""
     docs = _split(d_len, w)
     service = np.zeros(len(docs))
     results = []
@@ -123,6 +131,7 @@ def run_serial(server: FoldInServer, d_len, w, id_base: int = 0):
             got = server.poll()
             if got:
                 break
+            _stalled(t, stall_s)
         z, _ = server.result(got[0], len(doc))
         service[i] = time.perf_counter() - t
         results.append(z)

@@ -94,11 +94,13 @@ int gpulda_calibrate_copy(uint64_t bytes, uint32_t reps, float* ms_per_copy);
 
 /* ---- fold-in server (continuous batching against the resident model) ----------------------
  * Keeps phi, sigma_a and the alias tables of the current model in HBM.  Requests are admitted
- * while a slot and arena space (hbm_budget_bytes / 6 tokens) are free; flush launches every
- * admitted request as one batch on the next of n_streams streams; each request runs `sweeps`
- * passes, then its topics and completion flag become visible in pinned host memory.  A request's
- * randomness depends on its id only: its result is the same alone or in any batch. */
-int gpulda_serve_open(uint64_t hbm_budget_bytes, uint32_t sweeps, uint32_t n_streams, uint32_t max_slots);
+ * while a slot and arena space (hbm_budget_bytes / 6 tokens) are free; flush publishes every
+ * admitted request to a request ring that a resident pool of worker wavefronts (waves_per_cu per
+ * compute unit, one wavefront per request) drains as requests arrive, so new requests start while
+ * earlier ones are still running.  Each request runs `sweeps` passes, then its topics and
+ * completion flag become visible in host memory (poll / result).  A request's randomness depends
+ * on its id only: its result is the same alone or beside any other requests. */
+int gpulda_serve_open(uint64_t hbm_budget_bytes, uint32_t sweeps, uint32_t waves_per_cu, uint32_t max_slots);
 /* 0 = admitted (slot set), 1 = no room now (retry after results are collected). */
 int gpulda_serve_submit(const uint32_t* w, uint32_t L, uint32_t request_id, uint32_t* slot);
 int gpulda_serve_flush(void);

@@ -352,17 +352,19 @@ def test_server_alone_equals_mixed(orc):
         sig = rc.sigma_a()
         prob, al = rc.alias()
         d_len, arr, w = corpus.requests(phi, 60, median=120, sigma=1.0, lo=10, hi=1500, rate=3000.0, seed=4)
-        with FoldInServer(hbm_budget_bytes=64 << 20, sweeps=20, n_streams=4, max_slots=256) as srv:
-            mixed = run_stream(srv, d_len, arr, w, window=0.0005)
+        with FoldInServer(hbm_budget_bytes=64 << 20, sweeps=20, waves_per_cu=1, max_slots=256) as srv:
+            mixed = run_stream(srv, d_len, arr, w)
             alone = run_serial(srv, d_len[:12], w[:int(d_len[:12].sum())])
         say(f"server p50={np.median(mixed['latency_s']) * 1e3:.2f} ms serial mean={alone['service_s'].mean() * 1e3:.2f} ms")
-    for i in range(12):
-        assert np.array_equal(mixed["results"][i], alone["results"][i]), i
     off = np.concatenate([[0], np.cumsum(d_len)]).astype(np.uint64)
-    zs, _ = stages.foldin(w, off, np.arange(len(d_len), dtype=np.uint32), 20, 1, spec.K, phi, sig, prob, al)
-    assert np.array_equal(np.concatenate(mixed["results"]), zs)
-    zc, _ = orc.foldin(w, off, np.arange(len(d_len), dtype=np.uint32), 20, 1, spec.K, phi, sig, prob, al, threads=8)
+    ids = np.arange(len(d_len), dtype=np.uint32)
+    zs, _ = stages.foldin(w, off, ids, 20, 1, spec.K, phi, sig, prob, al)
+    zc, _ = orc.foldin(w, off, ids, 20, 1, spec.K, phi, sig, prob, al, threads=8)
     assert np.array_equal(zs, zc)
+    want = [zs[off[i]:off[i + 1]] for i in range(len(d_len))]
+    bad_mixed = [i for i in range(len(d_len)) if not np.array_equal(mixed["results"][i], want[i])]
+    bad_alone = [i for i in range(12) if not np.array_equal(alone["results"][i], want[i])]
+    assert not bad_mixed and not bad_alone, f"mixed wrong {bad_mixed}, alone wrong {bad_alone}"
 
 
 def test_medium_config_parity_and_ll_trace(orc):
