@@ -178,9 +178,10 @@ def serve_bench(rc, spec, n_req, n_serial):
            "docs_per_s": round(docs_s, 1),
            "latency_ms": {k: round(v * 1e3, 3) for k, v in percentiles(lat).items()},
            "latency_ms_le1000_tokens": {k: round(v * 1e3, 3) for k, v in percentiles(lat[short]).items()},
-           "serial_docs_per_s": round(serial_docs_s, 2),
+           "in_server_ms": {k: round(v * 1e3, 3) for k, v in percentiles(mixed["in_server_s"]).items()},
+           "serial_docs_per_s": round(float(serial_docs_s), 2),
            "serial_sample": n_s,
-           "speedup_vs_serial": round(docs_s / serial_docs_s, 1),
+           "speedup_vs_serial": round(float(docs_s / serial_docs_s), 1),
            "results_identical_alone_vs_mixed": bool(same),
            "mean_tokens": round(float(d_len.mean()), 1)}
     log(f"serving: {out}")

@@ -1159,12 +1159,15 @@ extern "C" int gpulda_corpus_release(void) {
 // A pool of worker wavefronts (k_serve_worker, waves_per_cu per CU) stays resident and takes
 // requests from the ring as soon as they are published, one wavefront per request, so requests
 // that arrive while others run start at once and overlap on the device.  A pool stops claiming
-// after kPoolLifetime seconds; the host starts a younger pool on a second stream once the running
-// one is half way through, and again whenever none is running while requests are outstanding.
+// after kPoolLifetime seconds and its kernel ends once its last request is done; the host starts a
+// younger pool on another of kPools streams once the newest is half way through (and whenever none
+// is running while requests are outstanding).  An expiring pool may take as long as its longest
+// request to drain, so a third stream keeps one pool claiming meanwhile.
 // A request's randomness is keyed by its request id, so its result is the same whatever runs
 // beside it.
 namespace {
-constexpr double kPoolLifetime = 0.5;  // seconds a pool keeps claiming requests
+constexpr double kPoolLifetime = 2.0;  // seconds a pool keeps claiming requests
+constexpr int kPools = 3;              // pool streams (with the feeder's: four hardware queues)
 
 struct ServePool {
   hipStream_t st = nullptr;
@@ -1202,7 +1205,7 @@ struct ServeState {
   std::vector<std::pair<u64, u64>> free_spans;  // (offset, length), sorted by offset
   std::vector<u32> pending;   // admitted, not published
   std::vector<u32> inflight;  // published, not reported
-  ServePool pools[2];
+  ServePool pools[kPools];
   hipStream_t feed = nullptr;
   u64 lds = 0;
 };

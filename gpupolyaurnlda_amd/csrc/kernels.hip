@@ -566,6 +566,26 @@ __device__ __forceinline__ void sweep_pass(const SweepParams& P, u32* T, DocStat
   }
 }
 
+// Fold-in carries the map from sweep to sweep.  Once the document's topics have thinned out, the
+// map is rebuilt at the capacity its keys need (the layout is a function of the key set and the
+// capacity), so the later sweeps scan fewer lines.
+__device__ __forceinline__ void shrink_doc_map(const SweepParams& P, u32* T, DocState& ds) {
+  const u32 want = map_capacity(ds.occupied, P.K);
+  if (want < ds.cap) {
+    ds.cap = want;
+    wave_sync();
+    build_doc_map(T, ds.cap, P.z, ds.b, ds.L);
+  }
+}
+
+// All sweeps of a fold-in (counters (token, request, s, FOLD_SWEEP), s = 1..sweeps).
+__device__ __forceinline__ void foldin_sweeps(const SweepParams& P, u32* T, DocState& ds, u32 sweeps, u64& visited) {
+  for (u32 sweep = 1; sweep <= sweeps; ++sweep) {
+    if (sweep > 1) shrink_doc_map(P, T, ds);
+    sweep_pass<8>(P, T, ds, sweep, kTagFoldSweep, visited);
+  }
+}
+
 __device__ __forceinline__ u32 sweep_iteration(const SweepParams& P) { return P.iter_ptr ? *P.iter_ptr : P.iter; }
 
 template <int LMAX>
@@ -756,7 +776,7 @@ __global__ __launch_bounds__(64) void k_foldin(SweepParams P, FoldParams F) {
   __syncthreads();
   DocState ds = open_span(P, b, L, rid, fold_smem);
   u64 visited = 0;
-  for (u32 sweep = 1; sweep <= F.sweeps; ++sweep) sweep_pass<LMAX>(P, fold_smem, ds, sweep, kTagFoldSweep, visited);
+  foldin_sweeps(P, fold_smem, ds, F.sweeps, visited);
   if (F.z_out) {
     __syncthreads();
     for (u32 i = lane; i < L; i += 64) F.z_out[b + i] = P.z[b + i];
@@ -788,8 +808,11 @@ template __global__ void k_foldin<8>(SweepParams, FoldParams);
 // an entry, a record or words at most that long, so every wave reaches its exit.  Data that does
 // not validate in time is never used: the request's flag reports the failure.
 // ---------------------------------------------------------------------------------------------
+// Polling read of a shared counter: coherent (agent scope) but relaxed, so an idle wave does not
+// invalidate its compute unit's caches under the waves that are working there; the one acquire
+// fence follows a successful claim.
 __device__ __forceinline__ u32 scalar_load(const u32* p) {
-  return uniform_u32(__hip_atomic_load(const_cast<u32*>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+  return uniform_u32(__hip_atomic_load(const_cast<u32*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 // A load that bypasses the vector and L2 caches (system scope), for data another agent wrote.
 __device__ __forceinline__ u32 fresh_load(const u32* p) {
@@ -830,7 +853,7 @@ __global__ __launch_bounds__(64) void k_serve_worker(SweepParams P, FoldParams F
         u32 won = 0;
         if (lane == 0) {
           u32 e = h;
-          won = __hip_atomic_compare_exchange_strong(R.head, &e, h + 1, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+          won = __hip_atomic_compare_exchange_strong(R.head, &e, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                      __HIP_MEMORY_SCOPE_AGENT)
                     ? 1u
                     : 0u;
@@ -850,7 +873,7 @@ __global__ __launch_bounds__(64) void k_serve_worker(SweepParams P, FoldParams F
     u32 slot = kNoSlot;
     for (const u64 t0 = wall_clock64();;) {
       u32 v = 0;
-      if (lane == 0) v = __hip_atomic_exchange(&R.ring[item % R.ring_size], 0u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) v = __hip_atomic_exchange(&R.ring[item % R.ring_size], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       v = readlane_u32(v, 0);
       if (v != 0u) {
         slot = v - 1u;
@@ -860,6 +883,7 @@ __global__ __launch_bounds__(64) void k_serve_worker(SweepParams P, FoldParams F
       __builtin_amdgcn_s_sleep(8);
     }
     if (slot >= R.ring_size) return;  // no entry in time: the host reports the request as lost
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);  // the entry was published with a release
     // the record, until it names the claimed entry
     const u32* rec = R.rec + (u64)slot * kServeRecWords;
     u32 rv = 0;
@@ -906,7 +930,7 @@ __global__ __launch_bounds__(64) void k_serve_worker(SweepParams P, FoldParams F
       wave_sync();
       DocState ds = open_span(P, b, L, rid, serve_smem);
       u64 visited = 0;
-      for (u32 sweep = 1; sweep <= F.sweeps; ++sweep) sweep_pass<8>(P, serve_smem, ds, sweep, kTagFoldSweep, visited);
+      foldin_sweeps(P, serve_smem, ds, F.sweeps, visited);
       wave_sync();
       // the topics go to host memory with system-scope stores and their checksum in the record
       for (u32 i = lane; i < L; i += 64) {

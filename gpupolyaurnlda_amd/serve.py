@@ -76,10 +76,13 @@ def _stalled(last_progress: float, limit: float) -> None:
 
 def run_stream(server: FoldInServer, d_len, arrival, w, window: float = 0.0002, id_base: int = 0,
                max_batch: int = 4096, stall_s: float = 60.0):
-    """Open-loop replay of a request stream.  Returns dict(latency_s, results, wall_s)."""
+    """Open-loop replay of a request stream.  Returns dict(latency_s, in_server_s (admission to
+    completion), results, wall_s)."""
     docs = _split(d_len, w)
     n = len(docs)
     latency = np.zeros(n)
+    in_server = np.zeros(n)  # admission to completion
+    admitted_at = np.zeros(n)
     results = [None] * n
     slot_of = {}
     nxt = 0
@@ -96,6 +99,7 @@ def run_stream(server: FoldInServer, d_len, arrival, w, window: float = 0.0002, 
             if slot is None:
                 break
             slot_of[slot] = nxt
+            admitted_at[nxt] = time.perf_counter() - t0
             nxt += 1
             pending += 1
         if pending and (now - last_flush >= window or pending >= max_batch):
@@ -105,7 +109,9 @@ def run_stream(server: FoldInServer, d_len, arrival, w, window: float = 0.0002, 
         for slot in server.poll():
             progress = time.perf_counter()
             i = slot_of.pop(slot)
-            latency[i] = (time.perf_counter() - t0) - arrival[i]
+            t_done = time.perf_counter() - t0
+            latency[i] = t_done - arrival[i]
+            in_server[i] = t_done - admitted_at[i]
             results[i], _ = server.result(slot, len(docs[i]))
             done += 1
         if pending == 0 and nxt < n and done < nxt:
@@ -113,7 +119,7 @@ def run_stream(server: FoldInServer, d_len, arrival, w, window: float = 0.0002, 
             wait = arrival[nxt] - (time.perf_counter() - t0)
             if wait > 2e-4:
                 time.sleep(min(wait, 2e-4))
-    return {"latency_s": latency, "results": results, "wall_s": time.perf_counter() - t0}
+    return {"latency_s": latency, "in_server_s": in_server, "results": results, "wall_s": time.perf_counter() - t0}
 
 
 def run_serial(server: FoldInServer, d_len, w, id_base: int = 0, stall_s: float = 60.0):

@@ -565,7 +565,7 @@ u64 orc_sweep_doc(u32* z, const u32* w, u32 L, u32 doc, u32 iter, u64 seed, u32 
 /* Fold-in of one request against a fixed model: topics start from the word's alias table
  * (a draw from p(k | w) proportional to phi[w][k]; uniform for an all-zero row), then `sweeps`
  * passes with counters (token, request, sweep s = 1..sweeps, FOLD_SWEEP), the map carried from
- * pass to pass. */
+ * pass to pass and shrunk before a pass when its keys need less capacity. */
 u64 orc_foldin_doc(u32* z, const u32* w, u32 L, u32 req, u32 sweeps, u64 seed, u32 K, const float* phi,
                    const float* sigma_a, const OrcAlias* alias, u32* scratch) {
   for (u32 i = 0; i < L; ++i) {
@@ -575,7 +575,16 @@ u64 orc_foldin_doc(u32* z, const u32* w, u32 L, u32 req, u32 sweeps, u64 seed, u
   }
   OrcDoc d = doc_open(z, L, K, scratch);
   u64 visited = 0;
-  for (u32 s = 1; s <= sweeps; ++s) visited += doc_pass(&d, z, w, req, s, TAG_FOLD_SWEEP, seed, phi, sigma_a, alias);
+  for (u32 s = 1; s <= sweeps; ++s) {
+    /* the map is carried from pass to pass; once the document's topics have thinned out it is
+     * rebuilt at the capacity its keys need, orc_map_capacity(keys, K) */
+    if (s > 1 && orc_map_capacity(d.keys, K) < d.cap) {
+      u32 ncap = orc_map_capacity(d.keys, K);
+      map_regrow(d.T, d.cap, ncap, d.scratch);
+      d.cap = ncap;
+    }
+    visited += doc_pass(&d, z, w, req, s, TAG_FOLD_SWEEP, seed, phi, sigma_a, alias);
+  }
   return visited;
 }
 
