@@ -48,6 +48,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the fold-in serving measurement")
     ap.add_argument("--serve-requests", type=int, default=4000)
     ap.add_argument("--serve-serial", type=int, default=100, help="requests in the serial baseline sample")
+    ap.add_argument("--serve-waves", type=int, default=2, help="worker wavefronts per CU of the fold-in server")
+    ap.add_argument("--abi-rate", action="store_true",
+                    help="also time the five-call ABI path (host buffers, z over PCIe each iteration)")
     ap.add_argument("--compare-graph", action="store_true",
                     help="also time the iteration as a replayed hipGraph against plain per-bucket launches "
                          "(identical Philox streams, z compared)")
@@ -120,11 +123,15 @@ def main():
 
     serving = None
     if not args.no_serve:
-        serving = serve_bench(rc, spec, args.serve_requests, args.serve_serial)
+        serving = serve_bench(rc, spec, args.serve_requests, args.serve_serial, args.serve_waves)
 
     graph = None
     if args.compare_graph:  # last: it releases rc (the library holds one model at a time)
         graph = compare_graph(rc, c, spec, args.steps)
+    abi = None
+    if args.abi_rate:
+        rc.close()  # the library holds one model at a time
+        abi = abi_rate(c, spec, 3)
 
     out = {
         "metric": "tokens sampled/sec at K=1024 (% HBM roofline); fold-in docs/s + p99 latency",
@@ -145,7 +152,7 @@ def main():
                                 "sweep": round(float(stage_ms[2]), 3), "count": round(float(stage_ms[3]), 3)},
                    "parallelism": "1 GPU",
                    "schedule": tune if tune is not None else args.schedule,
-                   "graph_vs_plain": graph},
+                   "graph_vs_plain": graph, "abi_path_pcie_inclusive": abi},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serving,
@@ -154,7 +161,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def serve_bench(rc, spec, n_req, n_serial):
+def serve_bench(rc, spec, n_req, n_serial, waves_per_cu=2):
     """Fold-in serving against the model just trained (phi and alias tables resident): requests
     with lognormal(300, 1.2) lengths clipped to [10, 5000], words from a random mixture of 1-5
     topics, Poisson arrivals at 2,000 docs/s, 50 sweeps each, admission bounded by 16 GB of HBM.
@@ -165,7 +172,7 @@ def serve_bench(rc, spec, n_req, n_serial):
     phi = rc.phi()
     d_len, arrival, w = corpus.requests(phi, n_req, median=300.0, sigma=1.2, lo=10, hi=5000, rate=2000.0, seed=4)
     del phi
-    with FoldInServer(hbm_budget_bytes=16 << 30, sweeps=50, waves_per_cu=2, max_slots=16384) as srv:
+    with FoldInServer(hbm_budget_bytes=16 << 30, sweeps=50, waves_per_cu=waves_per_cu, max_slots=16384) as srv:
         # warm the kernels once outside the measurement
         run_serial(srv, d_len[:2], w[:int(d_len[:2].sum())], id_base=1 << 30)
         mixed = run_stream(srv, d_len, arrival, w)
@@ -187,6 +194,29 @@ def serve_bench(rc, spec, n_req, n_serial):
            "results_identical_alone_vs_mixed": bool(same),
            "mean_tokens": round(float(d_len.mean()), 1)}
     log(f"serving: {out}")
+    return out
+
+
+def abi_rate(c, spec, iters):
+    """The reference host's path: train.rs's loop over the five calls with the whole corpus in one
+    host buffer, so every iteration copies z (u32) to the device and back over PCIe.  Returns the
+    PCIe-inclusive tokens/s (one warm-up iteration, which also uploads w and d once)."""
+    from gpupolyaurnlda_amd.train import Sampler, train
+
+    N, D = len(c["w"]), len(c["d_len"])
+    z = c["z"].astype(np.uint32)
+    w = np.ascontiguousarray(c["w"], dtype=np.uint32)
+    d = np.ascontiguousarray(c["d_len"], dtype=np.uint32)
+    k_d = np.zeros(D, dtype=np.uint32)
+    C_counts = np.bincount(w, minlength=spec.V).astype(np.uint32)
+    with Sampler(spec.alpha, spec.beta, spec.K, C_counts, N, D, seed=spec.seed) as smp:
+        train(smp, z, w, d, k_d, 1, N, D)
+        t0 = time.perf_counter()
+        train(smp, z, w, d, k_d, iters, N, D)
+        dt = (time.perf_counter() - t0) / iters
+    out = {"tokens_per_s": round(N / dt, 1), "ms_per_iteration": round(dt * 1e3, 3), "iterations": iters,
+           "buffer": "whole corpus in one host buffer (train.rs loop, z as u32 both ways per iteration)"}
+    log(f"ABI path: {out}")
     return out
 
 

@@ -119,17 +119,19 @@ class Sampler:
 def fill_buffer(d: np.ndarray, doc_pos: int, buffer_size: int, max_D: int):
     """Documents and tokens of the next buffer starting at document doc_pos (train.rs:19-45).
 
-    Returns (n_docs, n_tokens).  Raises if one document exceeds buffer_size (train.rs:27-29)."""
-    n_docs = 0
-    n_tokens = 0
-    while doc_pos + n_docs < len(d):
-        size = int(d[doc_pos + n_docs])
-        if size > buffer_size:
-            raise ValueError(f"document size exceeded buffer size: {size}")
-        if n_docs + 1 > max_D or n_tokens + size > buffer_size:
-            break
-        n_docs += 1
-        n_tokens += size
+    Documents are taken while the buffer has room for one more document (max_D) and its tokens
+    (buffer_size).  Returns (n_docs, n_tokens).  Raises if a document it examines exceeds
+    buffer_size (train.rs:27-29): every document taken and the one that ends the buffer."""
+    window = np.asarray(d[doc_pos:doc_pos + max_D + 1], dtype=np.uint64)
+    if window.size == 0:
+        return 0, 0
+    cs = np.cumsum(window[:max_D])
+    n_docs = int(np.searchsorted(cs, buffer_size, side="right"))
+    examined = window[:n_docs + 1]
+    if int(examined.max()) > buffer_size:
+        first = int(np.argmax(examined > buffer_size))
+        raise ValueError(f"document size exceeded buffer size: {int(examined[first])}")
+    n_tokens = int(cs[n_docs - 1]) if n_docs else 0
     return n_docs, n_tokens
 
 

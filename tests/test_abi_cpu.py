@@ -77,6 +77,36 @@ def test_fill_buffer_restates_train_rs():
     assert fill_buffer(np.array([2, 0, 3], np.uint32), 0, 100, 100) == (3, 5)
 
 
+def _fill_buffer_loop(d, doc_pos, buffer_size, max_D):
+    # train.rs:19-45 document by document: check the size, then stop when it would not fit
+    n_docs = n_tokens = 0
+    while doc_pos + n_docs < len(d):
+        size = int(d[doc_pos + n_docs])
+        if size > buffer_size:
+            raise ValueError(size)
+        if n_docs + 1 > max_D or n_tokens + size > buffer_size:
+            break
+        n_docs += 1
+        n_tokens += size
+    return n_docs, n_tokens
+
+
+def test_fill_buffer_matches_document_loop():
+    from gpupolyaurnlda_amd.train import fill_buffer
+    rng = np.random.default_rng(7)
+    for _ in range(2000):
+        D = int(rng.integers(0, 30))
+        d = rng.integers(0, 20, D).astype(np.uint32)
+        args = (int(rng.integers(0, D + 1)), int(rng.integers(0, 40)), int(rng.integers(1, 10)))
+        try:
+            want = _fill_buffer_loop(d, *args)
+        except ValueError:
+            with pytest.raises(ValueError):
+                fill_buffer(d, *args)
+            continue
+        assert fill_buffer(d, *args) == want, (d, args)
+
+
 def test_generator_is_deterministic_and_shaped():
     from gpupolyaurnlda_amd import corpus
     spec = corpus.TINY
