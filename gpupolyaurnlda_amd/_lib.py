@@ -93,6 +93,8 @@ _SIGS = {
     "gpulda_copy_sigma_a": (C.c_int, [f32p]),
     "gpulda_copy_counts": (C.c_int, [u32p]),
     "gpulda_set_counts": (C.c_int, [u32p]),
+    "gpulda_set_iteration": (C.c_int, [C.c_uint32]),
+    "gpulda_set_part_tokens": (C.c_int, [C.c_uint64]),
     "gpulda_copy_topic_sums": (C.c_int, [u64p]),
     "gpulda_copy_poisson_tables": (C.c_int, [f32p, u32p]),
     "gpulda_sweep_stats": (C.c_int, [u64p]),

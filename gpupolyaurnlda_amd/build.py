@@ -79,10 +79,10 @@ def build_host_lib() -> str:
 
 
 def build_driver() -> str:
-    src = os.path.join(CSRC, "train_driver.cpp")
+    srcs = [os.path.join(CSRC, s) for s in ("train_driver.cpp", "corpus_io.cpp")]
     tmp = DRIVER + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", "-o", tmp, src,
-          f"-L{PKG}", "-lGPUPolyaUrnLDA", "-Wl,-rpath,$ORIGIN"])
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wno-unknown-pragmas", "-ffp-contract=off", f"-I{INCLUDE}", "-o", tmp, *srcs,
+          f"-L{PKG}", "-lGPUPolyaUrnLDA", "-Wl,-rpath,$ORIGIN", "-pthread"])
     os.replace(tmp, DRIVER)
     return DRIVER
 

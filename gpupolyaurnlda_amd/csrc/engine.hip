@@ -194,21 +194,33 @@ struct Segment {
   std::vector<u32> lens_desc;  // document lengths in doc_lpt order (host copy)
   struct Bucket {
     int lmax;      // register-resident lines of the kernel (0 = generic)
-    u32 first;     // offset into doc_order
+    u32 first;     // offset into doc_order (part_order for a part's buckets)
     u32 count;
     u32 max_cap;   // largest map of the bucket (LDS slots)
   };
   std::vector<Bucket> buckets;
+  // Pipeline parts of the five-call path: contiguous document ranges of about equal tokens, each
+  // swept on its own once its z has arrived, so copies of one part overlap the sweep of another.
+  struct Part {
+    u32 doc0, doc1;
+    u64 tok0, tok1;
+    std::vector<Bucket> buckets;
+  };
+  std::vector<Part> parts;
+  u32* part_order = nullptr;  // each part's documents grouped by bucket, parts one after another
 };
 
-// z-sweep of a segment: one launch per length bucket, the longest maps first.
-int launch_sweep(SweepParams P, const Segment* sg, hipStream_t st) {
+constexpr u32 kMaxParts = 8;
+u64 g_part_min_tokens = 4ull << 20;  // smallest pipeline part (gpulda_set_part_tokens)
+
+// z-sweep of buckets of documents (order[first ..]): one launch per bucket, the longest maps first.
+int launch_buckets(SweepParams P, const std::vector<Segment::Bucket>& buckets, const u32* order, hipStream_t st) {
   const u64 lim = (u64)max_lds_per_block();
-  for (const Segment::Bucket& bk : sg->buckets) {
+  for (const Segment::Bucket& bk : buckets) {
     if (bk.count == 0) continue;
     u64 lds = ((u64)bk.max_cap + (P.K + 31) / 32) * sizeof(u32);  // map + distinct-topic bitmap
     if (lds > lim) return fail("doc-topic map of %u slots exceeds LDS", bk.max_cap);
-    P.doc_order = sg->doc_order + bk.first;
+    P.doc_order = order + bk.first;
     switch (bk.lmax) {
       case 1: k_sweep_lines<1><<<bk.count, 64, lds, st>>>(P); break;
       case 2: k_sweep_lines<2><<<bk.count, 64, lds, st>>>(P); break;
@@ -223,6 +235,11 @@ int launch_sweep(SweepParams P, const Segment* sg, hipStream_t st) {
   return 0;
 }
 
+// z-sweep of a whole segment.
+int launch_sweep(const SweepParams& P, const Segment* sg, hipStream_t st) {
+  return launch_buckets(P, sg->buckets, sg->doc_order, st);
+}
+
 void free_segment(Segment* s) {
   if (!s) return;
   dfree(s->w);
@@ -232,6 +249,7 @@ void free_segment(Segment* s) {
   dfree(s->tasks);
   dfree(s->doc_order);
   dfree(s->doc_lpt);
+  dfree(s->part_order);
   delete s;
 }
 
@@ -309,6 +327,42 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
     s->buckets.back().count++;
     s->buckets.back().max_cap = std::max(s->buckets.back().max_cap, cap);
   }
+  // pipeline parts: about n_tokens / kMaxParts tokens each (at least g_part_min_tokens), whole
+  // documents, buckets ordered as above inside each part
+  s->parts.clear();
+  std::vector<u32> porder;
+  porder.reserve(n_docs);
+  {
+    const u64 part_tokens = std::max<u64>(std::max<u64>(1, g_part_min_tokens), (n_tokens + kMaxParts - 1) / kMaxParts);
+    u32 doc = 0;
+    u64 tok = 0;
+    while (doc < n_docs) {
+      Segment::Part pt;
+      pt.doc0 = doc;
+      pt.tok0 = tok;
+      while (doc < n_docs && (doc == pt.doc0 || tok - pt.tok0 < part_tokens)) tok += d[doc++];
+      pt.doc1 = doc;
+      pt.tok1 = tok;
+      std::vector<u32> o(pt.doc1 - pt.doc0);
+      for (u32 i = 0; i < o.size(); ++i) o[i] = pt.doc0 + i;
+      std::stable_sort(o.begin(), o.end(), [&](u32 a, u32 b) {
+        if (lmax_of[a] != lmax_of[b]) return rank(lmax_of[a]) > rank(lmax_of[b]);
+        return d[a] > d[b];
+      });
+      for (u32 i = 0; i < o.size(); ++i) {
+        const u32 dd = o[i];
+        const u32 cap = map_capacity(d[dd], K);
+        if (pt.buckets.empty() || pt.buckets.back().lmax != lmax_of[dd])
+          pt.buckets.push_back({lmax_of[dd], (u32)porder.size() + i, 0, 0});
+        pt.buckets.back().count++;
+        pt.buckets.back().max_cap = std::max(pt.buckets.back().max_cap, cap);
+      }
+      porder.insert(porder.end(), o.begin(), o.end());
+      s->parts.push_back(std::move(pt));
+    }
+  }
+  CK(dalloc(&s->part_order, n_docs));
+  if (n_docs) HCK(hipMemcpy(s->part_order, porder.data(), (u64)n_docs * sizeof(u32), hipMemcpyHostToDevice));
   s->n_docs = n_docs;
   s->n_tokens = n_tokens;
   s->max_len = max_len;
@@ -340,7 +394,12 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
 // Library state.
 // ---------------------------------------------------------------------------------------------
 struct BufferState {
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;    // sweeps and counts of this buffer
+  hipStream_t copy_in = nullptr;   // host -> device copies of z
+  hipStream_t copy_out = nullptr;  // device -> host copies of z
+  hipEvent_t ev_in[kMaxParts] = {};
+  hipEvent_t ev_swept[kMaxParts] = {};
+  hipEvent_t ev_done = nullptr;
   u32* staging = nullptr;  // u32 z for the PCIe copies (buffer_size)
   u32* bad = nullptr;      // device count of indicators >= K
   Segment* seg = nullptr;  // segment of the last call (uncached mode owns it)
@@ -467,12 +526,6 @@ int launch_sweep_schedule(const SweepParams& P0, Segment* sg, hipStream_t st) {
     HCK(hipGetLastError());
   }
   if (n_long > 0) HCK(hipStreamWaitEvent(st, S.join, 0));
-  return 0;
-}
-
-int sweep_and_count(Segment* sg, u32 doc_base, hipStream_t st) {
-  CK(launch_sweep(sweep_params(sg, doc_base), sg, st));
-  CK(launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, S.n, S.K, st));
   return 0;
 }
 
@@ -612,7 +665,15 @@ int do_initialize(Args* args, Buffer* buffers, u32 n_buffers) {
   for (u32 i = 0; i < n_buffers; ++i) {
     Buffer* B = &buffers[i];
     BufferState* bs = new BufferState();
-    HCK(hipStreamCreate(&bs->stream));
+    // non-blocking: the synchronous uploads of another buffer's slice do not wait for this one
+    HCK(hipStreamCreateWithFlags(&bs->stream, hipStreamNonBlocking));
+    HCK(hipStreamCreateWithFlags(&bs->copy_in, hipStreamNonBlocking));
+    HCK(hipStreamCreateWithFlags(&bs->copy_out, hipStreamNonBlocking));
+    for (u32 p = 0; p < kMaxParts; ++p) {
+      HCK(hipEventCreateWithFlags(&bs->ev_in[p], hipEventDisableTiming));
+      HCK(hipEventCreateWithFlags(&bs->ev_swept[p], hipEventDisableTiming));
+    }
+    HCK(hipEventCreateWithFlags(&bs->ev_done, hipEventDisableTiming));
     CK(dalloc(&bs->staging, std::max<u32>(1, args->buffer_size)));
     CK(dalloc(&bs->bad, 1));
     B->gpu_z = bs->staging;
@@ -646,19 +707,46 @@ int do_sample_z_async(Buffer* B) {
   Segment* sg = nullptr;
   CK(get_segment(bs, B, &sg));
   hipStream_t st = bs->stream;
-  const u64 N = B->n_tokens;
   CK(fill_u32(bs->bad, 1, 0, st));
-  if (N) {
-    HCK(hipMemcpyAsync(bs->staging, B->z, N * sizeof(u32), hipMemcpyHostToDevice, st));
-    k_z_narrow<<<grid_for(N, 256), 256, 0, st>>>(bs->staging, sg->z, N, S.K, bs->bad);
-    HCK(hipGetLastError());
+  // Per part p: copy its z in (copy_in), narrow and sweep it (st), widen it and copy it out
+  // (copy_out).  The host enqueues the copy-out of part p - 1 after the sweep of part p, so that
+  // the copies overlap the sweeps even where a copy from pageable memory holds the host thread.
+  const SweepParams P = sweep_params(sg, S.doc_counter);
+  const u32 np = (u32)sg->parts.size();
+  if (np > kMaxParts) return fail("slice split into %u parts (limit %u)", np, kMaxParts);
+  HCK(hipEventRecord(bs->ev_done, st));  // this call's fill; the previous call ended at sync_buffer
+  HCK(hipStreamWaitEvent(bs->copy_in, bs->ev_done, 0));
+  auto copy_out = [&](u32 p) -> int {
+    const Segment::Part& pt = sg->parts[p];
+    const u64 n = pt.tok1 - pt.tok0;
+    HCK(hipStreamWaitEvent(bs->copy_out, bs->ev_swept[p], 0));
+    if (n) {
+      k_z_widen<<<grid_for(n, 256), 256, 0, bs->copy_out>>>(sg->z + pt.tok0, bs->staging + pt.tok0, n);
+      HCK(hipGetLastError());
+      HCK(hipMemcpyAsync(B->z + pt.tok0, bs->staging + pt.tok0, n * sizeof(u32), hipMemcpyDeviceToHost,
+                         bs->copy_out));
+    }
+    return 0;
+  };
+  for (u32 p = 0; p < np; ++p) {
+    const Segment::Part& pt = sg->parts[p];
+    const u64 n = pt.tok1 - pt.tok0;
+    if (n) HCK(hipMemcpyAsync(bs->staging + pt.tok0, B->z + pt.tok0, n * sizeof(u32), hipMemcpyHostToDevice,
+                              bs->copy_in));
+    HCK(hipEventRecord(bs->ev_in[p], bs->copy_in));
+    HCK(hipStreamWaitEvent(st, bs->ev_in[p], 0));
+    if (n) {
+      k_z_narrow<<<grid_for(n, 256), 256, 0, st>>>(bs->staging + pt.tok0, sg->z + pt.tok0, n, S.K, bs->bad);
+      HCK(hipGetLastError());
+    }
+    CK(launch_buckets(P, pt.buckets, sg->part_order, st));
+    HCK(hipEventRecord(bs->ev_swept[p], st));
+    if (p > 0) CK(copy_out(p - 1));
   }
-  CK(sweep_and_count(sg, S.doc_counter, st));
-  if (N) {
-    k_z_widen<<<grid_for(N, 256), 256, 0, st>>>(sg->z, bs->staging, N);
-    HCK(hipGetLastError());
-    HCK(hipMemcpyAsync(B->z, bs->staging, N * sizeof(u32), hipMemcpyDeviceToHost, st));
-  }
+  if (np > 0) CK(copy_out(np - 1));
+  CK(launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, S.n, S.K, st));
+  HCK(hipEventRecord(bs->ev_done, bs->copy_out));
+  HCK(hipStreamWaitEvent(st, bs->ev_done, 0));  // sync_buffer waits on st alone
   S.doc_counter += B->n_docs;
   B->gpu_w = sg->w;
   B->gpu_d_len = sg->doc_off;
@@ -687,6 +775,13 @@ int do_cleanup(Buffer* buffers, u32 n_buffers) {
     free_segment(bs->seg);
     dfree(bs->staging);
     dfree(bs->bad);
+    for (u32 p = 0; p < kMaxParts; ++p) {
+      if (bs->ev_in[p]) (void)hipEventDestroy(bs->ev_in[p]);
+      if (bs->ev_swept[p]) (void)hipEventDestroy(bs->ev_swept[p]);
+    }
+    if (bs->ev_done) (void)hipEventDestroy(bs->ev_done);
+    if (bs->copy_in) (void)hipStreamDestroy(bs->copy_in);
+    if (bs->copy_out) (void)hipStreamDestroy(bs->copy_out);
     if (bs->stream) (void)hipStreamDestroy(bs->stream);
     delete bs;
     B->gpu_z = B->gpu_w = B->gpu_d_len = B->gpu_d_idx = B->gpu_K_d = B->gpu_rng = B->stream = nullptr;
@@ -783,6 +878,12 @@ extern "C" int gpulda_device_info(char* name, int name_len, int* cu_count, uint6
   return 0;
 }
 
+extern "C" int gpulda_set_part_tokens(uint64_t min_tokens) {
+  if (min_tokens == 0) return fail("part size must be positive");
+  g_part_min_tokens = min_tokens;
+  return 0;
+}
+
 extern "C" int gpulda_set_slice_cache(int enabled) {
   S.slice_cache = enabled != 0;
   return 0;
@@ -830,6 +931,18 @@ extern "C" int gpulda_set_counts(const uint32_t* n) {
   CK(require_ready());
   HCK(hipDeviceSynchronize());
   HCK(hipMemcpy(S.n, n, (u64)S.V * S.K * sizeof(u32), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Sets the number of Phi steps taken so far; the next sample_phi is step iteration + 1.  Together
+// with n_wk restored from a saved z (gpulda_set_counts), a resumed run draws exactly what the
+// uninterrupted run would have drawn.
+extern "C" int gpulda_set_iteration(uint32_t iteration) {
+  CK(require_ready());
+  HCK(hipDeviceSynchronize());
+  S.iter = iteration;
+  CK(fill_u32(S.iter_dev, 1, iteration, S.stream));
+  HCK(hipStreamSynchronize(S.stream));
   return 0;
 }
 

@@ -127,6 +127,34 @@ __device__ __forceinline__ MapProbe map_probe(const u32* T, u32 cap, u32 key) {
   return p;
 }
 
+// Slot and entry of a key that is in the map.  Only the match is tested: a present key lies before
+// the first empty slot of its probe sequence, so no stop test is needed.  (The empty entry's key
+// field, 0xFFFF, never equals a topic.)
+__device__ __forceinline__ MapProbe map_find(const u32* T, u32 cap, u32 key) {
+  const u32 lane = lane_id();
+  const u32 h = map_home(key, cap);
+  wave_sync();
+  MapProbe p;
+  p.slot = 0;
+  p.found = 0;
+  p.entry = kMapEmpty;
+  for (u32 base = 0; base < cap; base += kMapLine) {
+    u32 s = h + base + lane;
+    if (s >= cap) s -= cap;
+    const u32 e = T[s];
+    const u64 bm = __ballot(map_key(e) == key);
+    if (bm) {
+      const u32 fl = first_lane(bm);
+      const u32 sl = h + base + fl;
+      p.slot = sl >= cap ? sl - cap : sl;
+      p.found = 1;
+      p.entry = readlane_u32(e, fl);
+      return p;
+    }
+  }
+  return p;
+}
+
 // Count of `key` +1 given its probe p; inserts it (Robin Hood shift) when absent.
 __device__ __forceinline__ u32 map_inc_at(u32* T, u32 cap, u32 key, MapProbe p) {
   const u32 lane = lane_id();
@@ -158,7 +186,8 @@ __device__ __forceinline__ u32 map_inc(u32* T, u32 cap, u32 key) { return map_in
 // Returns 1 if the key was removed.
 __device__ __forceinline__ u32 map_dec(u32* T, u32 cap, u32 key) {
   const u32 lane = lane_id();
-  MapProbe p = map_probe(T, cap, key);
+  MapProbe p = map_find(T, cap, key);
+  if (!p.found) return 0;  // not reached: every decremented topic is in the map
   if (map_count(p.entry) > 1) {
     if (lane == 0) T[p.slot] = p.entry - 1;
     return 0;

@@ -46,6 +46,10 @@ int gpulda_device_info(char* name, int name_len, int* cu_count, uint64_t* hbm_by
 /* 1 = cache device copies of buffer slices keyed by their host pointers (default), 0 = re-upload
  * w and d on every sample_z_async (for callers that refill one host buffer). */
 int gpulda_set_slice_cache(int enabled);
+/* sample_z_async splits a slice into up to 8 parts of whole documents and at least min_tokens
+ * tokens each (default 4 Mi), and overlaps the copies of z for one part with the sweep of another.
+ * Applies to slices first seen after the call.  Results do not depend on it. */
+int gpulda_set_part_tokens(uint64_t min_tokens);
 
 /* ---- model state (valid between initialize and cleanup) ---------------------------------- */
 uint32_t gpulda_iteration(void);                         /* number of sample_phi calls so far */
@@ -54,6 +58,9 @@ int gpulda_copy_alias(float* prob, uint32_t* alias);     /* V*K each */
 int gpulda_copy_sigma_a(float* out);                     /* V */
 int gpulda_copy_counts(uint32_t* out);                   /* n_wk, V*K */
 int gpulda_set_counts(const uint32_t* n);                /* overwrite n_wk, V*K */
+/* Number of sample_phi calls taken so far (the next one is iteration + 1).  With n_wk set to the
+ * counts of a saved z, a resumed run continues the uninterrupted chain bit for bit. */
+int gpulda_set_iteration(uint32_t iteration);
 int gpulda_copy_topic_sums(uint64_t* out);               /* sum_v phi~_vk of the last Phi step, K */
 int gpulda_copy_poisson_tables(float* prob, uint32_t* alias);  /* 100 x 200 each */
 /* Doc-topic map entries read by the sparse sums since the last reset (sum over tokens). */

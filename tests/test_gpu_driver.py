@@ -33,7 +33,7 @@ def test_driver_matches_oracle(tmp_path, orc):
     _write_formats(tmp, c, spec.V)
     n_mc, buffer_size, max_D, seed = 3, 2048, 32, 7
     exe = os.path.join(ROOT, "gpupolyaurnlda_amd", "gpulda_train")
-    out = subprocess.run([exe, "--alpha", str(spec.alpha), "--beta", str(spec.beta), "--K", str(spec.K),
+    out = subprocess.run([exe, "--stages", "train", "--alpha", str(spec.alpha), "--beta", str(spec.beta), "--K", str(spec.K),
                           "--buffer_size", str(buffer_size), "--max_D", str(max_D), "--n_mc", str(n_mc),
                           "--seed", str(seed),
                           "--z_temp_file", f"{tmp}/z.bin", "--w_temp_file", f"{tmp}/w.bin",
@@ -56,3 +56,128 @@ def test_driver_matches_oracle(tmp_path, orc):
             doc += nd
         model.n = orc.count(model.w, model.z, spec.V, spec.K)
     assert np.array_equal(z_gpu, model.z)
+
+
+def _replay(orc, w, d_len, z0, V, K, alpha, beta, seed, n_mc, buffer_size, max_D):
+    """The oracle replaying train.rs's buffer sequence from the given start."""
+    from gpupolyaurnlda_amd.train import fill_buffer
+    doc_off = np.concatenate([[0], np.cumsum(d_len)]).astype(np.uint64)
+    model = orc.Model(w, doc_off, z0, V, K, alpha, beta, seed=seed, counts="C")
+    for _ in range(n_mc):
+        model.phi_step()
+        doc = 0
+        while True:
+            nd, _ = fill_buffer(d_len, doc, buffer_size, max_D)
+            if nd == 0:
+                break
+            model.z_step(doc_base=doc, lo=doc, hi=doc + nd)
+            doc += nd
+        model.n = orc.count(model.w, model.z, V, K)
+    return model
+
+
+def _mallet_file(path, c):
+    """A Mallet-style input whose tokens are the synthetic word ids."""
+    with open(path, "w") as f:
+        for i in range(len(c["d_len"])):
+            toks = c["w"][c["doc_off"][i]:c["doc_off"][i + 1]]
+            f.write(f"doc{i}\tlabel\t" + " ".join(f"t{v}" for v in toks) + "\n")
+
+
+def _driver(tmp, *args):
+    exe = os.path.join(ROOT, "gpupolyaurnlda_amd", "gpulda_train")
+    cmd = [exe, "--z_temp_file", f"{tmp}/z.bin", "--w_temp_file", f"{tmp}/w.bin", "--d_temp_file", f"{tmp}/d.bin",
+           "--K_d_temp_file", f"{tmp}/k_d.bin", "--c_temp_file", f"{tmp}/c.txt", *map(str, args)]
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    return out.stdout
+
+
+def test_driver_full_pipeline(tmp_path, orc):
+    """preprocess -> train -> output as src/main.rs runs them, plus the Phi and theta exports:
+    the trained z equals the oracle's replay from the preprocessed files, Phi is the oracle's last
+    Phi bit for bit, and the output and theta files restate that z."""
+    from gpupolyaurnlda_amd import corpus
+    spec = corpus.TINY.scaled(200)
+    c = corpus.generate(spec)
+    tmp = str(tmp_path)
+    _mallet_file(f"{tmp}/in.txt", c)
+    K, seed, n_mc, buffer_size, max_D = spec.K, 11, 3, 4096, 64
+    common = ("--K", K, "--alpha", spec.alpha, "--beta", spec.beta, "--seed", seed)
+    _driver(tmp, "--stages", "preprocess", "--input", f"{tmp}/in.txt", *common)
+    z0 = np.fromfile(f"{tmp}/z.bin", "<u4")
+    w = np.fromfile(f"{tmp}/w.bin", "<u4")
+    d_len = np.fromfile(f"{tmp}/d.bin", "<u4")
+    names = [ln.split("\t")[0] for ln in open(f"{tmp}/c.txt").read().splitlines()]
+    V = len(names)
+    out = _driver(tmp, "--stages", "train,output", "--output", f"{tmp}/out.txt", "--phi_file", f"{tmp}/phi.bin",
+                  "--theta_file", f"{tmp}/theta.txt", "--n_mc", n_mc, "--buffer_size", buffer_size,
+                  "--max_D", max_D, *common)
+    assert "finished iteration: 2" in out and "writing output" in out
+    z = np.fromfile(f"{tmp}/z.bin", "<u4")
+    model = _replay(orc, w, d_len, z0, V, K, spec.alpha, spec.beta, seed, n_mc, buffer_size, max_D)
+    assert np.array_equal(z, model.z)
+    phi = np.fromfile(f"{tmp}/phi.bin", "<f4").reshape(V, K)
+    assert np.array_equal(phi.view(np.uint32), np.asarray(model.phi, np.float32).view(np.uint32))
+    off = np.concatenate([[0], np.cumsum(d_len)]).astype(np.int64)
+    want_out, want_theta = [], []
+    for i in range(len(d_len)):
+        zz, ww = z[off[i]:off[i + 1]], w[off[i]:off[i + 1]]
+        want_out.append("".join(f"{names[v]}:{k}\t" for v, k in zip(ww, zz)))
+        ks, ns = np.unique(zz, return_counts=True)
+        want_theta.append(" ".join(f"{k}:{n}" for k, n in zip(ks, ns)))
+    assert open(f"{tmp}/out.txt").read() == "".join(s + "\n" for s in want_out)
+    assert open(f"{tmp}/theta.txt").read() == "".join(s + "\n" for s in want_theta)
+
+
+def test_driver_checkpoint_resume(tmp_path):
+    """Two iterations, a checkpoint, then a resumed run to four (with another buffer size) give
+    the z.bin of four uninterrupted iterations, bit for bit."""
+    import shutil
+    from gpupolyaurnlda_amd import corpus
+    spec = corpus.TINY.scaled(300)
+    c = corpus.generate(spec)
+    base, a, b = (str(tmp_path / n) for n in ("base", "a", "b"))
+    os.makedirs(base)
+    _mallet_file(f"{base}/in.txt", c)
+    common = ("--K", spec.K, "--alpha", spec.alpha, "--beta", spec.beta, "--seed", 5)
+    _driver(base, "--stages", "preprocess", "--input", f"{base}/in.txt", *common)
+    shutil.copytree(base, a)
+    shutil.copytree(base, b)
+    _driver(a, "--stages", "train", "--n_mc", 4, "--buffer_size", 3000, "--max_D", 40, *common)
+    ck = f"{b}/ck.txt"
+    _driver(b, "--stages", "train", "--n_mc", 2, "--buffer_size", 3000, "--max_D", 40, "--checkpoint_file", ck,
+            *common)
+    assert "iterations 2" in open(ck).read()
+    out = _driver(b, "--stages", "train", "--n_mc", 4, "--buffer_size", 1000, "--max_D", 17, "--checkpoint_file", ck,
+                  "--resume", *common)
+    assert "resuming after iteration 2" in out and "finished iteration: 3" in out
+    assert "finished iteration: 1" not in out
+    assert "iterations 4" in open(ck).read()
+    za = np.fromfile(f"{a}/z.bin", "<u4")
+    zb = np.fromfile(f"{b}/z.bin", "<u4")
+    z0 = np.fromfile(f"{base}/z.bin", "<u4")
+    assert not np.array_equal(za, z0)
+    assert np.array_equal(za, zb)
+
+
+def test_driver_buffers_in_flight(tmp_path):
+    """Several buffers in flight (each synchronised before reuse) give the z.bin of one buffer
+    synchronised after every call, as train.rs runs it."""
+    import shutil
+    from gpupolyaurnlda_amd import corpus
+    spec = corpus.TINY.scaled(400)
+    c = corpus.generate(spec)
+    base, a, b = (str(tmp_path / n) for n in ("base", "a", "b"))
+    os.makedirs(base)
+    _mallet_file(f"{base}/in.txt", c)
+    common = ("--K", spec.K, "--alpha", spec.alpha, "--beta", spec.beta, "--seed", 9, "--n_mc", 3,
+              "--buffer_size", 1500, "--max_D", 30)
+    _driver(base, "--stages", "preprocess", "--input", f"{base}/in.txt", *common)
+    shutil.copytree(base, a)
+    shutil.copytree(base, b)
+    _driver(a, "--stages", "train", *common)
+    _driver(b, "--stages", "train", "--n_buffers", 3, *common)
+    za = np.fromfile(f"{a}/z.bin", "<u4")
+    assert not np.array_equal(za, np.fromfile(f"{base}/z.bin", "<u4"))
+    assert np.array_equal(za, np.fromfile(f"{b}/z.bin", "<u4"))

@@ -222,6 +222,39 @@ def test_abi_tiny_100_iterations(orc):
         assert smp.iteration == 100
 
 
+@pytest.mark.parametrize("buffer_size,max_D", [(1 << 20, 1 << 16), (4096, 64)])
+def test_abi_pipelined_parts(orc, buffer_size, max_D):
+    """sample_z_async split into parts whose copies overlap other parts' sweeps (small parts so
+    that a tiny corpus has up to eight of them): the same chain as the oracle."""
+    from gpupolyaurnlda_amd._lib import check, lib
+    from gpupolyaurnlda_amd.corpus import TINY
+    from gpupolyaurnlda_amd.train import Sampler, fill_buffer, train
+    c = _tiny_corpus()
+    w, d, z0 = c["w"], c["d_len"], c["z"]
+    k_d = np.zeros(len(d), np.uint32)
+    model = orc.Model(w, c["doc_off"], z0, TINY.V, TINY.K, TINY.alpha, TINY.beta, seed=2, counts="C", threads=8)
+    z = z0.copy()
+    check(lib().gpulda_set_part_tokens(257), "set_part_tokens")
+    try:
+        with Sampler(TINY.alpha, TINY.beta, TINY.K, c["C"], buffer_size, max_D, seed=2) as smp:
+            train(smp, z, w, d, k_d, n_mc=5, buffer_size=buffer_size, max_D=max_D)
+            n_gpu = smp.counts()
+    finally:
+        check(lib().gpulda_set_part_tokens(4 << 20), "set_part_tokens")
+    for _ in range(5):
+        model.phi_step()
+        doc = 0
+        while True:
+            nd, _ = fill_buffer(d, doc, buffer_size, max_D)
+            if nd == 0:
+                break
+            model.z_step(doc_base=doc, lo=doc, hi=doc + nd)
+            doc += nd
+        model.n = orc.count(model.w, model.z, TINY.V, TINY.K)
+    assert np.array_equal(z, model.z)
+    assert np.array_equal(n_gpu, model.n)
+
+
 def test_resident_matches_oracle_and_loglik(orc):
     from gpupolyaurnlda_amd.corpus import TINY
     from gpupolyaurnlda_amd.resident import ResidentCorpus
@@ -324,11 +357,21 @@ def test_autotune_keeps_chain():
         assert np.array_equal(rc.z(), z_tuned)
 
 
-def test_foldin_bit_exact(S, orc):
-    rng = np.random.default_rng(21)
-    V, K = 500, 256
-    phi, sig, prob, al = _tiny_phi(orc, V, K, seed=3, alpha=0.1)
-    L = np.concatenate([[1, 5], rng.integers(10, 800, 60)]).astype(np.uint32)
+@pytest.mark.parametrize("V,K,Lmax,n_req,zero_rows", [(500, 256, 800, 60, 0), (3000, 1024, 2500, 24, 40)])
+def test_foldin_bit_exact(S, orc, V, K, Lmax, n_req, zero_rows):
+    """Fold-in against the oracle.  The second case has maps beyond eight lines (the line-loop
+    draw) and words whose phi row is all zero (no draw), both of which end the next-token
+    gathers."""
+    rng = np.random.default_rng(21 + K)
+    if zero_rows:
+        pt = rng.integers(0, 5, size=(V, K)).astype(np.float32) * (rng.random((V, K)) < 0.6)
+        pt[:, 0] += 1
+        pt[rng.choice(V, zero_rows, replace=False)] = 0
+        phi, sig, prob, al = orc.phi_finish(pt, pt.sum(0).astype(np.uint64), 0.1)
+        assert (sig == 0).sum() == zero_rows
+    else:
+        phi, sig, prob, al = _tiny_phi(orc, V, K, seed=3, alpha=0.1)
+    L = np.concatenate([[1, 5], rng.integers(10, Lmax, n_req)]).astype(np.uint32)
     off = np.concatenate([[0], np.cumsum(L)]).astype(np.uint64)
     w = rng.integers(0, V, int(off[-1])).astype(np.uint32)
     ids = (np.arange(len(L)) * 7 + 100).astype(np.uint32)
