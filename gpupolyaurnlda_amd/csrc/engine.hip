@@ -194,7 +194,7 @@ struct Segment {
   std::vector<u32> lens_desc;  // document lengths in doc_lpt order (host copy)
   struct Bucket {
     int lmax;      // register-resident lines of the kernel (0 = generic)
-    u32 first;     // offset into doc_order (part_order for a part's buckets)
+    u32 first;     // offset into doc_order
     u32 count;
     u32 max_cap;   // largest map of the bucket (LDS slots)
   };
@@ -204,10 +204,10 @@ struct Segment {
   struct Part {
     u32 doc0, doc1;
     u64 tok0, tok1;
-    std::vector<Bucket> buckets;
+    u32 max_len;  // longest document of the part
   };
   std::vector<Part> parts;
-  u32* part_order = nullptr;  // each part's documents grouped by bucket, parts one after another
+  u32* part_lpt = nullptr;  // each part's documents longest first, parts one after another
 };
 
 constexpr u32 kMaxParts = 8;
@@ -249,7 +249,7 @@ void free_segment(Segment* s) {
   dfree(s->tasks);
   dfree(s->doc_order);
   dfree(s->doc_lpt);
-  dfree(s->part_order);
+  dfree(s->part_lpt);
   delete s;
 }
 
@@ -328,10 +328,9 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
     s->buckets.back().max_cap = std::max(s->buckets.back().max_cap, cap);
   }
   // pipeline parts: about n_tokens / kMaxParts tokens each (at least g_part_min_tokens), whole
-  // documents, buckets ordered as above inside each part
+  // documents, each part's documents longest first
   s->parts.clear();
-  std::vector<u32> porder;
-  porder.reserve(n_docs);
+  std::vector<u32> plpt(n_docs);
   {
     const u64 part_tokens = std::max<u64>(std::max<u64>(1, g_part_min_tokens), (n_tokens + kMaxParts - 1) / kMaxParts);
     u32 doc = 0;
@@ -343,26 +342,14 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
       while (doc < n_docs && (doc == pt.doc0 || tok - pt.tok0 < part_tokens)) tok += d[doc++];
       pt.doc1 = doc;
       pt.tok1 = tok;
-      std::vector<u32> o(pt.doc1 - pt.doc0);
-      for (u32 i = 0; i < o.size(); ++i) o[i] = pt.doc0 + i;
-      std::stable_sort(o.begin(), o.end(), [&](u32 a, u32 b) {
-        if (lmax_of[a] != lmax_of[b]) return rank(lmax_of[a]) > rank(lmax_of[b]);
-        return d[a] > d[b];
-      });
-      for (u32 i = 0; i < o.size(); ++i) {
-        const u32 dd = o[i];
-        const u32 cap = map_capacity(d[dd], K);
-        if (pt.buckets.empty() || pt.buckets.back().lmax != lmax_of[dd])
-          pt.buckets.push_back({lmax_of[dd], (u32)porder.size() + i, 0, 0});
-        pt.buckets.back().count++;
-        pt.buckets.back().max_cap = std::max(pt.buckets.back().max_cap, cap);
-      }
-      porder.insert(porder.end(), o.begin(), o.end());
-      s->parts.push_back(std::move(pt));
+      for (u32 i = pt.doc0; i < pt.doc1; ++i) plpt[i] = i;
+      std::stable_sort(plpt.begin() + pt.doc0, plpt.begin() + pt.doc1, [&](u32 a, u32 b) { return d[a] > d[b]; });
+      pt.max_len = d[plpt[pt.doc0]];
+      s->parts.push_back(pt);
     }
   }
-  CK(dalloc(&s->part_order, n_docs));
-  if (n_docs) HCK(hipMemcpy(s->part_order, porder.data(), (u64)n_docs * sizeof(u32), hipMemcpyHostToDevice));
+  CK(dalloc(&s->part_lpt, n_docs));
+  if (n_docs) HCK(hipMemcpy(s->part_lpt, plpt.data(), (u64)n_docs * sizeof(u32), hipMemcpyHostToDevice));
   s->n_docs = n_docs;
   s->n_tokens = n_tokens;
   s->max_len = max_len;
@@ -402,6 +389,7 @@ struct BufferState {
   hipEvent_t ev_done = nullptr;
   u32* staging = nullptr;  // u32 z for the PCIe copies (buffer_size)
   u32* bad = nullptr;      // device count of indicators >= K
+  u32* heads = nullptr;    // queue heads of the parts' persistent sweeps (kMaxParts)
   Segment* seg = nullptr;  // segment of the last call (uncached mode owns it)
 };
 
@@ -526,6 +514,26 @@ int launch_sweep_schedule(const SweepParams& P0, Segment* sg, hipStream_t st) {
     HCK(hipGetLastError());
   }
   if (n_long > 0) HCK(hipStreamWaitEvent(st, S.join, 0));
+  return 0;
+}
+
+// z-sweep of one pipeline part of a five-call slice: one persistent launch over the part's
+// documents, longest first (head: the part's queue counter, zero), at kAbiWavesPerCu wavefronts
+// per CU or as many as LDS holds -- the schedule the autotuner picks for the resident corpus on
+// MI355X.
+constexpr u32 kAbiWavesPerCu = 32;
+int launch_part_sweep(const SweepParams& P0, const Segment* sg, u32 p, u32* head, hipStream_t st) {
+  const Segment::Part& pt = sg->parts[p];
+  const u32 n = pt.doc1 - pt.doc0;
+  if (n == 0) return 0;
+  SweepParams P = P0;
+  P.doc_order = sg->part_lpt + pt.doc0;
+  const u64 lds = ((u64)map_capacity(pt.max_len, S.K) + (S.K + 31) / 32) * 4;
+  if (lds > (u64)max_lds_per_block()) return fail("doc-topic map exceeds LDS");
+  const u32 per_cu = std::max<u32>(1, std::min<u32>(kAbiWavesPerCu, (u32)(lds_per_cu() / lds)));
+  const u32 waves = std::min<u32>((u32)S.cu_count * per_cu, n);
+  k_sweep_persistent<8><<<waves, 64, lds, st>>>(P, head, n);
+  HCK(hipGetLastError());
   return 0;
 }
 
@@ -676,6 +684,7 @@ int do_initialize(Args* args, Buffer* buffers, u32 n_buffers) {
     HCK(hipEventCreateWithFlags(&bs->ev_done, hipEventDisableTiming));
     CK(dalloc(&bs->staging, std::max<u32>(1, args->buffer_size)));
     CK(dalloc(&bs->bad, 1));
+    CK(dalloc(&bs->heads, kMaxParts));
     B->gpu_z = bs->staging;
     B->gpu_w = nullptr;
     B->gpu_d_len = nullptr;
@@ -708,6 +717,7 @@ int do_sample_z_async(Buffer* B) {
   CK(get_segment(bs, B, &sg));
   hipStream_t st = bs->stream;
   CK(fill_u32(bs->bad, 1, 0, st));
+  CK(fill_u32(bs->heads, kMaxParts, 0, st));
   // Per part p: copy its z in (copy_in), narrow and sweep it (st), widen it and copy it out
   // (copy_out).  The host enqueues the copy-out of part p - 1 after the sweep of part p, so that
   // the copies overlap the sweeps even where a copy from pageable memory holds the host thread.
@@ -739,7 +749,7 @@ int do_sample_z_async(Buffer* B) {
       k_z_narrow<<<grid_for(n, 256), 256, 0, st>>>(bs->staging + pt.tok0, sg->z + pt.tok0, n, S.K, bs->bad);
       HCK(hipGetLastError());
     }
-    CK(launch_buckets(P, pt.buckets, sg->part_order, st));
+    CK(launch_part_sweep(P, sg, p, bs->heads + p, st));
     HCK(hipEventRecord(bs->ev_swept[p], st));
     if (p > 0) CK(copy_out(p - 1));
   }
@@ -775,6 +785,7 @@ int do_cleanup(Buffer* buffers, u32 n_buffers) {
     free_segment(bs->seg);
     dfree(bs->staging);
     dfree(bs->bad);
+    dfree(bs->heads);
     for (u32 p = 0; p < kMaxParts; ++p) {
       if (bs->ev_in[p]) (void)hipEventDestroy(bs->ev_in[p]);
       if (bs->ev_swept[p]) (void)hipEventDestroy(bs->ev_swept[p]);
