@@ -175,8 +175,8 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
     u64 a = valid ? alias_fixed_weight(x[k], T, m) * m : 0;
     bool heavy = valid && a >= Q;
     bool light = valid && a < Q;
-    u64 hb = __ballot(heavy);
-    u64 lb = __ballot(light);
+    u64 hb = ballot(heavy);
+    u64 lb = ballot(light);
     u64 sur = heavy ? a - Q : 0;
     u64 def = light ? Q - a : 0;
     u64 incS = wave_scan_u64(sur);
@@ -205,8 +205,8 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
     u64 a = valid ? alias_fixed_weight(x[k], T, m) * m : 0;
     bool heavy = valid && a >= Q;
     bool light = valid && a < Q;
-    u64 hb = __ballot(heavy);
-    u64 lb = __ballot(light);
+    u64 hb = ballot(heavy);
+    u64 lb = ballot(light);
     AliasSlot s;
     s.prob = 1.0f;
     s.alias = k;
@@ -260,14 +260,12 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
 // ---------------------------------------------------------------------------------------------
 constexpr u32 kNoSlot = 0xFFFFFFFFu;
 
-// Mass of one map entry for row prow.  An empty slot counts 0 at key 0, so the gather and the
-// product are unconditional (no branch, all gathers of a token in flight together) and its mass
-// is 0 * phi[w][0] = +0, the value the definition gives it.
+// Mass of one map entry for row prow.  An empty slot (0) reads as count 0 at key 0, so the gather
+// and the product are unconditional (no branch, all gathers of a token in flight together) and
+// its mass is 0 * phi[w][0] = +0, the value the definition gives it.
 __device__ __forceinline__ f32 slot_mass(u32 e, const f32* __restrict__ prow) {
-  const bool empty = e == kMapEmpty;
-  const u32 cnt = empty ? 0u : map_count(e);
-  const u32 key = empty ? 0u : map_key(e);
-  return (f32)cnt * prow[key];
+  static_assert(kMapEmpty == 0u, "an empty slot must read as count 0 at key 0");
+  return (f32)map_count(e) * prow[map_key(e)];
 }
 
 __device__ __forceinline__ f32 combine_groups(f32 g0, f32 g1, f32 g2, f32 g3) { return ((g0 + g1) + g2) + g3; }
@@ -362,8 +360,8 @@ struct Pick {
 
 // Lane l* of the sparse pick and its exclusive prefix E.
 __device__ __forceinline__ u32 pick_lane(f32 A, f32 P, f32 tgt, f32& E) {
-  const u64 c = __ballot(A > 0.0f && P > tgt);
-  const u64 pos = __ballot(A > 0.0f);
+  const u64 pos = ballot(A > 0.0f);
+  const u64 c = pos & ballot(P > tgt);
   const u32 ls = c ? first_lane(c) : 63u - (u32)__clzll((long long)pos);
   E = ls ? readlane_f32(P, ls - 1) : 0.0f;
   return ls;
@@ -380,9 +378,9 @@ __device__ __forceinline__ u32 column_pick(const u32* T, u32 lines, u32 ls, cons
     const u32 j = c0 + lane;
     const f32 m = slot_mass(j < lines ? T[j * 64 + ls] : kMapEmpty, prow);
     const f32 Q = wave_scan_f32(m);
-    const u64 hit = __ballot(m > 0.0f && E + (carry + Q) > tgt);
+    const u64 pos = ballot(m > 0.0f);
+    const u64 hit = pos & ballot(E + (carry + Q) > tgt);
     if (hit) return c0 + first_lane(hit);
-    const u64 pos = __ballot(m > 0.0f);
     if (pos) last = c0 + 63u - (u32)__clzll((long long)pos);
     carry = carry + readlane_f32(Q, 63);
   }
@@ -428,15 +426,19 @@ __device__ __forceinline__ Pick draw_topic_regs(const u32* T, u32 lines, const f
   for (int off = 1; off < 16; off <<= 1)
 #pragma unroll
     for (int j = NL - 1; j >= off; --j) q[j] = q[j] + q[j - off];
-  u32 hit = kNoSlot, last = 0;
+  u32 hit = kNoSlot;
 #pragma unroll
   for (int j = NL - 1; j >= 0; --j) {
     if (m[j] > 0.0f && E + q[j] > tgt) hit = (u32)j;  // carry 0: E + (0 + q) == E + q
   }
+  u32 js = readlane_u32(hit, ls);
+  if (js == kNoSlot) {  // no line passes the target (rounding): the last line with mass
+    u32 last = 0;
 #pragma unroll
-  for (int j = 0; j < NL; ++j)
-    if (m[j] > 0.0f) last = (u32)j;
-  const u32 js = readlane_u32(hit != kNoSlot ? hit : last, ls);
+    for (int j = 0; j < NL; ++j)
+      if (m[j] > 0.0f) last = (u32)j;
+    js = readlane_u32(last, ls);
+  }
   u32 es = e[0];
 #pragma unroll
   for (int j = 1; j < NL; ++j)
@@ -912,7 +914,7 @@ __global__ __launch_bounds__(64) void k_serve_worker(SweepParams P, FoldParams F
           out_of_range |= w >= R.V ? 1u : 0u;
         }
         for (u32 off = 32; off > 0; off >>= 1) fp += __shfl_xor(fp, off, 64);
-        const bool range_ok = __ballot(out_of_range != 0u) == 0ull;
+        const bool range_ok = ballot(out_of_range != 0u) == 0ull;
         ok = fp == want_sum && range_ok;
         if (ok || wall_clock64() - t0 > R.lifetime_ticks) break;
         __builtin_amdgcn_s_sleep(4);

@@ -108,10 +108,12 @@ LDA_HD u32 bounded(u32 r, u32 n) { return (u32)(((u64)r * (u64)n) >> 32); }
 // history-independent: Robin Hood linear probing with ties broken by key, and deletion by
 // backward shift, so its layout is a function of the current key set only.
 // ---------------------------------------------------------------------------------------------
-constexpr u32 kMapEmpty = 0xFFFFFFFFu;
+// An empty slot holds 0 (a stored entry has a count of at least 1), so it reads as count 0 at
+// key 0 and its mass count * phi[w][key] is +0 with no test.
+constexpr u32 kMapEmpty = 0u;
 constexpr u32 kMapLine = 64;          // one wavefront probes 64 consecutive slots
 constexpr u32 kMaxDocLen = 65535;     // counts are 16-bit
-constexpr u32 kMaxTopics = 65535;     // keys are 16-bit, 0xFFFF marks empty
+constexpr u32 kMaxTopics = 65536;     // keys are 16-bit
 
 // Completion flag of a served request: a fingerprint of its words (sum of w_i * (2i + 1), any
 // order), length and request id, never 0 (0 means not finished).

@@ -36,6 +36,10 @@ __device__ __forceinline__ u64 lanemask_lt() {
 
 __device__ __forceinline__ u32 first_lane(u64 mask) { return (u32)(__ffsll((unsigned long long)mask) - 1); }
 
+// Mask of the active lanes where pred holds.  The builtin takes the comparison's lane mask as it
+// is (no 0/1 value materialised and compared again, as __ballot's int argument requires).
+__device__ __forceinline__ u64 ballot(bool pred) { return (u64)__builtin_amdgcn_ballot_w64(pred); }
+
 __device__ __forceinline__ f32 readlane_f32(f32 v, u32 l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)l));
 }
@@ -46,8 +50,9 @@ __device__ __forceinline__ u32 readlane_u32(u32 v, u32 l) {
 // Canonical inclusive f32 scan over the 64 lanes of a wavefront:
 //   1. Hillis-Steele inside each 16-lane row with offsets 1, 2, 4, 8 (lanes below the offset
 //      keep their value);
-//   2. row carries c1 = T0, c2 = T0 + T1, c3 = c2 + T2 (T_r = total of row r), added to rows
-//      1..3.
+//   2. rows 1 and 3 add the last value of rows 0 and 2 (DPP row_bcast:15);
+//   3. rows 2 and 3 add lane 31 (row 1's last value after step 2: T1 + T0; DPP row_bcast:31).
+//   So with row totals T_r: row 1 += T0, row 2 += T1 + T0, row 3 gets (x + T2) + (T1 + T0).
 // The CPU checker evaluates the same network, so results agree bit for bit.
 // Row shift-right by n inside 16-lane rows; lanes below n read 0 (DPP row_shr:n, bound_ctrl).
 // Adding that 0 leaves the value unchanged, which is the "keep" of the row-local step; values
@@ -57,19 +62,21 @@ __device__ __forceinline__ f32 row_shr_zero(f32 v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 + N, 0xF, 0xF, true));
 }
 
+// Broadcast steps: rows outside row_mask read 0 (and so keep their value after the add).
+__device__ __forceinline__ f32 row_bcast15_odd(f32 v) {  // rows 1, 3 <- lane 15 of rows 0, 2
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xA, 0xF, false));
+}
+__device__ __forceinline__ f32 row_bcast31_high(f32 v) {  // rows 2, 3 <- lane 31
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x143, 0xC, 0xF, false));
+}
+
 __device__ __forceinline__ f32 wave_scan_f32(f32 v) {
-  const u32 lane = lane_id();
   v = v + row_shr_zero<1>(v);
   v = v + row_shr_zero<2>(v);
   v = v + row_shr_zero<4>(v);
   v = v + row_shr_zero<8>(v);
-  f32 t0 = readlane_f32(v, 15);
-  f32 t1 = readlane_f32(v, 31);
-  f32 t2 = readlane_f32(v, 47);
-  f32 c2 = t0 + t1;
-  f32 c3 = c2 + t2;
-  f32 c = lane < 32 ? t0 : (lane < 48 ? c2 : c3);
-  if (lane >= 16) v = v + c;
+  v = v + row_bcast15_odd(v);
+  v = v + row_bcast31_high(v);
   return v;
 }
 
@@ -88,6 +95,9 @@ __device__ __forceinline__ u64 wave_scan_u64(u64 v) {
 // Doc-topic map operations.  T points at `cap` LDS slots (cap a multiple of 64, never full).
 // All 64 lanes of the wave call these with the same (uniform) arguments.
 // ---------------------------------------------------------------------------------------------
+// x mod cap for x < 2 * cap: x - cap wraps past 2^32 when x < cap, so the minimum is the slot.
+__device__ __forceinline__ u32 wrap_slot(u32 x, u32 cap) { return min(x, x - cap); }
+
 struct MapProbe {
   u32 slot;   // slot of the key, or the slot where it belongs
   u32 found;  // 1 if present
@@ -104,16 +114,17 @@ __device__ __forceinline__ MapProbe map_probe(const u32* T, u32 cap, u32 key) {
   p.entry = kMapEmpty;
   for (u32 base = 0; base < cap; base += kMapLine) {
     u32 d = base + lane;
-    u32 s = h + d;
-    if (s >= cap) s -= cap;
+    const u32 s = wrap_slot(h + d, cap);
     u32 e = T[s];
     u32 ek = map_key(e);
     bool empty = e == kMapEmpty;
-    u32 de = empty ? 0 : map_disp(s, map_home(ek, cap), cap);
-    bool match = !empty && ek == key;
-    bool stop = empty || de < d || (de == d && ek > key);
-    u64 bm = __ballot(match);
-    u64 bs = __ballot(stop);
+    // (computed for empty slots too, where the stop mask below already holds)
+    u32 de = map_disp(s, map_home(ek, cap), cap);
+    // match = !empty && ek == key; stop = empty || de < d || (de == d && ek > key), combined as
+    // lane masks of single comparisons
+    const u64 b_empty = ballot(empty);
+    u64 bm = ballot(ek == key) & ~b_empty;
+    u64 bs = b_empty | ballot(de < d) | (ballot(de == d) & ballot(ek > key));
     u64 ev = bm | bs;
     if (ev) {
       u32 fl = first_lane(ev);
@@ -128,8 +139,7 @@ __device__ __forceinline__ MapProbe map_probe(const u32* T, u32 cap, u32 key) {
 }
 
 // Slot and entry of a key that is in the map.  Only the match is tested: a present key lies before
-// the first empty slot of its probe sequence, so no stop test is needed.  (The empty entry's key
-// field, 0xFFFF, never equals a topic.)
+// the first empty slot of its probe sequence, so no stop test is needed.
 __device__ __forceinline__ MapProbe map_find(const u32* T, u32 cap, u32 key) {
   const u32 lane = lane_id();
   const u32 h = map_home(key, cap);
@@ -139,10 +149,9 @@ __device__ __forceinline__ MapProbe map_find(const u32* T, u32 cap, u32 key) {
   p.found = 0;
   p.entry = kMapEmpty;
   for (u32 base = 0; base < cap; base += kMapLine) {
-    u32 s = h + base + lane;
-    if (s >= cap) s -= cap;
+    const u32 s = wrap_slot(h + base + lane, cap);
     const u32 e = T[s];
-    const u64 bm = __ballot(map_key(e) == key);
+    const u64 bm = ballot(e != kMapEmpty) & ballot(map_key(e) == key);
     if (bm) {
       const u32 fl = first_lane(bm);
       const u32 sl = h + base + fl;
@@ -165,10 +174,9 @@ __device__ __forceinline__ u32 map_inc_at(u32* T, u32 cap, u32 key, MapProbe p) 
   u32 carry = map_entry(key, 1);
   for (u32 base = 0; base < cap; base += kMapLine) {
     wave_sync();
-    u32 s = p.slot + base + lane;
-    if (s >= cap) s -= cap;
+    const u32 s = wrap_slot(p.slot + base + lane, cap);
     u32 e = T[s];
-    u64 be = __ballot(e == kMapEmpty);
+    u64 be = ballot(e == kMapEmpty);
     u32 fe = be ? first_lane(be) : 64;
     u32 prev = __shfl_up(e, 1, 64);
     if (lane == 0) prev = carry;
@@ -196,12 +204,9 @@ __device__ __forceinline__ u32 map_dec(u32* T, u32 cap, u32 key) {
   if (start >= cap) start -= cap;
   for (u32 base = 0; base < cap; base += kMapLine) {
     wave_sync();
-    u32 s = start + base + lane;
-    if (s >= cap) s -= cap;
+    const u32 s = wrap_slot(start + base + lane, cap);
     u32 e = T[s];
-    bool empty = e == kMapEmpty;
-    bool stop = empty || map_disp(s, map_home(map_key(e), cap), cap) == 0;
-    u64 bs = __ballot(stop);
+    u64 bs = ballot(e == kMapEmpty) | ballot(map_disp(s, map_home(map_key(e), cap), cap) == 0);
     u32 fs = bs ? first_lane(bs) : 64;
     u32 dst = s == 0 ? cap - 1 : s - 1;
     if (lane <= fs) T[dst] = lane < fs ? e : kMapEmpty;

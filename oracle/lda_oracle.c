@@ -156,8 +156,9 @@ u32 orc_poisson_ptrs(double lam, u32 c0, u32 c1, u32 c2, u32 tag, u64 seed) {
 
 /* ------------------------------------------------------------------------------------------ */
 /* Canonical 64-lane f32 networks (shared definition with the device kernels).                 */
-/* scan: Hillis-Steele inside each 16-lane row (offsets 1,2,4,8), then row carries             */
-/*       c1 = T0, c2 = T0 + T1, c3 = c2 + T2 (T_r = row r total), lane value += c_r for r>0.   */
+/* scan: Hillis-Steele inside each 16-lane row (offsets 1,2,4,8), then rows 1 and 3 add the    */
+/*       last value of rows 0 and 2, then rows 2 and 3 add lane 31 (T1 + T0): row 3 ends as    */
+/*       (x + T2) + (T1 + T0) (T_r = row r total).                                             */
 /* ------------------------------------------------------------------------------------------ */
 void orc_scan64(float v[64]) {
   for (int off = 1; off < 16; off <<= 1) {
@@ -165,12 +166,12 @@ void orc_scan64(float v[64]) {
     for (int l = 0; l < 64; ++l) nv[l] = (l % 16 >= off) ? v[l] + v[l - off] : v[l];
     memcpy(v, nv, sizeof(nv));
   }
-  float t0 = v[15], t1 = v[31], t2 = v[47];
-  float c[4];
-  c[1] = t0;
-  c[2] = t0 + t1;
-  c[3] = c[2] + t2;
-  for (int l = 16; l < 64; ++l) v[l] = v[l] + c[l / 16];
+  /* rows 1 and 3 add the last value of rows 0 and 2; then rows 2 and 3 add lane 31 */
+  float t0 = v[15], t2 = v[47];
+  for (int l = 16; l < 32; ++l) v[l] = v[l] + t0;
+  for (int l = 48; l < 64; ++l) v[l] = v[l] + t2;
+  float c = v[31];
+  for (int l = 32; l < 64; ++l) v[l] = v[l] + c;
 }
 
 /* Sum of x[0..n): lane l accumulates x[l], x[l+64], ... left to right, then scan64's total. */
@@ -328,7 +329,7 @@ void orc_phi_finish(float* phi /* in: phi~, out: phi */, const u64* topic_sum, u
 /* Doc-topic map: textbook sequential Robin Hood linear probing, ties broken by key, backward  */
 /* shift deletion.  Restates the HashMap of hashmap.cuh:13-334 (entry = key<<16 | count).     */
 /* ------------------------------------------------------------------------------------------ */
-#define MAP_EMPTY 0xFFFFFFFFu
+#define MAP_EMPTY 0u /* a stored entry has count >= 1, so 0 never names one */
 
 u32 orc_map_capacity(u32 L, u32 K) {
   u32 need = L < K ? L : K;

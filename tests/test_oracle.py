@@ -199,7 +199,7 @@ def test_poisson_draws_chisquare(orc, n):
 
 def _check_map(T, cap, counts: dict, orc):
     home = lambda k: ((k * 0x9E3779B1) % 2**32 * cap) >> 32  # noqa: E731
-    EMPTY = 0xFFFFFFFF
+    EMPTY = 0  # an empty slot; a stored entry has count >= 1
     seen = {}
     for s in range(cap):
         e = int(T[s])
@@ -254,12 +254,12 @@ def test_map_reference_known_answers(orc):
     # count_topics (test_topics.cu:134-189): 100 tokens, z = i % 25 -> each count 4
     cap = orc.map_capacity(100, 25)
     T = orc.map_apply([i % 25 for i in range(100)], cap)
-    got = {int(e) >> 16: int(e) & 0xFFFF for e in T if e != 0xFFFFFFFF}
+    got = {int(e) >> 16: int(e) & 0xFFFF for e in T if e != 0}
     assert got == {k: 4 for k in range(25)}
     # test_hashmap.cu:224-266: 90 inserts over 9 keys -> every count is 10
     cap = orc.map_capacity(90, 9)
     T = orc.map_apply([i % 9 for i in range(90)], cap)
-    got = {int(e) >> 16: int(e) & 0xFFFF for e in T if e != 0xFFFFFFFF}
+    got = {int(e) >> 16: int(e) & 0xFFFF for e in T if e != 0}
     assert got == {k: 10 for k in range(9)}
     assert orc.map_capacity(100, 1024) == 128 and orc.map_capacity(50, 20) == 64
     assert orc.map_capacity(5000, 4096) == 64 * math.ceil((4096 + 1024 + 1) / 64)
