@@ -387,16 +387,16 @@ __device__ __forceinline__ u32 column_pick(const u32* T, u32 lines, u32 ls, cons
   return last;
 }
 
-// Sparse/dense pick with every line of the map held in registers (NL >= lines): all phi gathers
-// of the token are issued before any arithmetic on them.
+// Sparse/dense pick with the NL lines of the map held in registers (one variant per line count):
+// all phi gathers of the token are issued before any arithmetic on them.
 template <int NL>
-__device__ __forceinline__ Pick draw_topic_regs(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa,
-                                                u32 rx, u32 rw, u32 dense) {
+__device__ __forceinline__ Pick draw_topic_regs(const u32* T, const f32* __restrict__ prow, f32 sa, u32 rx, u32 rw,
+                                                u32 dense) {
   const u32 lane = lane_id();
   u32 e[NL];
   f32 m[NL];
 #pragma unroll
-  for (int j = 0; j < NL; ++j) e[j] = (u32)j < lines ? T[j * 64 + lane] : kMapEmpty;
+  for (int j = 0; j < NL; ++j) e[j] = T[j * 64 + lane];
 #pragma unroll
   for (int j = 0; j < NL; ++j) m[j] = slot_mass(e[j], prow);
   // G_g starts at line g's mass (0 + m == m, masses are never -0)
@@ -490,15 +490,23 @@ __device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const f
 }
 
 // Sparse/dense pick for one token (sigma_a > 0).  The register variant is chosen by the map's
-// current line count, up to LMAX lines (LMAX caps the registers of a kernel); larger maps use
-// the line-loop form.  All variants evaluate the same operations in the same order.
+// current line count (up to 8, within the kernel's LMAX, which caps its registers); larger maps
+// use the line-loop form.  All variants evaluate the same operations in the same order: a
+// variant for fewer lines drops only lines that would hold no mass.
 template <int LMAX>
 __device__ __forceinline__ Pick draw_topic(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa, u32 rx,
                                            u32 rw, u32 dense) {
-  if (LMAX >= 1 && lines <= 1) return draw_topic_regs<1>(T, lines, prow, sa, rx, rw, dense);
-  if (LMAX >= 2 && lines <= 2) return draw_topic_regs<2>(T, lines, prow, sa, rx, rw, dense);
-  if (LMAX >= 4 && lines <= 4) return draw_topic_regs<4>(T, lines, prow, sa, rx, rw, dense);
-  if (LMAX >= 8 && lines <= 8) return draw_topic_regs<8>(T, lines, prow, sa, rx, rw, dense);
+  switch (lines) {  // a map always has at least one line
+    case 1: if (LMAX >= 1) return draw_topic_regs<1>(T, prow, sa, rx, rw, dense); break;
+    case 2: if (LMAX >= 2) return draw_topic_regs<2>(T, prow, sa, rx, rw, dense); break;
+    case 3: if (LMAX >= 4) return draw_topic_regs<3>(T, prow, sa, rx, rw, dense); break;
+    case 4: if (LMAX >= 4) return draw_topic_regs<4>(T, prow, sa, rx, rw, dense); break;
+    case 5: if (LMAX >= 8) return draw_topic_regs<5>(T, prow, sa, rx, rw, dense); break;
+    case 6: if (LMAX >= 8) return draw_topic_regs<6>(T, prow, sa, rx, rw, dense); break;
+    case 7: if (LMAX >= 8) return draw_topic_regs<7>(T, prow, sa, rx, rw, dense); break;
+    case 8: if (LMAX >= 8) return draw_topic_regs<8>(T, prow, sa, rx, rw, dense); break;
+    default: break;
+  }
   return draw_topic_loop(T, lines, prow, sa, rx, rw, dense);
 }
 

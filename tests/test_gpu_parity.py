@@ -128,7 +128,7 @@ def _tiny_phi(orc, V, K, seed=5, alpha=0.3):
 
 
 @pytest.mark.parametrize("K,V,D,meanL", [(5, 3, 200, 6), (20, 1000, 500, 50), (256, 2000, 300, 300),
-                                         (1024, 3000, 40, 1500)])
+                                         (1024, 3000, 40, 1500), (4096, 2000, 12, 6000)])
 def test_sweep_bit_exact(S, orc, K, V, D, meanL):
     rng = np.random.default_rng(K + D)
     phi, sig, prob, al = _tiny_phi(orc, V, K, seed=K)
@@ -144,6 +144,22 @@ def test_sweep_bit_exact(S, orc, K, V, D, meanL):
     assert vg == vc
     assert np.array_equal(zg, zc)
     assert not np.array_equal(zg, z)
+
+
+def test_sweep_length_limits(S, orc):
+    """Empty documents beside one of the longest allowed length (65,535 tokens, 16-bit counts)."""
+    rng = np.random.default_rng(77)
+    V, K = 500, 64
+    phi, sig, prob, al = _tiny_phi(orc, V, K, seed=9)
+    L = np.array([0, 65535, 0, 3, 0], np.uint32)
+    off = np.concatenate([[0], np.cumsum(L)]).astype(np.uint64)
+    N = int(off[-1])
+    w = rng.integers(0, V, N).astype(np.uint32)
+    z = np.zeros(N, np.uint32)  # one topic holds every token of the long document
+    zg, vg = S.sweep(z, w, off, 5, 1, 3, K, phi, sig, prob, al)
+    zc, vc = orc.sweep(z, w, off, 5, 1, 3, K, phi, sig, prob, al, threads=8)
+    assert vg == vc and np.array_equal(zg, zc)
+    assert len(np.unique(zg[1:65536])) > 1
 
 
 def test_sweep_exact_conditional_gpu(S, orc):
