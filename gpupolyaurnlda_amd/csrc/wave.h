@@ -148,19 +148,20 @@ __device__ __forceinline__ MapProbe map_find(const u32* T, u32 cap, u32 key) {
   p.slot = 0;
   p.found = 0;
   p.entry = kMapEmpty;
-  for (u32 base = 0; base < cap; base += kMapLine) {
-    const u32 s = wrap_slot(h + base + lane, cap);
-    const u32 e = T[s];
-    const u64 bm = ballot(e != kMapEmpty) & ballot(map_key(e) == key);
-    if (bm) {
-      const u32 fl = first_lane(bm);
-      const u32 sl = h + base + fl;
-      p.slot = sl >= cap ? sl - cap : sl;
-      p.found = 1;
-      p.entry = readlane_u32(e, fl);
-      return p;
-    }
+  // the first line holds the key unless it was displaced past it (the common case is peeled)
+  u32 base = 0;
+  u32 e = T[wrap_slot(h + lane, cap)];
+  u64 bm = ballot(e != kMapEmpty) & ballot(map_key(e) == key);
+  while (!bm) {
+    base += kMapLine;
+    if (base >= cap) return p;
+    e = T[wrap_slot(h + base + lane, cap)];
+    bm = ballot(e != kMapEmpty) & ballot(map_key(e) == key);
   }
+  const u32 fl = first_lane(bm);
+  p.slot = wrap_slot(h + base + fl, cap);
+  p.found = 1;
+  p.entry = readlane_u32(e, fl);
   return p;
 }
 
