@@ -557,7 +557,7 @@ __device__ __forceinline__ void sweep_pass(const SweepParams& P, u32* T, DocStat
     for (u32 t = 0; t < nt; ++t) {
       const u32 wt = readlane_u32(ww, t);
       const u32 zt = readlane_u32(zz, t);
-      ds.occupied -= map_dec(T, ds.cap, zt);
+      ds.occupied -= uniform_u32(map_dec(T, ds.cap, zt));
       const f32 sa = readlane_f32(sa_l, t);
       Pick pk;
       pk.topic = zt;
@@ -565,7 +565,7 @@ __device__ __forceinline__ void sweep_pass(const SweepParams& P, u32* T, DocStat
       if (sa > 0.0f) {
         wave_sync();
         visited += ds.occupied;
-        pk = draw_topic<LMAX>(T, ds.cap / kMapLine, P.phi + (u64)wt * K, sa, readlane_u32(r.x, t),
+        pk = draw_topic<LMAX>(T, uniform_u32(ds.cap / kMapLine), P.phi + (u64)wt * K, sa, readlane_u32(r.x, t),
                               readlane_u32(r.w, t), readlane_u32(dense_l, t));
       }
       add_topic(P, T, ds, pk, c0, t, znew_lane, zz);
@@ -698,7 +698,7 @@ __global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u
         const u32 zt = readlane_u32(zz, t);
         const f32 sa = readlane_f32(sa_l, t);
         if (wave == 0) {
-          ds.occupied -= map_dec(T, ds.cap, zt);
+          ds.occupied -= uniform_u32(map_dec(T, ds.cap, zt));
           if (lane == 0) sh_cap = ds.cap;
         }
         __syncthreads();

@@ -99,10 +99,19 @@ __device__ __forceinline__ u64 wave_scan_u64(u64 v) {
 __device__ __forceinline__ u32 wrap_slot(u32 x, u32 cap) { return min(x, x - cap); }
 
 struct MapProbe {
-  u32 slot;   // slot of the key, or the slot where it belongs
-  u32 found;  // 1 if present
-  u32 entry;  // entry at `slot` when found
+  u32 slot;       // slot of the key, or the slot where it belongs
+  u32 found;      // 1 if present
+  u32 entry;      // entry at `slot` when found
+  u32 lane;       // lane of `slot` in the line the probe stopped in
+  u32 line_slot;  // per lane: the slot this lane read in that line
+  u32 line_e;     // per lane: the entry it read there
 };
+
+// Rewrites a line the wave has just read, with one lane's entry replaced: every lane stores its
+// own slot, so no lane needs to be masked off (no exec-mask change on the scalar unit).
+__device__ __forceinline__ void map_store_line(u32* T, u32 line_slot, u32 line_e, u32 at_lane, u32 entry) {
+  T[line_slot] = lane_id() == at_lane ? entry : line_e;
+}
 
 __device__ __forceinline__ MapProbe map_probe(const u32* T, u32 cap, u32 key) {
   const u32 lane = lane_id();
@@ -112,6 +121,9 @@ __device__ __forceinline__ MapProbe map_probe(const u32* T, u32 cap, u32 key) {
   p.slot = 0;
   p.found = 0;
   p.entry = kMapEmpty;
+  p.lane = 0;
+  p.line_slot = 0;
+  p.line_e = kMapEmpty;
   for (u32 base = 0; base < cap; base += kMapLine) {
     u32 d = base + lane;
     const u32 s = wrap_slot(h + d, cap);
@@ -132,6 +144,9 @@ __device__ __forceinline__ MapProbe map_probe(const u32* T, u32 cap, u32 key) {
       p.slot = sl >= cap ? sl - cap : sl;
       p.found = (u32)((bm >> fl) & 1ull);
       p.entry = readlane_u32(e, fl);
+      p.lane = fl;
+      p.line_slot = s;
+      p.line_e = e;
       return p;
     }
   }
@@ -148,6 +163,9 @@ __device__ __forceinline__ MapProbe map_find(const u32* T, u32 cap, u32 key) {
   p.slot = 0;
   p.found = 0;
   p.entry = kMapEmpty;
+  p.lane = 0;
+  p.line_slot = 0;
+  p.line_e = kMapEmpty;
   // the first line holds the key unless it was displaced past it (the common case is peeled)
   u32 base = 0;
   u32 e = T[wrap_slot(h + lane, cap)];
@@ -162,6 +180,9 @@ __device__ __forceinline__ MapProbe map_find(const u32* T, u32 cap, u32 key) {
   p.slot = wrap_slot(h + base + fl, cap);
   p.found = 1;
   p.entry = readlane_u32(e, fl);
+  p.lane = fl;
+  p.line_slot = wrap_slot(h + base + lane, cap);
+  p.line_e = e;
   return p;
 }
 
@@ -198,7 +219,7 @@ __device__ __forceinline__ u32 map_dec(u32* T, u32 cap, u32 key) {
   MapProbe p = map_find(T, cap, key);
   if (!p.found) return 0;  // not reached: every decremented topic is in the map
   if (map_count(p.entry) > 1) {
-    if (lane == 0) T[p.slot] = p.entry - 1;
+    map_store_line(T, p.line_slot, p.line_e, p.lane, p.entry - 1);
     return 0;
   }
   u32 start = p.slot + 1;
