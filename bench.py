@@ -41,10 +41,12 @@ def main():
     ap.add_argument("--docs", type=int, default=0, help="override D (smaller runs for profiling)")
     ap.add_argument("--cpu-docs", type=int, default=20000, help="documents in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--schedule", default="auto", choices=["auto", "buckets", "persistent", "block"],
+    ap.add_argument("--schedule", default="auto", choices=["auto", "buckets", "persistent", "block", "split"],
                     help="sweep schedule; auto = the autotuner's choice after warmup")
     ap.add_argument("--waves-per-cu", type=int, default=32,
                     help="wavefronts per CU of the persistent schedules when --schedule is not auto")
+    ap.add_argument("--split-len", type=int, default=256,
+                    help="--schedule split: documents longer than this get the large-LDS launch")
     ap.add_argument("--no-serve", action="store_true", help="skip the fold-in serving measurement")
     ap.add_argument("--serve-requests", type=int, default=4000)
     ap.add_argument("--serve-serial", type=int, default=100, help="requests in the serial baseline sample")
@@ -86,8 +88,8 @@ def main():
         tune = {"candidates": rows, "chosen": rows[chosen]}
         log(f"autotune: chose {rows[chosen]} from {rows}")
     else:
-        kind = {"buckets": 0, "persistent": 1, "block": 2}[args.schedule]
-        rc.schedule(kind, args.waves_per_cu, 4096 if kind == 2 else 0)
+        kind = {"buckets": 0, "persistent": 1, "block": 2, "split": 3}[args.schedule]
+        rc.schedule(kind, args.waves_per_cu, {2: 4096, 3: args.split_len}.get(kind, 0))
     n_before = rc.counts() if not args.no_cpu else None
     z_before = rc.z() if not args.no_cpu else None
     rc.reset_stats()

@@ -393,7 +393,7 @@ struct BufferState {
   Segment* seg = nullptr;  // segment of the last call (uncached mode owns it)
 };
 
-enum Schedule : int { kSchedBuckets = 0, kSchedPersistent = 1, kSchedPersistentBlock = 2 };
+enum Schedule : int { kSchedBuckets = 0, kSchedPersistent = 1, kSchedPersistentBlock = 2, kSchedPersistentSplit = 3 };
 
 struct State {
   bool ready = false;
@@ -485,7 +485,11 @@ u64 sweep_lds_bytes(const Segment* sg) { return ((u64)map_capacity(sg->max_len, 
 //   persistent           one launch of cu_count * waves_per_cu wavefronts taking documents from a
 //                        longest-first queue (k_sweep_persistent);
 //   persistent + block   documents of at least block_min_len tokens go to a block-per-document
-//                        persistent launch (k_sweep_block) on a second stream, the rest as above.
+//                        persistent launch (k_sweep_block) on a second stream, the rest as above;
+//   persistent split     documents longer than block_min_len go to a persistent launch sized for
+//                        the longest map on a second stream; the others to a persistent launch
+//                        whose LDS is sized for their own longest map, so more of its wavefronts
+//                        fit on a CU.
 // All schedules produce bit-identical z: a document's randomness depends on its global id only.
 int launch_sweep_schedule(const SweepParams& P0, Segment* sg, hipStream_t st) {
   if (S.schedule == kSchedBuckets || sg->n_docs == 0) return launch_sweep(P0, sg, st);
@@ -493,6 +497,31 @@ int launch_sweep_schedule(const SweepParams& P0, Segment* sg, hipStream_t st) {
   P.doc_order = sg->doc_lpt;
   const u64 lds = sweep_lds_bytes(sg);
   if (lds > (u64)max_lds_per_block()) return fail("doc-topic map exceeds LDS");
+  if (S.schedule == kSchedPersistentSplit) {
+    u32 n_long = 0;
+    while (n_long < sg->n_docs && sg->lens_desc[n_long] > S.block_min_len) ++n_long;
+    CK(fill_u32(S.heads, 2, 0, st));
+    if (n_long > 0) {
+      HCK(hipEventRecord(S.fork, st));
+      HCK(hipStreamWaitEvent(S.stream2, S.fork, 0));
+      const u32 per_cu = std::max<u32>(1, std::min<u32>(S.waves_per_cu, (u32)(lds_per_cu() / lds)));
+      const u32 waves = std::min<u32>((u32)S.cu_count * per_cu, n_long);
+      k_sweep_persistent<8><<<waves, 64, lds, S.stream2>>>(P, S.heads + 1, n_long);
+      HCK(hipGetLastError());
+      HCK(hipEventRecord(S.join, S.stream2));
+    }
+    if (n_long < sg->n_docs) {
+      SweepParams Q = P;
+      Q.doc_order = sg->doc_lpt + n_long;
+      const u64 lds_s = ((u64)map_capacity(sg->lens_desc[n_long], S.K) + (S.K + 31) / 32) * 4;
+      const u32 per_cu = std::max<u32>(1, std::min<u32>(S.waves_per_cu, (u32)(lds_per_cu() / lds_s)));
+      const u32 waves = std::min<u32>((u32)S.cu_count * per_cu, sg->n_docs - n_long);
+      k_sweep_persistent<8><<<waves, 64, lds_s, st>>>(Q, S.heads, sg->n_docs - n_long);
+      HCK(hipGetLastError());
+    }
+    if (n_long > 0) HCK(hipStreamWaitEvent(st, S.join, 0));
+    return 0;
+  }
   u32 n_long = 0;
   if (S.schedule == kSchedPersistentBlock && S.block_min_len > 0)
     while (n_long < sg->n_docs && sg->lens_desc[n_long] >= S.block_min_len) ++n_long;
@@ -1118,7 +1147,7 @@ extern "C" int gpulda_corpus_iterate(uint32_t n_iters, float* stage_ms) {
 
 extern "C" int gpulda_corpus_schedule(int schedule, uint32_t waves_per_cu, uint32_t block_min_len) {
   CK(require_ready());
-  if (schedule < kSchedBuckets || schedule > kSchedPersistentBlock) return fail("unknown schedule %d", schedule);
+  if (schedule < kSchedBuckets || schedule > kSchedPersistentSplit) return fail("unknown schedule %d", schedule);
   S.schedule = schedule;
   S.waves_per_cu = waves_per_cu ? waves_per_cu : 16;
   S.block_min_len = block_min_len;
@@ -1151,6 +1180,8 @@ extern "C" int gpulda_corpus_autotune(float* table, uint32_t max_candidates, uin
   const u32 longest = sg->lens_desc.empty() ? 0 : sg->lens_desc[0];
   for (u32 m : {2048u, 8192u})
     if (longest >= m) cands.push_back({kSchedPersistentBlock, 16, m});
+  for (u32 m : {256u, 512u})
+    if (longest > m) cands.push_back({kSchedPersistentSplit, 40, m});
   u16* saved = nullptr;
   CK(dalloc(&saved, sg->n_tokens));
   unsigned long long vis0 = 0;

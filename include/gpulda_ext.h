@@ -82,7 +82,10 @@ int gpulda_corpus_iterate(uint32_t n_iters, float* stage_ms);
  *   0  one launch per length bucket (wavefront per document; default);
  *   1  one persistent launch of cu_count * waves_per_cu wavefronts over a longest-first queue;
  *   2  as 1, plus a block-per-document persistent launch (4 wavefronts share a document) for
- *      documents of at least block_min_len tokens, on a second stream. */
+ *      documents of at least block_min_len tokens, on a second stream;
+ *   3  two persistent launches: documents longer than block_min_len (LDS for the longest map)
+ *      on a second stream, the rest with LDS sized for their own longest map (more wavefronts
+ *      per CU). */
 int gpulda_corpus_schedule(int schedule, uint32_t waves_per_cu, uint32_t block_min_len);
 /* 1 = capture one iteration into a hipGraph and replay it (the iteration number lives in device
  * memory); stage_ms of gpulda_corpus_iterate then reports only the total, in stage_ms[2]. */

@@ -333,16 +333,18 @@ def _schedules_corpus():
 
 
 def test_schedules_and_graph_bit_identical():
-    """Per-bucket launches, the persistent queue, persistent + block-per-document and a replayed
-    hipGraph give the same chain (same Philox streams) on a power-law corpus."""
+    """Per-bucket launches, the persistent queue, persistent + block-per-document, the split
+    persistent launches and a replayed hipGraph give the same chain (same Philox streams) on a
+    power-law corpus."""
     from gpupolyaurnlda_amd.resident import ResidentCorpus
     spec, c = _schedules_corpus()
     say(f"schedules corpus N={len(c['w'])} max L={int(c['d_len'].max())}")
     results = {}
     for name, sched, graph in [("buckets", 0, False), ("persistent", 1, False), ("block", 2, False),
-                               ("graph", 0, True), ("graph-block", 2, True)]:
+                               ("split", 3, False), ("graph", 0, True), ("graph-block", 2, True),
+                               ("graph-split", 3, True)]:
         with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=4, z=c["z"]) as rc:
-            rc.schedule(sched, 8, 1000 if sched == 2 else 0)
+            rc.schedule(sched, 8, {2: 1000, 3: 64}.get(sched, 0))
             rc.use_graph(graph)
             zs = []
             for _ in range(3):  # replays with eager reads in between
