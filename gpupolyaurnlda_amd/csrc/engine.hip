@@ -21,6 +21,8 @@
 #include <ctime>
 #include <string>
 #include <vector>
+#include <thread>
+#include <atomic>
 
 #include "../../include/gpulda_abi.h"
 #include "../../include/gpulda_ext.h"
@@ -386,12 +388,62 @@ struct BufferState {
   hipStream_t copy_out = nullptr;  // device -> host copies of z
   hipEvent_t ev_in[kMaxParts] = {};
   hipEvent_t ev_swept[kMaxParts] = {};
+  hipEvent_t ev_out[kMaxParts] = {};
   hipEvent_t ev_done = nullptr;
-  u32* staging = nullptr;  // u32 z for the PCIe copies (buffer_size)
-  u32* bad = nullptr;      // device count of indicators >= K
+  u16* host_z = nullptr;   // pinned u16 z of the slice in flight (buffer_size)
+  u32 host_bad = 0;        // indicators >= K seen while narrowing (reported by sync_buffer)
   u32* heads = nullptr;    // queue heads of the parts' persistent sweeps (kMaxParts)
   Segment* seg = nullptr;  // segment of the last call (uncached mode owns it)
+  // work left for sync_buffer: widen the parts' topics into the caller's z as they arrive
+  Segment* pending = nullptr;
+  u32* pending_z = nullptr;
 };
+
+// Threads for host-side conversions of z (the caller's thread takes the first share).
+u32 host_threads() {
+  const u32 hc = std::thread::hardware_concurrency();
+  return std::max<u32>(1, std::min<u32>(16, hc ? hc : 1));
+}
+
+// f(begin, end) over [0, n), split across host_threads() threads.
+template <class F>
+void host_parallel(u64 n, F f) {
+  const u32 T = host_threads();
+  if (T == 1 || n < (1ull << 16)) {
+    f((u64)0, n);
+    return;
+  }
+  const u64 chunk = (n + T - 1) / T;
+  std::vector<std::thread> th;
+  for (u32 t = 1; t < T; ++t) {
+    const u64 a = (u64)t * chunk, b = std::min<u64>(n, a + chunk);
+    if (a < b) th.emplace_back(f, a, b);
+  }
+  f((u64)0, std::min<u64>(n, chunk));
+  for (std::thread& x : th) x.join();
+}
+
+// u32 topics -> u16 (an indicator >= K becomes topic 0 and is counted).
+u32 narrow_topics(const u32* in, u16* out, u64 n, u32 K) {
+  std::atomic<u32> bad{0};
+  host_parallel(n, [&](u64 a, u64 b) {
+    u32 nb = 0;
+    for (u64 i = a; i < b; ++i) {
+      const u32 v = in[i];
+      const bool ok = v < K;
+      nb += !ok;
+      out[i] = (u16)(ok ? v : 0u);
+    }
+    bad += nb;
+  });
+  return bad.load();
+}
+
+void widen_topics(const u16* in, u32* out, u64 n) {
+  host_parallel(n, [&](u64 a, u64 b) {
+    for (u64 i = a; i < b; ++i) out[i] = in[i];
+  });
+}
 
 enum Schedule : int { kSchedBuckets = 0, kSchedPersistent = 1, kSchedPersistentBlock = 2, kSchedPersistentSplit = 3 };
 
@@ -709,12 +761,12 @@ int do_initialize(Args* args, Buffer* buffers, u32 n_buffers) {
     for (u32 p = 0; p < kMaxParts; ++p) {
       HCK(hipEventCreateWithFlags(&bs->ev_in[p], hipEventDisableTiming));
       HCK(hipEventCreateWithFlags(&bs->ev_swept[p], hipEventDisableTiming));
+      HCK(hipEventCreateWithFlags(&bs->ev_out[p], hipEventDisableTiming));
     }
     HCK(hipEventCreateWithFlags(&bs->ev_done, hipEventDisableTiming));
-    CK(dalloc(&bs->staging, std::max<u32>(1, args->buffer_size)));
-    CK(dalloc(&bs->bad, 1));
+    HCK(hipHostMalloc((void**)&bs->host_z, std::max<u64>(1, args->buffer_size) * sizeof(u16), hipHostMallocDefault));
     CK(dalloc(&bs->heads, kMaxParts));
-    B->gpu_z = bs->staging;
+    B->gpu_z = nullptr;  // z is staged through pinned host memory (u16); the device copy is per slice
     B->gpu_w = nullptr;
     B->gpu_d_len = nullptr;
     B->gpu_d_idx = nullptr;
@@ -745,47 +797,39 @@ int do_sample_z_async(Buffer* B) {
   Segment* sg = nullptr;
   CK(get_segment(bs, B, &sg));
   hipStream_t st = bs->stream;
-  CK(fill_u32(bs->bad, 1, 0, st));
   CK(fill_u32(bs->heads, kMaxParts, 0, st));
-  // Per part p: copy its z in (copy_in), narrow and sweep it (st), widen it and copy it out
-  // (copy_out).  The host enqueues the copy-out of part p - 1 after the sweep of part p, so that
-  // the copies overlap the sweeps even where a copy from pageable memory holds the host thread.
+  // Per part p: the host narrows its z into pinned u16 memory and copies it in (copy_in), the
+  // part is swept (st), and its topics are copied back to the pinned memory (copy_out);
+  // sync_buffer widens each part into the caller's z once its copy has landed.  The host narrows
+  // part p + 1 while the device copies and sweeps part p.
   const SweepParams P = sweep_params(sg, S.doc_counter);
   const u32 np = (u32)sg->parts.size();
   if (np > kMaxParts) return fail("slice split into %u parts (limit %u)", np, kMaxParts);
-  HCK(hipEventRecord(bs->ev_done, st));  // this call's fill; the previous call ended at sync_buffer
-  HCK(hipStreamWaitEvent(bs->copy_in, bs->ev_done, 0));
-  auto copy_out = [&](u32 p) -> int {
-    const Segment::Part& pt = sg->parts[p];
-    const u64 n = pt.tok1 - pt.tok0;
-    HCK(hipStreamWaitEvent(bs->copy_out, bs->ev_swept[p], 0));
-    if (n) {
-      k_z_widen<<<grid_for(n, 256), 256, 0, bs->copy_out>>>(sg->z + pt.tok0, bs->staging + pt.tok0, n);
-      HCK(hipGetLastError());
-      HCK(hipMemcpyAsync(B->z + pt.tok0, bs->staging + pt.tok0, n * sizeof(u32), hipMemcpyDeviceToHost,
-                         bs->copy_out));
-    }
-    return 0;
-  };
+  if (bs->pending) return fail("sample_z_async: the previous call on this buffer was not synchronised");
+  bs->host_bad = 0;
   for (u32 p = 0; p < np; ++p) {
     const Segment::Part& pt = sg->parts[p];
     const u64 n = pt.tok1 - pt.tok0;
-    if (n) HCK(hipMemcpyAsync(bs->staging + pt.tok0, B->z + pt.tok0, n * sizeof(u32), hipMemcpyHostToDevice,
-                              bs->copy_in));
+    if (n) {
+      bs->host_bad += narrow_topics(B->z + pt.tok0, bs->host_z + pt.tok0, n, S.K);
+      HCK(hipMemcpyAsync(sg->z + pt.tok0, bs->host_z + pt.tok0, n * sizeof(u16), hipMemcpyHostToDevice,
+                         bs->copy_in));
+    }
     HCK(hipEventRecord(bs->ev_in[p], bs->copy_in));
     HCK(hipStreamWaitEvent(st, bs->ev_in[p], 0));
-    if (n) {
-      k_z_narrow<<<grid_for(n, 256), 256, 0, st>>>(bs->staging + pt.tok0, sg->z + pt.tok0, n, S.K, bs->bad);
-      HCK(hipGetLastError());
-    }
     CK(launch_part_sweep(P, sg, p, bs->heads + p, st));
     HCK(hipEventRecord(bs->ev_swept[p], st));
-    if (p > 0) CK(copy_out(p - 1));
+    HCK(hipStreamWaitEvent(bs->copy_out, bs->ev_swept[p], 0));
+    if (n)
+      HCK(hipMemcpyAsync(bs->host_z + pt.tok0, sg->z + pt.tok0, n * sizeof(u16), hipMemcpyDeviceToHost,
+                         bs->copy_out));
+    HCK(hipEventRecord(bs->ev_out[p], bs->copy_out));
   }
-  if (np > 0) CK(copy_out(np - 1));
   CK(launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, S.n, S.K, st));
   HCK(hipEventRecord(bs->ev_done, bs->copy_out));
-  HCK(hipStreamWaitEvent(st, bs->ev_done, 0));  // sync_buffer waits on st alone
+  HCK(hipStreamWaitEvent(st, bs->ev_done, 0));  // the buffer's stream covers everything
+  bs->pending = sg;
+  bs->pending_z = B->z;
   S.doc_counter += B->n_docs;
   B->gpu_w = sg->w;
   B->gpu_d_len = sg->doc_off;
@@ -796,10 +840,18 @@ int do_sync_buffer(Buffer* B) {
   CK(require_ready());
   if (!B || !B->gpu_rng) return fail("sync_buffer: buffer was not passed to initialize");
   BufferState* bs = (BufferState*)B->gpu_rng;
+  if (Segment* sg = bs->pending) {
+    // widen each part as soon as its copy has landed, while later parts are still on the device
+    for (u32 p = 0; p < (u32)sg->parts.size(); ++p) {
+      const Segment::Part& pt = sg->parts[p];
+      HCK(hipEventSynchronize(bs->ev_out[p]));
+      widen_topics(bs->host_z + pt.tok0, bs->pending_z + pt.tok0, pt.tok1 - pt.tok0);
+    }
+    bs->pending = nullptr;
+    bs->pending_z = nullptr;
+  }
   HCK(hipStreamSynchronize(bs->stream));
-  u32 bad = 0;
-  HCK(hipMemcpy(&bad, bs->bad, sizeof(u32), hipMemcpyDeviceToHost));
-  if (bad) return fail("%u topic indicators were >= K = %u (treated as topic 0)", bad, S.K);
+  if (bs->host_bad) return fail("%u topic indicators were >= K = %u (treated as topic 0)", bs->host_bad, S.K);
   return 0;
 }
 
@@ -812,12 +864,12 @@ int do_cleanup(Buffer* buffers, u32 n_buffers) {
     BufferState* bs = (BufferState*)B->gpu_rng;
     if (!bs) continue;
     free_segment(bs->seg);
-    dfree(bs->staging);
-    dfree(bs->bad);
+    if (bs->host_z) (void)hipHostFree(bs->host_z);
     dfree(bs->heads);
     for (u32 p = 0; p < kMaxParts; ++p) {
       if (bs->ev_in[p]) (void)hipEventDestroy(bs->ev_in[p]);
       if (bs->ev_swept[p]) (void)hipEventDestroy(bs->ev_swept[p]);
+      if (bs->ev_out[p]) (void)hipEventDestroy(bs->ev_out[p]);
     }
     if (bs->ev_done) (void)hipEventDestroy(bs->ev_done);
     if (bs->copy_in) (void)hipStreamDestroy(bs->copy_in);

@@ -13,6 +13,8 @@ import os
 import numpy as np
 import pytest
 
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -269,6 +271,29 @@ def test_abi_pipelined_parts(orc, buffer_size, max_D):
         model.n = orc.count(model.w, model.z, TINY.V, TINY.K)
     assert np.array_equal(z, model.z)
     assert np.array_equal(n_gpu, model.n)
+
+
+def test_abi_reports_topic_out_of_range():
+    """A topic indicator >= K in the caller's z ends the run at sync_buffer with a message, as
+    every ABI error does (src/cu/error.cuh:13-18); the sweep itself runs normally."""
+    import subprocess
+    import sys
+    code = (
+        "import sys, numpy as np\n"
+        f"sys.path.insert(0, {ROOT!r})\n"
+        "from gpupolyaurnlda_amd.train import Sampler, train\n"
+        "rng = np.random.default_rng(0)\n"
+        "V, K, D = 50, 8, 20\n"
+        "d = np.full(D, 10, np.uint32); w = rng.integers(0, V, 200).astype(np.uint32)\n"
+        "z = rng.integers(0, K, 200).astype(np.uint32); z[37] = K\n"
+        "C = np.bincount(w, minlength=V).astype(np.uint32)\n"
+        "with Sampler(0.1, 0.01, K, C, 1000, 100) as s:\n"
+        "    train(s, z, w, d, np.zeros(D, np.uint32), 1, 1000, 100)\n"
+        "print('not reached')\n")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert out.returncode != 0
+    assert "topic indicators were >= K" in out.stdout + out.stderr
+    assert "not reached" not in out.stdout
 
 
 def test_resident_matches_oracle_and_loglik(orc):
