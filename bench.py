@@ -217,7 +217,7 @@ def abi_rate(c, spec, iters):
         train(smp, z, w, d, k_d, iters, N, D)
         dt = (time.perf_counter() - t0) / iters
     out = {"tokens_per_s": round(N / dt, 1), "ms_per_iteration": round(dt * 1e3, 3), "iterations": iters,
-           "buffer": "whole corpus in one host buffer (train.rs loop, z as u32 both ways per iteration)"}
+           "buffer": "whole corpus in one pageable host buffer (train.rs loop; z narrowed to u16 in pinned memory, both ways per iteration)"}
     log(f"ABI path: {out}")
     return out
 

@@ -20,7 +20,8 @@
  *   - Single host thread, not re-entrant.  The library keeps the Args* passed to initialize()
  *     and reads it until cleanup(); the caller keeps Args and C alive until then.
  *   - initialize() allocates device state into the caller's Buffer structs (the gpu_* fields and
- *     stream are library-owned opaque handles); cleanup() frees them.
+ *     stream are library-owned opaque handles; gpu_z stays NULL because z is staged through the
+ *     library's pinned host memory as u16); cleanup() frees them.
  *   - sample_phi() is synchronous.  sample_z_async() returns before the sweep finishes; the caller
  *     must not touch buffer->z until sync_buffer(buffer) returns.
  *   - Host arrays w, d and K_d are read-only to the library (train.rs maps them read-only); z is
