@@ -455,14 +455,26 @@ __device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const f
                                                 u32 rx, u32 rw, u32 dense) {
   const u32 lane = lane_id();
   f32 G0 = 0.0f, G1 = 0.0f, G2 = 0.0f, G3 = 0.0f;
-  // eight lines per step, so eight gathers are in flight at once
-  for (u32 j = 0; j < lines; j += 8) {
-    u32 e[8];
-#pragma unroll
-    for (u32 k = 0; k < 8; ++k) e[k] = j + k < lines ? T[(j + k) * 64 + lane] : kMapEmpty;
+  // eight lines per step, so eight gathers are in flight at once; full steps read without a
+  // bounds test, the last partial step reads an empty slot past the map's end
+  u32 j = 0;
+  for (; j + 8 <= lines; j += 8) {
     f32 m[8];
 #pragma unroll
-    for (u32 k = 0; k < 8; ++k) m[k] = slot_mass(e[k], prow);
+    for (u32 k = 0; k < 8; ++k) m[k] = slot_mass(T[(j + k) * 64 + lane], prow);
+    G0 = G0 + m[0];
+    G1 = G1 + m[1];
+    G2 = G2 + m[2];
+    G3 = G3 + m[3];
+    G0 = G0 + m[4];
+    G1 = G1 + m[5];
+    G2 = G2 + m[6];
+    G3 = G3 + m[7];
+  }
+  if (j < lines) {
+    f32 m[8];
+#pragma unroll
+    for (u32 k = 0; k < 8; ++k) m[k] = slot_mass(j + k < lines ? T[(j + k) * 64 + lane] : kMapEmpty, prow);
     G0 = G0 + m[0];
     G1 = G1 + m[1];
     G2 = G2 + m[2];
