@@ -288,15 +288,49 @@ __device__ __forceinline__ u32 distinct_topics(u32* bits, u32 K, const u16* __re
 }
 
 // Builds the doc-topic map of one document from its current topics (count_topics,
-// src/cu/topics.cuh:142-158).
-__device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u16* __restrict__ z, u64 b, u32 L) {
+// src/cu/topics.cuh:142-158).  Every distinct topic (set in the bitmap `bits`, `distinct` of them)
+// is inserted once, with count 1, so the layout -- a function of the key set and the capacity --
+// is the one token-by-token inserts give.  Then each token adds one to its topic's slot (a
+// per-lane probe and an LDS atomic), and the extra 1 of the inserts is taken back.
+__device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u32* bits, u32 K, u32 distinct,
+                                              const u16* __restrict__ z, u64 b, u32 L) {
   const u32 lane = lane_id();
   for (u32 s = lane; s < cap; s += 64) T[s] = kMapEmpty;
-  for (u32 c0 = 0; c0 < L; c0 += 64) {
-    u32 zz = c0 + lane < L ? (u32)z[b + c0 + lane] : 0;
-    u32 nt = L - c0 < 64 ? L - c0 : 64;
-    for (u32 t = 0; t < nt; ++t) map_inc(T, cap, readlane_u32(zz, t));
+  const u32 words = (K + 31) / 32;
+  for (u32 wi = 0; wi < words; ++wi) {
+    wave_sync();
+    u32 bw = uniform_u32(bits[wi]);
+    while (bw) {
+      const u32 bit = (u32)__builtin_ctz(bw);
+      bw &= bw - 1u;
+      map_inc(T, cap, wi * 32u + bit);
+    }
   }
+  wave_sync();
+  if (distinct == 1) {
+    // one topic holds all L tokens (1 + L would not fit the 16-bit count at L = 65535)
+    for (u32 s = lane; s < cap; s += 64)
+      if (T[s] != kMapEmpty) T[s] = map_entry(map_key(T[s]), L);
+    return;
+  }
+  // with two or more topics no topic has more than L - 1 <= 65534 tokens, so 1 + count fits
+  for (u32 c0 = 0; c0 < L; c0 += 64) {
+    if (c0 + lane < L) {
+      const u32 key = z[b + c0 + lane];
+      u32 s = map_home(key, cap);
+      for (u32 n = 0; n < cap; ++n) {  // the key is present, so the probe reaches it first
+        const u32 e = T[s];
+        if (e != kMapEmpty && map_key(e) == key) {
+          atomicAdd(&T[s], 1u);
+          break;
+        }
+        s = s + 1u == cap ? 0u : s + 1u;
+      }
+    }
+  }
+  wave_sync();
+  for (u32 s = lane; s < cap; s += 64)
+    if (T[s] != kMapEmpty) T[s] -= 1u;
 }
 
 // Regrows the map to capacity ncap in the middle of a sweep: clears it and re-inserts the topic
@@ -341,7 +375,7 @@ __device__ __forceinline__ DocState open_span(const SweepParams& P, u64 b, u32 L
   ds.gdoc = uniform_u32(gdoc);
   ds.occupied = uniform_u32(distinct_topics(T + ds.cap_max, P.K, P.z, ds.b, ds.L));
   ds.cap = map_capacity(ds.occupied, P.K);
-  build_doc_map(T, ds.cap, P.z, ds.b, ds.L);
+  build_doc_map(T, ds.cap, T + ds.cap_max, P.K, ds.occupied, P.z, ds.b, ds.L);
   return ds;
 }
 
@@ -596,7 +630,8 @@ __device__ __forceinline__ void shrink_doc_map(const SweepParams& P, u32* T, Doc
   if (want < ds.cap) {
     ds.cap = want;
     wave_sync();
-    build_doc_map(T, ds.cap, P.z, ds.b, ds.L);
+    const u32 distinct = uniform_u32(distinct_topics(T + ds.cap_max, P.K, P.z, ds.b, ds.L));
+    build_doc_map(T, ds.cap, T + ds.cap_max, P.K, distinct, P.z, ds.b, ds.L);
   }
 }
 
