@@ -296,14 +296,21 @@ __device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u32* bits, 
                                               const u16* __restrict__ z, u64 b, u32 L) {
   const u32 lane = lane_id();
   for (u32 s = lane; s < cap; s += 64) T[s] = kMapEmpty;
+  // 64 bitmap words per step; only words with a topic set are visited
   const u32 words = (K + 31) / 32;
-  for (u32 wi = 0; wi < words; ++wi) {
+  for (u32 w0 = 0; w0 < words; w0 += 64) {
     wave_sync();
-    u32 bw = uniform_u32(bits[wi]);
-    while (bw) {
-      const u32 bit = (u32)__builtin_ctz(bw);
-      bw &= bw - 1u;
-      map_inc(T, cap, wi * 32u + bit);
+    const u32 mine = w0 + lane < words ? bits[w0 + lane] : 0u;
+    u64 nz = ballot(mine != 0u);
+    while (nz) {
+      const u32 l = first_lane(nz);
+      nz &= nz - 1ull;
+      u32 bw = readlane_u32(mine, l);
+      while (bw) {
+        const u32 bit = (u32)__builtin_ctz(bw);
+        bw &= bw - 1u;
+        map_inc(T, cap, (w0 + l) * 32u + bit);
+      }
     }
   }
   wave_sync();
