@@ -103,7 +103,8 @@ __global__ void k_init_counts(const u32* __restrict__ C, u32* __restrict__ n, u3
 //      weight a_k = m q_k, so that total deficit equals total surplus exactly.  a < Q is light
 //      (deficit Q - a), otherwise heavy (surplus a - Q).  Exclusive deficit prefixes D over the
 //      lights and inclusive surplus prefixes E over the heavies, in index order, are exact u64
-//      sums; lights and heavies are compacted to LDS in index order.
+//      sums; they share one LDS array of m entries (the nh heavies' E from the front, the nl
+//      lights' D from the back, nh + nl = m), beside the heavies' indices.
 //   C. light i: alias = first heavy with E > D_i, prob = a_i / Q.
 //      heavy j: its turn ends at D_end = D of the first light with D >= E_j, or the total deficit
 //      when there is none; if D_end > E_j it keeps prob = (Q + E_j - D_end) / Q and its
@@ -125,9 +126,9 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
   const u32 row = blockIdx.x * rows_per_block_waves + wave;
   if (row >= n_rows) return;
   const u32 m_even = (m + 1) & ~1u;
-  u32* hidx = reinterpret_cast<u32*>(alias_smem) + (u64)wave * m_even * 5;
-  u64* hE = reinterpret_cast<u64*>(hidx + m_even);
-  u64* lD = hE + m_even;
+  u32* hidx = reinterpret_cast<u32*>(alias_smem) + (u64)wave * m_even * 3;
+  u64* hE = reinterpret_cast<u64*>(hidx + m_even);  // heavy j's E at hE[j]
+  u64* lDr = hE + (m - 1);                           // light b's D at lDr[-b]
 
   f32* x = rows + (u64)row * m;
   AliasSlot* o = out + (u64)row * m;
@@ -188,7 +189,7 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
     }
     if (light) {
       u32 pos = nl + (u32)__popcll(lb & lt);
-      lD[pos] = sD + incD - def;
+      *(lDr - pos) = sD + incD - def;
     }
     sE += readlane_u64(incS, 63);
     sD += readlane_u64(incD, 63);
@@ -212,7 +213,7 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
     s.alias = k;
     if (light) {
       u32 pos = lc + (u32)__popcll(lb & lt);
-      u64 D = lD[pos];
+      u64 D = *(lDr - pos);
       u32 lo = 0, hi = nh;  // first j with hE[j] > D
       while (lo < hi) {
         u32 mid = (lo + hi) >> 1;
@@ -226,13 +227,13 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
     if (heavy) {
       u32 pos = hc + (u32)__popcll(hb & lt);
       u64 E = hE[pos];
-      u32 lo = 0, hi = nl;  // first b with lD[b] >= E
+      u32 lo = 0, hi = nl;  // first light b with D_b >= E
       while (lo < hi) {
         u32 mid = (lo + hi) >> 1;
-        if (lD[mid] >= E) hi = mid;
+        if (*(lDr - mid) >= E) hi = mid;
         else lo = mid + 1;
       }
-      const u64 dend = lo < nl ? lD[lo] : sD;
+      const u64 dend = lo < nl ? *(lDr - lo) : sD;
       if (dend > E && pos + 1 < nh) {
         s.prob = (f32)((f64)(Q + E - dend) / (f64)Q);
         s.alias = hidx[pos + 1];
