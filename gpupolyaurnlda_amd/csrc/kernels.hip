@@ -211,28 +211,28 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
     AliasSlot s;
     s.prob = 1.0f;
     s.alias = k;
+    // one binary search per lane, lights and heavies together: a light looks for the first heavy
+    // with E > D (E >= D + 1, exact in u64), a heavy for the first light with D >= E; both are
+    // "first index whose value is >= key" over an ascending array (heavies' E forward from hE,
+    // lights' D backward from lDr)
+    const u32 pos = light ? lc + (u32)__popcll(lb & lt) : hc + (u32)__popcll(hb & lt);
+    const u64 own = light ? *(lDr - pos) : heavy ? hE[pos] : 0;  // D of a light, E of a heavy
+    const u64 key = light ? own + 1 : own;
+    const u64* base = light ? hE : lDr;
+    const long step = light ? 1 : -1;
+    u32 lo = 0, hi = light ? nh : heavy ? nl : 0;
+    while (lo < hi) {
+      const u32 mid = (lo + hi) >> 1;
+      if (base[step * (long)mid] >= key) hi = mid;
+      else lo = mid + 1;
+    }
     if (light) {
-      u32 pos = lc + (u32)__popcll(lb & lt);
-      u64 D = *(lDr - pos);
-      u32 lo = 0, hi = nh;  // first j with hE[j] > D
-      while (lo < hi) {
-        u32 mid = (lo + hi) >> 1;
-        if (hE[mid] > D) hi = mid;
-        else lo = mid + 1;
-      }
       if (lo == nh) lo = nh - 1;
       s.prob = (f32)((f64)a / (f64)Q);
       s.alias = hidx[lo];
     }
     if (heavy) {
-      u32 pos = hc + (u32)__popcll(hb & lt);
-      u64 E = hE[pos];
-      u32 lo = 0, hi = nl;  // first light b with D_b >= E
-      while (lo < hi) {
-        u32 mid = (lo + hi) >> 1;
-        if (*(lDr - mid) >= E) hi = mid;
-        else lo = mid + 1;
-      }
+      const u64 E = own;
       const u64 dend = lo < nl ? *(lDr - lo) : sD;
       if (dend > E && pos + 1 < nh) {
         s.prob = (f32)((f64)(Q + E - dend) / (f64)Q);
