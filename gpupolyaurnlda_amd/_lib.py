@@ -111,7 +111,7 @@ _SIGS = {
     "gpulda_calibrate_copy": (C.c_int, [C.c_uint64, C.c_uint32, f32p]),
     "gpulda_stage_philox": (C.c_int, [u32p, u32p, u32p, C.c_uint32]),
     "gpulda_stage_scan64": (C.c_int, [f32p, f32p, C.c_uint32]),
-    "gpulda_stage_map": (C.c_int, [u32p, C.c_uint32, C.c_uint32, u32p]),
+    "gpulda_stage_map": (C.c_int, [u32p, C.c_uint32, C.c_uint32, C.c_uint32, u32p]),
     "gpulda_stage_alias": (C.c_int, [f32p, C.c_uint32, C.c_uint32, u64p, C.c_float, f32p, f32p, f32p, u32p]),
     "gpulda_stage_alias_draw": (C.c_int, [f32p, u32p, C.c_uint32, u32p, u32p, C.c_uint32, u32p]),
     "gpulda_stage_poisson": (C.c_int, [u32p, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, u32p]),

@@ -1810,14 +1810,17 @@ extern "C" int gpulda_stage_scan64(const float* in, float* out, uint32_t n_group
   return 0;
 }
 
-extern "C" int gpulda_stage_map(const uint32_t* ops, uint32_t n_ops, uint32_t cap, uint32_t* table_out) {
+extern "C" int gpulda_stage_map(const uint32_t* ops, uint32_t n_ops, uint32_t cap, uint32_t K, uint32_t* table_out) {
   if (cap == 0 || cap % kMapLine) return fail("map capacity must be a positive multiple of 64");
+  if (K == 0 || K > kMaxTopics) return fail("K = %u out of range", K);
+  for (u32 i = 0; i < n_ops; ++i)
+    if ((ops[i] & 0x7FFFFFFFu) >= K) return fail("op %u names topic %u >= K", i, ops[i] & 0x7FFFFFFFu);
   if ((u64)cap * 4 > (u64)max_lds_per_block()) return fail("map capacity exceeds LDS");
   Scratch sc;
   u32 *dops, *dout;
   CK(sc.upload(&dops, ops, n_ops));
   CK(sc.alloc(&dout, cap));
-  k_stage_map<<<1, 64, (u64)cap * 4>>>(dops, n_ops, cap, dout);
+  k_stage_map<<<1, 64, (u64)cap * 4>>>(dops, n_ops, cap, K, dout);
   HCK(hipGetLastError());
   HCK(hipMemcpy(table_out, dout, (u64)cap * 4, hipMemcpyDeviceToHost));
   return 0;

@@ -296,6 +296,7 @@ __device__ __forceinline__ u32 distinct_topics(u32* bits, u32 K, const u16* __re
 __device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u32* bits, u32 K, u32 distinct,
                                               const u16* __restrict__ z, u64 b, u32 L) {
   const u32 lane = lane_id();
+  const u32 hm = map_home_mult(K);
   for (u32 s = lane; s < cap; s += 64) T[s] = kMapEmpty;
   // 64 bitmap words per step; only words with a topic set are visited
   const u32 words = (K + 31) / 32;
@@ -310,7 +311,7 @@ __device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u32* bits, 
       while (bw) {
         const u32 bit = (u32)__builtin_ctz(bw);
         bw &= bw - 1u;
-        map_inc(T, cap, (w0 + l) * 32u + bit);
+        map_inc(T, cap, hm, (w0 + l) * 32u + bit);
       }
     }
   }
@@ -325,7 +326,7 @@ __device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u32* bits, 
   for (u32 c0 = 0; c0 < L; c0 += 64) {
     if (c0 + lane < L) {
       const u32 key = z[b + c0 + lane];
-      u32 s = map_home(key, cap);
+      u32 s = map_home(key, cap, hm);
       for (u32 n = 0; n < cap; ++n) {  // the key is present, so the probe reaches it first
         const u32 e = T[s];
         if (e != kMapEmpty && map_key(e) == key) {
@@ -346,8 +347,8 @@ __device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u32* bits, 
 // in z, lanes < t of the chunk in znew_lane, lanes > t in zz, and tokens after the chunk their
 // old topics in z -- exactly the multiset the map held, so (the layout being a function of the
 // key set) the result equals any other way of moving the entries.
-__device__ __noinline__ void regrow_doc_map(u32* T, u32 ncap, const u16* __restrict__ z, u64 b, u32 L, u32 c0,
-                                            u32 t, u32 znew_lane, u32 zz) {
+__device__ __noinline__ void regrow_doc_map(u32* T, u32 ncap, u32 hm, const u16* __restrict__ z, u64 b, u32 L,
+                                            u32 c0, u32 t, u32 znew_lane, u32 zz) {
   const u32 lane = lane_id();
   wave_sync();
   for (u32 s = lane; s < ncap; s += 64) T[s] = kMapEmpty;
@@ -357,7 +358,7 @@ __device__ __noinline__ void regrow_doc_map(u32* T, u32 ncap, const u16* __restr
     if (c == c0) v = lane < t ? znew_lane : zz;
     else v = c + lane < L ? (u32)z[b + c + lane] : 0;
     for (u32 tt = 0; tt < nt; ++tt)
-      if (!(c == c0 && tt == t)) map_inc(T, ncap, readlane_u32(v, tt));
+      if (!(c == c0 && tt == t)) map_inc(T, ncap, hm, readlane_u32(v, tt));
   }
 }
 
@@ -370,6 +371,7 @@ struct DocState {
   u32 cap;        // current capacity
   u32 occupied;   // distinct topics in the map
   u32 gdoc;       // global document id (Philox counter word 1)
+  u32 hm;         // map_home_mult(K)
 };
 
 // Capacity rule: start sized for the n0 distinct topics; grow (rebuild) when a new topic would
@@ -381,6 +383,7 @@ __device__ __forceinline__ DocState open_span(const SweepParams& P, u64 b, u32 L
   ds.cap_max = map_capacity(ds.L, P.K);
   ds.kmax = ds.L < P.K ? ds.L : P.K;
   ds.gdoc = uniform_u32(gdoc);
+  ds.hm = map_home_mult(P.K);
   ds.occupied = uniform_u32(distinct_topics(T + ds.cap_max, P.K, P.z, ds.b, ds.L));
   ds.cap = map_capacity(ds.occupied, P.K);
   build_doc_map(T, ds.cap, T + ds.cap_max, P.K, ds.occupied, P.z, ds.b, ds.L);
@@ -573,13 +576,13 @@ __device__ __forceinline__ void add_topic(const SweepParams& P, u32* T, DocState
     if (lane_id() == 0) T[pk.slot] = pk.entry + 1;
     return;
   }
-  MapProbe pz = map_probe(T, ds.cap, pk.topic);
+  MapProbe pz = map_probe(T, ds.cap, ds.hm, pk.topic);
   if (!pz.found) {
     if (map_capacity(ds.occupied + 1, P.K) > ds.cap) {
       const u32 want = 2 * (ds.occupied + 1) < ds.kmax ? 2 * (ds.occupied + 1) : ds.kmax;
       ds.cap = map_capacity(want, P.K);
-      regrow_doc_map(T, ds.cap, P.z, ds.b, ds.L, c0, t, znew_lane, zz);
-      pz = map_probe(T, ds.cap, pk.topic);
+      regrow_doc_map(T, ds.cap, ds.hm, P.z, ds.b, ds.L, c0, t, znew_lane, zz);
+      pz = map_probe(T, ds.cap, ds.hm, pk.topic);
     }
     ++ds.occupied;
   }
@@ -611,7 +614,7 @@ __device__ __forceinline__ void sweep_pass(const SweepParams& P, u32* T, DocStat
     for (u32 t = 0; t < nt; ++t) {
       const u32 wt = readlane_u32(ww, t);
       const u32 zt = readlane_u32(zz, t);
-      ds.occupied -= uniform_u32(map_dec(T, ds.cap, zt));
+      ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, zt));
       const f32 sa = readlane_f32(sa_l, t);
       Pick pk;
       pk.topic = zt;
@@ -753,7 +756,7 @@ __global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u
         const u32 zt = readlane_u32(zz, t);
         const f32 sa = readlane_f32(sa_l, t);
         if (wave == 0) {
-          ds.occupied -= uniform_u32(map_dec(T, ds.cap, zt));
+          ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, zt));
           if (lane == 0) sh_cap = ds.cap;
         }
         __syncthreads();
@@ -1071,14 +1074,15 @@ __global__ void k_stage_scan(const f32* in, f32* out, u32 n_groups) {
 }
 
 // ops[i] = key | (1u << 31) for a decrement; the final table is written to out.
-__global__ void k_stage_map(const u32* ops, u32 n_ops, u32 cap, u32* out) {
+__global__ void k_stage_map(const u32* ops, u32 n_ops, u32 cap, u32 K, u32* out) {
   extern __shared__ u32 map_smem[];
   const u32 lane = lane_id();
+  const u32 hm = map_home_mult(K);
   for (u32 s = lane; s < cap; s += 64) map_smem[s] = kMapEmpty;
   for (u32 i = 0; i < n_ops; ++i) {
     u32 op = ops[i];
-    if (op >> 31) map_dec(map_smem, cap, op & 0x7FFFFFFFu);
-    else map_inc(map_smem, cap, op);
+    if (op >> 31) map_dec(map_smem, cap, hm, op & 0x7FFFFFFFu);
+    else map_inc(map_smem, cap, hm, op);
   }
   wave_sync();
   for (u32 s = lane; s < cap; s += 64) out[s] = map_smem[s];

@@ -125,8 +125,12 @@ LDA_HD u32 map_key(u32 e) { return e >> 16; }
 LDA_HD u32 map_count(u32 e) { return e & 0xFFFFu; }
 LDA_HD u32 map_entry(u32 key, u32 count) { return (key << 16) | count; }
 
-// Home slot: Fibonacci hashing of the topic, reduced to [0, C) by multiply-shift.
-LDA_HD u32 map_home(u32 key, u32 cap) { return (u32)(((u64)(key * 0x9E3779B1u) * (u64)cap) >> 32); }
+// Home slot of a topic below K: key * hm with hm = floor((2^32 - 1) / K) stays below 2^32, and the
+// multiply-shift maps it to [0, C), non-decreasing in the key.  A line of the table then holds a
+// run of neighbouring topics, so the phi values its 64 slots gather lie in a few cache lines
+// (a multiplicative hash spreads them over the whole phi row, one cache line per slot).
+LDA_HD u32 map_home_mult(u32 K) { return 0xFFFFFFFFu / K; }
+LDA_HD u32 map_home(u32 key, u32 cap, u32 hm) { return (u32)(((u64)(key * hm) * (u64)cap) >> 32); }
 
 // Displacement of an entry with home h stored in slot s.
 LDA_HD u32 map_disp(u32 s, u32 h, u32 cap) { return s >= h ? s - h : s + cap - h; }

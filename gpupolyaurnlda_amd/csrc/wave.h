@@ -92,8 +92,9 @@ __device__ __forceinline__ u64 wave_scan_u64(u64 v) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Doc-topic map operations.  T points at `cap` LDS slots (cap a multiple of 64, never full).
-// All 64 lanes of the wave call these with the same (uniform) arguments.
+// Doc-topic map operations.  T points at `cap` LDS slots (cap a multiple of 64, never full); hm is
+// map_home_mult(K) for keys below K.  All 64 lanes of the wave call these with the same (uniform)
+// arguments.
 // ---------------------------------------------------------------------------------------------
 // x mod cap for x < 2 * cap: x - cap wraps past 2^32 when x < cap, so the minimum is the slot.
 __device__ __forceinline__ u32 wrap_slot(u32 x, u32 cap) { return min(x, x - cap); }
@@ -113,9 +114,9 @@ __device__ __forceinline__ void map_store_line(u32* T, u32 line_slot, u32 line_e
   T[line_slot] = lane_id() == at_lane ? entry : line_e;
 }
 
-__device__ __forceinline__ MapProbe map_probe(const u32* T, u32 cap, u32 key) {
+__device__ __forceinline__ MapProbe map_probe(const u32* T, u32 cap, u32 hm, u32 key) {
   const u32 lane = lane_id();
-  const u32 h = map_home(key, cap);
+  const u32 h = map_home(key, cap, hm);
   wave_sync();
   MapProbe p;
   p.slot = 0;
@@ -131,7 +132,7 @@ __device__ __forceinline__ MapProbe map_probe(const u32* T, u32 cap, u32 key) {
     u32 ek = map_key(e);
     bool empty = e == kMapEmpty;
     // (computed for empty slots too, where the stop mask below already holds)
-    u32 de = map_disp(s, map_home(ek, cap), cap);
+    u32 de = map_disp(s, map_home(ek, cap, hm), cap);
     // match = !empty && ek == key; stop = empty || de < d || (de == d && ek > key), combined as
     // lane masks of single comparisons
     const u64 b_empty = ballot(empty);
@@ -155,9 +156,9 @@ __device__ __forceinline__ MapProbe map_probe(const u32* T, u32 cap, u32 key) {
 
 // Slot and entry of a key that is in the map.  Only the match is tested: a present key lies before
 // the first empty slot of its probe sequence, so no stop test is needed.
-__device__ __forceinline__ MapProbe map_find(const u32* T, u32 cap, u32 key) {
+__device__ __forceinline__ MapProbe map_find(const u32* T, u32 cap, u32 hm, u32 key) {
   const u32 lane = lane_id();
-  const u32 h = map_home(key, cap);
+  const u32 h = map_home(key, cap, hm);
   wave_sync();
   MapProbe p;
   p.slot = 0;
@@ -210,13 +211,15 @@ __device__ __forceinline__ u32 map_inc_at(u32* T, u32 cap, u32 key, MapProbe p) 
 }
 
 // Count of `key` +1; inserts it (Robin Hood shift) when absent.  Returns 1 if a key was added.
-__device__ __forceinline__ u32 map_inc(u32* T, u32 cap, u32 key) { return map_inc_at(T, cap, key, map_probe(T, cap, key)); }
+__device__ __forceinline__ u32 map_inc(u32* T, u32 cap, u32 hm, u32 key) {
+  return map_inc_at(T, cap, key, map_probe(T, cap, hm, key));
+}
 
 // Count of `key` -1 (the key must be present); deletes it by backward shift at zero.
 // Returns 1 if the key was removed.
-__device__ __forceinline__ u32 map_dec(u32* T, u32 cap, u32 key) {
+__device__ __forceinline__ u32 map_dec(u32* T, u32 cap, u32 hm, u32 key) {
   const u32 lane = lane_id();
-  MapProbe p = map_find(T, cap, key);
+  MapProbe p = map_find(T, cap, hm, key);
   if (!p.found) return 0;  // not reached: every decremented topic is in the map
   if (map_count(p.entry) > 1) {
     map_store_line(T, p.line_slot, p.line_e, p.lane, p.entry - 1);
@@ -228,7 +231,7 @@ __device__ __forceinline__ u32 map_dec(u32* T, u32 cap, u32 key) {
     wave_sync();
     const u32 s = wrap_slot(start + base + lane, cap);
     u32 e = T[s];
-    u64 bs = ballot(e == kMapEmpty) | ballot(map_disp(s, map_home(map_key(e), cap), cap) == 0);
+    u64 bs = ballot(e == kMapEmpty) | ballot(map_disp(s, map_home(map_key(e), cap, hm), cap) == 0);
     u32 fs = bs ? first_lane(bs) : 64;
     u32 dst = s == 0 ? cap - 1 : s - 1;
     if (lane <= fs) T[dst] = lane < fs ? e : kMapEmpty;

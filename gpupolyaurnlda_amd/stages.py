@@ -39,11 +39,12 @@ def scan64(x) -> np.ndarray:
     return out
 
 
-def map_apply(ops, cap: int) -> np.ndarray:
-    """Applies increments (topic) / decrements (topic | 1<<31) to an empty doc-topic map."""
+def map_apply(ops, cap: int, K: int) -> np.ndarray:
+    """Applies increments (topic) / decrements (topic | 1<<31) to an empty doc-topic map of topics
+    below K."""
     ops = _u32(ops)
     out = np.empty(cap, dtype=np.uint32)
-    check(lib().gpulda_stage_map(ptr(ops, C.c_uint32), len(ops), cap, ptr(out, C.c_uint32)), "stage_map")
+    check(lib().gpulda_stage_map(ptr(ops, C.c_uint32), len(ops), cap, K, ptr(out, C.c_uint32)), "stage_map")
     return out
 
 

@@ -122,8 +122,9 @@ int gpulda_serve_close(void);
 /* ---- stage entry points (host arrays in and out; the library stages device copies) ------- */
 int gpulda_stage_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n);
 int gpulda_stage_scan64(const float* in, float* out, uint32_t n_groups);
-/* ops[i] = topic (increment) or topic | 1u<<31 (decrement); returns the final table. */
-int gpulda_stage_map(const uint32_t* ops, uint32_t n_ops, uint32_t cap, uint32_t* table_out);
+/* ops[i] = topic (increment) or topic | 1u<<31 (decrement), topics below K (the home slots
+ * depend on K); returns the final table. */
+int gpulda_stage_map(const uint32_t* ops, uint32_t n_ops, uint32_t cap, uint32_t K, uint32_t* table_out);
 /* Row normalisation (by divisor[k] when divisor != NULL), row sums and alias tables. */
 int gpulda_stage_alias(const float* rows, uint32_t n_rows, uint32_t m, const uint64_t* divisor, float alpha,
                        float* rows_out, float* sums_out, float* prob_out, uint32_t* alias_out);

@@ -336,33 +336,36 @@ u32 orc_map_capacity(u32 L, u32 K) {
   u32 c = need + need / 4 + 1;
   return ((c + 63) / 64) * 64;
 }
-static u32 home_of(u32 key, u32 cap) { return (u32)(((u64)(key * 0x9E3779B1u) * cap) >> 32); }
+/* home slot of a topic below K: key * hm, hm = floor((2^32 - 1) / K), scaled to [0, cap) by
+ * multiply-shift -- non-decreasing in the key, so the table is ordered by topic */
+u32 orc_map_home_mult(u32 K) { return 0xFFFFFFFFu / K; }
+static u32 home_of(u32 key, u32 cap, u32 hm) { return (u32)(((u64)(key * hm) * cap) >> 32); }
 static u32 dist(u32 s, u32 h, u32 cap) { return (s + cap - h) % cap; }
 
 /* returns slot of key or -1 */
-long orc_map_find(const u32* T, u32 cap, u32 key) {
-  u32 h = home_of(key, cap);
+long orc_map_find(const u32* T, u32 cap, u32 hm, u32 key) {
+  u32 h = home_of(key, cap, hm);
   for (u32 d = 0; d < cap; ++d) {
     u32 s = (h + d) % cap;
     u32 e = T[s];
     if (e == MAP_EMPTY) return -1;
     if ((e >> 16) == key) return (long)s;
-    u32 de = dist(s, home_of(e >> 16, cap), cap);
+    u32 de = dist(s, home_of(e >> 16, cap, hm), cap);
     if (de < d || (de == d && (e >> 16) > key)) return -1;
   }
   return -1;
 }
 
-void orc_map_inc(u32* T, u32 cap, u32 key) {
-  long s = orc_map_find(T, cap, key);
+void orc_map_inc(u32* T, u32 cap, u32 hm, u32 key) {
+  long s = orc_map_find(T, cap, hm, key);
   if (s >= 0) { T[s] += 1; return; }
   u32 cur = (key << 16) | 1u;
-  u32 slot = home_of(key, cap);
+  u32 slot = home_of(key, cap, hm);
   u32 d = 0;
   for (;;) {
     u32 e = T[slot];
     if (e == MAP_EMPTY) { T[slot] = cur; return; }
-    u32 de = dist(slot, home_of(e >> 16, cap), cap);
+    u32 de = dist(slot, home_of(e >> 16, cap, hm), cap);
     if (de < d || (de == d && (e >> 16) > (cur >> 16))) {
       T[slot] = cur;
       cur = e;
@@ -374,13 +377,13 @@ void orc_map_inc(u32* T, u32 cap, u32 key) {
 }
 
 /* Inserts an absent key with its count (Robin Hood order). */
-static void map_put(u32* T, u32 cap, u32 cur) {
-  u32 slot = home_of(cur >> 16, cap);
+static void map_put(u32* T, u32 cap, u32 hm, u32 cur) {
+  u32 slot = home_of(cur >> 16, cap, hm);
   u32 d = 0;
   for (;;) {
     u32 e = T[slot];
     if (e == MAP_EMPTY) { T[slot] = cur; return; }
-    u32 de = dist(slot, home_of(e >> 16, cap), cap);
+    u32 de = dist(slot, home_of(e >> 16, cap, hm), cap);
     if (de < d || (de == d && (e >> 16) > (cur >> 16))) {
       T[slot] = cur;
       cur = e;
@@ -393,23 +396,23 @@ static void map_put(u32* T, u32 cap, u32 cur) {
 
 /* Moves every entry of a table of capacity `cap` into a cleared table of capacity `ncap`
  * (same array, ncap >= cap). */
-static void map_regrow(u32* T, u32 cap, u32 ncap, u32* scratch) {
+static void map_regrow(u32* T, u32 cap, u32 ncap, u32 hm, u32* scratch) {
   u32 n = 0;
   for (u32 s = 0; s < cap; ++s)
     if (T[s] != MAP_EMPTY) scratch[n++] = T[s];
   for (u32 s = 0; s < ncap; ++s) T[s] = MAP_EMPTY;
-  for (u32 i = 0; i < n; ++i) map_put(T, ncap, scratch[i]);
+  for (u32 i = 0; i < n; ++i) map_put(T, ncap, hm, scratch[i]);
 }
 
-void orc_map_dec(u32* T, u32 cap, u32 key) {
-  long s = orc_map_find(T, cap, key);
+void orc_map_dec(u32* T, u32 cap, u32 hm, u32 key) {
+  long s = orc_map_find(T, cap, hm, key);
   if (s < 0) abort();
   if ((T[s] & 0xFFFFu) > 1) { T[s] -= 1; return; }
   u32 hole = (u32)s;
   for (;;) {
     u32 nx = (hole + 1) % cap;
     u32 e = T[nx];
-    if (e == MAP_EMPTY || dist(nx, home_of(e >> 16, cap), cap) == 0) { T[hole] = MAP_EMPTY; return; }
+    if (e == MAP_EMPTY || dist(nx, home_of(e >> 16, cap, hm), cap) == 0) { T[hole] = MAP_EMPTY; return; }
     T[hole] = e;
     hole = nx;
   }
@@ -439,7 +442,7 @@ typedef struct {
   u32* T;
   float* mass;
   u32* scratch;
-  u32 L, K, cap, keys, kmax;
+  u32 L, K, cap, keys, kmax, hm;
 } OrcDoc;
 
 /* Opens a document: scratch holds >= 5 * orc_map_capacity(L, K) words. */
@@ -451,15 +454,16 @@ static OrcDoc doc_open(const u32* z, u32 L, u32 K, u32* scratch) {
   d.scratch = (u32*)(d.mass + cap_max);
   d.L = L;
   d.K = K;
+  d.hm = orc_map_home_mult(K);
   d.kmax = L < K ? L : K;
   u32 n0 = 0;
   for (u32 s = 0; s < cap_max; ++s) d.T[s] = MAP_EMPTY;
   for (u32 i = 0; i < L; ++i) {
-    if (orc_map_find(d.T, cap_max, z[i]) < 0) ++n0;
-    orc_map_inc(d.T, cap_max, z[i]);
+    if (orc_map_find(d.T, cap_max, d.hm, z[i]) < 0) ++n0;
+    orc_map_inc(d.T, cap_max, d.hm, z[i]);
   }
   d.cap = orc_map_capacity(n0, K);
-  map_regrow(d.T, cap_max, d.cap, d.scratch); /* the same key set at the starting capacity */
+  map_regrow(d.T, cap_max, d.cap, d.hm, d.scratch); /* the same key set at the starting capacity */
   d.keys = n0;
   return d;
 }
@@ -475,9 +479,9 @@ static u64 doc_pass(OrcDoc* d, u32* z, const u32* w, u32 doc, u32 iter, u32 tag,
     u32 wi = w[i], zi = z[i];
     u32 r[4];
     draw4(i, doc, iter, tag, 0, seed, r);
-    long at = orc_map_find(d->T, d->cap, zi);
+    long at = orc_map_find(d->T, d->cap, d->hm, zi);
     if ((d->T[at] & 0xFFFFu) == 1) --d->keys;
-    orc_map_dec(d->T, d->cap, zi);
+    orc_map_dec(d->T, d->cap, d->hm, zi);
     float sa = sigma_a[wi];
     u32 znew = zi;
     if (sa > 0.0f) {
@@ -542,16 +546,16 @@ static u64 doc_pass(OrcDoc* d, u32* z, const u32* w, u32 doc, u32 iter, u32 tag,
         znew = draw_alias(alias + (u64)wi * K, K, r[1], r[2]);
       }
     }
-    if (orc_map_find(d->T, d->cap, znew) < 0) {
+    if (orc_map_find(d->T, d->cap, d->hm, znew) < 0) {
       if (orc_map_capacity(d->keys + 1, K) > d->cap) {
         u32 want = 2 * (d->keys + 1) < d->kmax ? 2 * (d->keys + 1) : d->kmax;
         u32 ncap = orc_map_capacity(want, K);
-        map_regrow(d->T, d->cap, ncap, d->scratch);
+        map_regrow(d->T, d->cap, ncap, d->hm, d->scratch);
         d->cap = ncap;
       }
       ++d->keys;
     }
-    orc_map_inc(d->T, d->cap, znew);
+    orc_map_inc(d->T, d->cap, d->hm, znew);
     z[i] = znew;
   }
   return visited;
@@ -581,7 +585,7 @@ u64 orc_foldin_doc(u32* z, const u32* w, u32 L, u32 req, u32 sweeps, u64 seed, u
      * rebuilt at the capacity its keys need, orc_map_capacity(keys, K) */
     if (s > 1 && orc_map_capacity(d.keys, K) < d.cap) {
       u32 ncap = orc_map_capacity(d.keys, K);
-      map_regrow(d.T, d.cap, ncap, d.scratch);
+      map_regrow(d.T, d.cap, ncap, d.hm, d.scratch);
       d.cap = ncap;
     }
     visited += doc_pass(&d, z, w, req, s, TAG_FOLD_SWEEP, seed, phi, sigma_a, alias);
@@ -659,13 +663,14 @@ int orc_threads(void) {
 #endif
 }
 
-/* Map layout after building from a list of keys with increments (+1) and decrements (-1),
- * for the table-layout tests. ops[i] = key | (is_dec << 31). */
-void orc_map_apply(u32* T, u32 cap, const u32* ops, u32 n_ops) {
+/* Map layout after building from a list of keys (below K) with increments (+1) and decrements
+ * (-1), for the table-layout tests. ops[i] = key | (is_dec << 31). */
+void orc_map_apply(u32* T, u32 cap, u32 K, const u32* ops, u32 n_ops) {
+  const u32 hm = orc_map_home_mult(K);
   for (u32 s = 0; s < cap; ++s) T[s] = MAP_EMPTY;
   for (u32 i = 0; i < n_ops; ++i) {
-    if (ops[i] >> 31) orc_map_dec(T, cap, ops[i] & 0x7FFFFFFFu);
-    else orc_map_inc(T, cap, ops[i]);
+    if (ops[i] >> 31) orc_map_dec(T, cap, hm, ops[i] & 0x7FFFFFFFu);
+    else orc_map_inc(T, cap, hm, ops[i]);
   }
 }
 

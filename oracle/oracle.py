@@ -39,7 +39,8 @@ _SIGS = {
     "orc_init_counts": (None, [u32p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, C.POINTER(OrcAlias), u32p]),
     "orc_phi_finish": (None, [f32p, u64p, C.c_uint32, C.c_uint32, C.c_float, f32p, C.POINTER(OrcAlias)]),
     "orc_map_capacity": (C.c_uint32, [C.c_uint32, C.c_uint32]),
-    "orc_map_apply": (None, [u32p, C.c_uint32, u32p, C.c_uint32]),
+    "orc_map_apply": (None, [u32p, C.c_uint32, C.c_uint32, u32p, C.c_uint32]),
+    "orc_map_home_mult": (C.c_uint32, [C.c_uint32]),
     "orc_sweep": (C.c_uint64, [u32p, u32p, u64p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, f32p,
                                f32p, C.POINTER(OrcAlias), C.c_int]),
     "orc_foldin": (C.c_uint64, [u32p, u32p, u64p, C.c_uint32, u32p, C.c_uint32, C.c_uint64, C.c_uint32, f32p,
@@ -124,10 +125,14 @@ def map_capacity(L, K):
     return int(lib().orc_map_capacity(L, K))
 
 
-def map_apply(ops, cap):
+def map_home_mult(K):
+    return int(lib().orc_map_home_mult(K))
+
+
+def map_apply(ops, cap, K):
     ops = np.ascontiguousarray(ops, dtype=np.uint32)
     T = np.empty(cap, dtype=np.uint32)
-    lib().orc_map_apply(_p(T, C.c_uint32), cap, _p(ops, C.c_uint32), len(ops))
+    lib().orc_map_apply(_p(T, C.c_uint32), cap, K, _p(ops, C.c_uint32), len(ops))
     return T
 
 

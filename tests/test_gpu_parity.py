@@ -78,7 +78,7 @@ def test_map_layout(S, orc, K, L):
             ops.append(k)
             counts[k] = counts.get(k, 0) + 1
             total += 1
-    assert np.array_equal(S.map_apply(ops, cap), orc.map_apply(ops, cap))
+    assert np.array_equal(S.map_apply(ops, cap, K), orc.map_apply(ops, cap, K))
 
 
 @pytest.mark.parametrize("m", [1, 5, 20, 64, 200, 1024, 3000])

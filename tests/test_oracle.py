@@ -197,8 +197,11 @@ def test_poisson_draws_chisquare(orc, n):
         assert abs(draws.var() - lam) < 0.1 * lam
 
 
-def _check_map(T, cap, counts: dict, orc):
-    home = lambda k: ((k * 0x9E3779B1) % 2**32 * cap) >> 32  # noqa: E731
+def _check_map(T, cap, K, counts: dict, orc):
+    hm = (2**32 - 1) // K  # the home multiplier: floor((2^32 - 1) / K)
+    assert orc.map_home_mult(K) == hm
+    home = lambda k: ((k * hm) * cap) >> 32  # noqa: E731
+    assert all(k * hm < 2**32 for k in range(K))  # no wrap: homes are non-decreasing in the key
     EMPTY = 0  # an empty slot; a stored entry has count >= 1
     seen = {}
     for s in range(cap):
@@ -241,24 +244,29 @@ def test_map_random_ops(orc, K, L):
             ops.append(k)
             counts[k] = counts.get(k, 0) + 1
             total += 1
-    T = orc.map_apply(ops, cap)
-    _check_map(T, cap, counts, orc)
+    T = orc.map_apply(ops, cap, K)
+    _check_map(T, cap, K, counts, orc)
     # history independence: the final multiset inserted in two other orders gives the same table
     final = [k for k, c in counts.items() for _ in range(c)]
     for seed in (1, 2):
         order = list(np.random.default_rng(seed).permutation(final)) if final else []
-        assert np.array_equal(orc.map_apply(order, cap), T)
+        assert np.array_equal(orc.map_apply(order, cap, K), T)
+    # order-preserving homes and Robin Hood order make the table sorted by key, read from the
+    # slot after the last wrapped run (at most one descent in the cyclic sequence of keys)
+    keys = [int(e) >> 16 for e in T if e != 0]
+    descents = sum(1 for i in range(len(keys)) if keys[i] > keys[(i + 1) % len(keys)])
+    assert len(keys) < 2 or descents == 1
 
 
 def test_map_reference_known_answers(orc):
     # count_topics (test_topics.cu:134-189): 100 tokens, z = i % 25 -> each count 4
     cap = orc.map_capacity(100, 25)
-    T = orc.map_apply([i % 25 for i in range(100)], cap)
+    T = orc.map_apply([i % 25 for i in range(100)], cap, 25)
     got = {int(e) >> 16: int(e) & 0xFFFF for e in T if e != 0}
     assert got == {k: 4 for k in range(25)}
     # test_hashmap.cu:224-266: 90 inserts over 9 keys -> every count is 10
     cap = orc.map_capacity(90, 9)
-    T = orc.map_apply([i % 9 for i in range(90)], cap)
+    T = orc.map_apply([i % 9 for i in range(90)], cap, 9)
     got = {int(e) >> 16: int(e) & 0xFFFF for e in T if e != 0}
     assert got == {k: 10 for k in range(9)}
     assert orc.map_capacity(100, 1024) == 128 and orc.map_capacity(50, 20) == 64
