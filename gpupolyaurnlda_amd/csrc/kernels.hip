@@ -261,11 +261,11 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
 // ---------------------------------------------------------------------------------------------
 constexpr u32 kNoSlot = 0xFFFFFFFFu;
 
-// Mass of one map entry for row prow.  An empty slot (0) reads as count 0 at key 0, so the gather
-// and the product are unconditional (no branch, all gathers of a token in flight together) and
-// its mass is 0 * phi[w][0] = +0, the value the definition gives it.
+// Mass of one map entry for row prow.  An empty slot reads as count 0 (at a topic below K), so the
+// gather and the product are unconditional (no branch, all gathers of a token in flight together)
+// and its mass is 0 * phi[w][key] = +0, the value the definition gives it.
 __device__ __forceinline__ f32 slot_mass(u32 e, const f32* __restrict__ prow) {
-  static_assert(kMapEmpty == 0u, "an empty slot must read as count 0 at key 0");
+  static_assert(kMapEmpty == 0u, "reads past a map read count 0 at key 0");
   return (f32)map_count(e) * prow[map_key(e)];
 }
 
@@ -297,7 +297,8 @@ __device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u32* bits, 
                                               const u16* __restrict__ z, u64 b, u32 L) {
   const u32 lane = lane_id();
   const u32 hm = map_home_mult(K);
-  for (u32 s = lane; s < cap; s += 64) T[s] = kMapEmpty;
+  const u32 eq = map_empty_q(K, cap);
+  for (u32 s = lane; s < cap; s += 64) T[s] = map_empty(s, eq);
   // 64 bitmap words per step; only words with a topic set are visited
   const u32 words = (K + 31) / 32;
   for (u32 w0 = 0; w0 < words; w0 += 64) {
@@ -319,7 +320,7 @@ __device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u32* bits, 
   if (distinct == 1) {
     // one topic holds all L tokens (1 + L would not fit the 16-bit count at L = 65535)
     for (u32 s = lane; s < cap; s += 64)
-      if (T[s] != kMapEmpty) T[s] = map_entry(map_key(T[s]), L);
+      if (!map_is_empty(T[s])) T[s] = map_entry(map_key(T[s]), L);
     return;
   }
   // with two or more topics no topic has more than L - 1 <= 65534 tokens, so 1 + count fits
@@ -329,7 +330,7 @@ __device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u32* bits, 
       u32 s = map_home(key, cap, hm);
       for (u32 n = 0; n < cap; ++n) {  // the key is present, so the probe reaches it first
         const u32 e = T[s];
-        if (e != kMapEmpty && map_key(e) == key) {
+        if (!map_is_empty(e) && map_key(e) == key) {
           atomicAdd(&T[s], 1u);
           break;
         }
@@ -339,7 +340,7 @@ __device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u32* bits, 
   }
   wave_sync();
   for (u32 s = lane; s < cap; s += 64)
-    if (T[s] != kMapEmpty) T[s] -= 1u;
+    if (!map_is_empty(T[s])) T[s] -= 1u;
 }
 
 // Regrows the map to capacity ncap in the middle of a sweep: clears it and re-inserts the topic
@@ -347,11 +348,13 @@ __device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u32* bits, 
 // in z, lanes < t of the chunk in znew_lane, lanes > t in zz, and tokens after the chunk their
 // old topics in z -- exactly the multiset the map held, so (the layout being a function of the
 // key set) the result equals any other way of moving the entries.
-__device__ __noinline__ void regrow_doc_map(u32* T, u32 ncap, u32 hm, const u16* __restrict__ z, u64 b, u32 L,
+__device__ __noinline__ void regrow_doc_map(u32* T, u32 ncap, u32 K, const u16* __restrict__ z, u64 b, u32 L,
                                             u32 c0, u32 t, u32 znew_lane, u32 zz) {
   const u32 lane = lane_id();
+  const u32 hm = map_home_mult(K);
+  const u32 eq = map_empty_q(K, ncap);
   wave_sync();
-  for (u32 s = lane; s < ncap; s += 64) T[s] = kMapEmpty;
+  for (u32 s = lane; s < ncap; s += 64) T[s] = map_empty(s, eq);
   for (u32 c = 0; c < L; c += 64) {
     u32 nt = L - c < 64 ? L - c : 64;
     u32 v;
@@ -372,6 +375,7 @@ struct DocState {
   u32 occupied;   // distinct topics in the map
   u32 gdoc;       // global document id (Philox counter word 1)
   u32 hm;         // map_home_mult(K)
+  u32 eq;         // map_empty_q(K, cap)
 };
 
 // Capacity rule: start sized for the n0 distinct topics; grow (rebuild) when a new topic would
@@ -386,6 +390,7 @@ __device__ __forceinline__ DocState open_span(const SweepParams& P, u64 b, u32 L
   ds.hm = map_home_mult(P.K);
   ds.occupied = uniform_u32(distinct_topics(T + ds.cap_max, P.K, P.z, ds.b, ds.L));
   ds.cap = map_capacity(ds.occupied, P.K);
+  ds.eq = map_empty_q(P.K, ds.cap);
   build_doc_map(T, ds.cap, T + ds.cap_max, P.K, ds.occupied, P.z, ds.b, ds.L);
   return ds;
 }
@@ -581,7 +586,8 @@ __device__ __forceinline__ void add_topic(const SweepParams& P, u32* T, DocState
     if (map_capacity(ds.occupied + 1, P.K) > ds.cap) {
       const u32 want = 2 * (ds.occupied + 1) < ds.kmax ? 2 * (ds.occupied + 1) : ds.kmax;
       ds.cap = map_capacity(want, P.K);
-      regrow_doc_map(T, ds.cap, ds.hm, P.z, ds.b, ds.L, c0, t, znew_lane, zz);
+      ds.eq = map_empty_q(P.K, ds.cap);
+      regrow_doc_map(T, ds.cap, P.K, P.z, ds.b, ds.L, c0, t, znew_lane, zz);
       pz = map_probe(T, ds.cap, ds.hm, pk.topic);
     }
     ++ds.occupied;
@@ -614,7 +620,7 @@ __device__ __forceinline__ void sweep_pass(const SweepParams& P, u32* T, DocStat
     for (u32 t = 0; t < nt; ++t) {
       const u32 wt = readlane_u32(ww, t);
       const u32 zt = readlane_u32(zz, t);
-      ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, zt));
+      ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, ds.eq, zt));
       const f32 sa = readlane_f32(sa_l, t);
       Pick pk;
       pk.topic = zt;
@@ -640,6 +646,7 @@ __device__ __forceinline__ void shrink_doc_map(const SweepParams& P, u32* T, Doc
   const u32 want = map_capacity(ds.occupied, P.K);
   if (want < ds.cap) {
     ds.cap = want;
+    ds.eq = map_empty_q(P.K, ds.cap);
     wave_sync();
     const u32 distinct = uniform_u32(distinct_topics(T + ds.cap_max, P.K, P.z, ds.b, ds.L));
     build_doc_map(T, ds.cap, T + ds.cap_max, P.K, distinct, P.z, ds.b, ds.L);
@@ -756,7 +763,7 @@ __global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u
         const u32 zt = readlane_u32(zz, t);
         const f32 sa = readlane_f32(sa_l, t);
         if (wave == 0) {
-          ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, zt));
+          ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, ds.eq, zt));
           if (lane == 0) sh_cap = ds.cap;
         }
         __syncthreads();
@@ -1078,10 +1085,11 @@ __global__ void k_stage_map(const u32* ops, u32 n_ops, u32 cap, u32 K, u32* out)
   extern __shared__ u32 map_smem[];
   const u32 lane = lane_id();
   const u32 hm = map_home_mult(K);
-  for (u32 s = lane; s < cap; s += 64) map_smem[s] = kMapEmpty;
+  const u32 eq = map_empty_q(K, cap);
+  for (u32 s = lane; s < cap; s += 64) map_smem[s] = map_empty(s, eq);
   for (u32 i = 0; i < n_ops; ++i) {
     u32 op = ops[i];
-    if (op >> 31) map_dec(map_smem, cap, hm, op & 0x7FFFFFFFu);
+    if (op >> 31) map_dec(map_smem, cap, hm, eq, op & 0x7FFFFFFFu);
     else map_inc(map_smem, cap, hm, op);
   }
   wave_sync();

@@ -93,8 +93,8 @@ __device__ __forceinline__ u64 wave_scan_u64(u64 v) {
 
 // ---------------------------------------------------------------------------------------------
 // Doc-topic map operations.  T points at `cap` LDS slots (cap a multiple of 64, never full); hm is
-// map_home_mult(K) for keys below K.  All 64 lanes of the wave call these with the same (uniform)
-// arguments.
+// map_home_mult(K) for keys below K, eq is map_empty_q(K, cap).  A slot is empty when its count is
+// 0.  All 64 lanes of the wave call these with the same (uniform) arguments.
 // ---------------------------------------------------------------------------------------------
 // x mod cap for x < 2 * cap: x - cap wraps past 2^32 when x < cap, so the minimum is the slot.
 __device__ __forceinline__ u32 wrap_slot(u32 x, u32 cap) { return min(x, x - cap); }
@@ -130,7 +130,7 @@ __device__ __forceinline__ MapProbe map_probe(const u32* T, u32 cap, u32 hm, u32
     const u32 s = wrap_slot(h + d, cap);
     u32 e = T[s];
     u32 ek = map_key(e);
-    bool empty = e == kMapEmpty;
+    bool empty = map_is_empty(e);
     // (computed for empty slots too, where the stop mask below already holds)
     u32 de = map_disp(s, map_home(ek, cap, hm), cap);
     // match = !empty && ek == key; stop = empty || de < d || (de == d && ek > key), combined as
@@ -170,12 +170,12 @@ __device__ __forceinline__ MapProbe map_find(const u32* T, u32 cap, u32 hm, u32 
   // the first line holds the key unless it was displaced past it (the common case is peeled)
   u32 base = 0;
   u32 e = T[wrap_slot(h + lane, cap)];
-  u64 bm = ballot(e != kMapEmpty) & ballot(map_key(e) == key);
+  u64 bm = ballot(!map_is_empty(e)) & ballot(map_key(e) == key);
   while (!bm) {
     base += kMapLine;
     if (base >= cap) return p;
     e = T[wrap_slot(h + base + lane, cap)];
-    bm = ballot(e != kMapEmpty) & ballot(map_key(e) == key);
+    bm = ballot(!map_is_empty(e)) & ballot(map_key(e) == key);
   }
   const u32 fl = first_lane(bm);
   p.slot = wrap_slot(h + base + fl, cap);
@@ -199,7 +199,7 @@ __device__ __forceinline__ u32 map_inc_at(u32* T, u32 cap, u32 key, MapProbe p) 
     wave_sync();
     const u32 s = wrap_slot(p.slot + base + lane, cap);
     u32 e = T[s];
-    u64 be = ballot(e == kMapEmpty);
+    u64 be = ballot(map_is_empty(e));
     u32 fe = be ? first_lane(be) : 64;
     u32 prev = __shfl_up(e, 1, 64);
     if (lane == 0) prev = carry;
@@ -217,7 +217,7 @@ __device__ __forceinline__ u32 map_inc(u32* T, u32 cap, u32 hm, u32 key) {
 
 // Count of `key` -1 (the key must be present); deletes it by backward shift at zero.
 // Returns 1 if the key was removed.
-__device__ __forceinline__ u32 map_dec(u32* T, u32 cap, u32 hm, u32 key) {
+__device__ __forceinline__ u32 map_dec(u32* T, u32 cap, u32 hm, u32 eq, u32 key) {
   const u32 lane = lane_id();
   MapProbe p = map_find(T, cap, hm, key);
   if (!p.found) return 0;  // not reached: every decremented topic is in the map
@@ -231,10 +231,10 @@ __device__ __forceinline__ u32 map_dec(u32* T, u32 cap, u32 hm, u32 key) {
     wave_sync();
     const u32 s = wrap_slot(start + base + lane, cap);
     u32 e = T[s];
-    u64 bs = ballot(e == kMapEmpty) | ballot(map_disp(s, map_home(map_key(e), cap, hm), cap) == 0);
+    u64 bs = ballot(map_is_empty(e)) | ballot(map_disp(s, map_home(map_key(e), cap, hm), cap) == 0);
     u32 fs = bs ? first_lane(bs) : 64;
     u32 dst = s == 0 ? cap - 1 : s - 1;
-    if (lane <= fs) T[dst] = lane < fs ? e : kMapEmpty;
+    if (lane <= fs) T[dst] = lane < fs ? e : map_empty(dst, eq);
     if (bs) break;
   }
   return 1;

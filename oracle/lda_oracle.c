@@ -329,7 +329,12 @@ void orc_phi_finish(float* phi /* in: phi~, out: phi */, const u64* topic_sum, u
 /* Doc-topic map: textbook sequential Robin Hood linear probing, ties broken by key, backward  */
 /* shift deletion.  Restates the HashMap of hashmap.cuh:13-334 (entry = key<<16 | count).     */
 /* ------------------------------------------------------------------------------------------ */
-#define MAP_EMPTY 0u /* a stored entry has count >= 1, so 0 never names one */
+/* A stored entry has count >= 1; an empty slot s holds count 0 at the topic (s * q) >> 16,
+ * q = floor(K * 2^16 / cap) -- a function of the slot alone (its mass 0 * phi is +0 whatever the
+ * topic). */
+#define IS_EMPTY(e) (((e) & 0xFFFFu) == 0u)
+u32 orc_map_empty_q(u32 K, u32 cap) { return (u32)(((u64)K << 16) / cap); }
+static u32 empty_of(u32 s, u32 q) { return (s * q) & 0xFFFF0000u; }
 
 u32 orc_map_capacity(u32 L, u32 K) {
   u32 need = L < K ? L : K;
@@ -348,7 +353,7 @@ long orc_map_find(const u32* T, u32 cap, u32 hm, u32 key) {
   for (u32 d = 0; d < cap; ++d) {
     u32 s = (h + d) % cap;
     u32 e = T[s];
-    if (e == MAP_EMPTY) return -1;
+    if (IS_EMPTY(e)) return -1;
     if ((e >> 16) == key) return (long)s;
     u32 de = dist(s, home_of(e >> 16, cap, hm), cap);
     if (de < d || (de == d && (e >> 16) > key)) return -1;
@@ -364,7 +369,7 @@ void orc_map_inc(u32* T, u32 cap, u32 hm, u32 key) {
   u32 d = 0;
   for (;;) {
     u32 e = T[slot];
-    if (e == MAP_EMPTY) { T[slot] = cur; return; }
+    if (IS_EMPTY(e)) { T[slot] = cur; return; }
     u32 de = dist(slot, home_of(e >> 16, cap, hm), cap);
     if (de < d || (de == d && (e >> 16) > (cur >> 16))) {
       T[slot] = cur;
@@ -382,7 +387,7 @@ static void map_put(u32* T, u32 cap, u32 hm, u32 cur) {
   u32 d = 0;
   for (;;) {
     u32 e = T[slot];
-    if (e == MAP_EMPTY) { T[slot] = cur; return; }
+    if (IS_EMPTY(e)) { T[slot] = cur; return; }
     u32 de = dist(slot, home_of(e >> 16, cap, hm), cap);
     if (de < d || (de == d && (e >> 16) > (cur >> 16))) {
       T[slot] = cur;
@@ -396,15 +401,16 @@ static void map_put(u32* T, u32 cap, u32 hm, u32 cur) {
 
 /* Moves every entry of a table of capacity `cap` into a cleared table of capacity `ncap`
  * (same array, ncap >= cap). */
-static void map_regrow(u32* T, u32 cap, u32 ncap, u32 hm, u32* scratch) {
+static void map_regrow(u32* T, u32 cap, u32 ncap, u32 K, u32* scratch) {
+  const u32 hm = orc_map_home_mult(K), q = orc_map_empty_q(K, ncap);
   u32 n = 0;
   for (u32 s = 0; s < cap; ++s)
-    if (T[s] != MAP_EMPTY) scratch[n++] = T[s];
-  for (u32 s = 0; s < ncap; ++s) T[s] = MAP_EMPTY;
+    if (!IS_EMPTY(T[s])) scratch[n++] = T[s];
+  for (u32 s = 0; s < ncap; ++s) T[s] = empty_of(s, q);
   for (u32 i = 0; i < n; ++i) map_put(T, ncap, hm, scratch[i]);
 }
 
-void orc_map_dec(u32* T, u32 cap, u32 hm, u32 key) {
+void orc_map_dec(u32* T, u32 cap, u32 hm, u32 q, u32 key) {
   long s = orc_map_find(T, cap, hm, key);
   if (s < 0) abort();
   if ((T[s] & 0xFFFFu) > 1) { T[s] -= 1; return; }
@@ -412,7 +418,7 @@ void orc_map_dec(u32* T, u32 cap, u32 hm, u32 key) {
   for (;;) {
     u32 nx = (hole + 1) % cap;
     u32 e = T[nx];
-    if (e == MAP_EMPTY || dist(nx, home_of(e >> 16, cap, hm), cap) == 0) { T[hole] = MAP_EMPTY; return; }
+    if (IS_EMPTY(e) || dist(nx, home_of(e >> 16, cap, hm), cap) == 0) { T[hole] = empty_of(hole, q); return; }
     T[hole] = e;
     hole = nx;
   }
@@ -457,13 +463,14 @@ static OrcDoc doc_open(const u32* z, u32 L, u32 K, u32* scratch) {
   d.hm = orc_map_home_mult(K);
   d.kmax = L < K ? L : K;
   u32 n0 = 0;
-  for (u32 s = 0; s < cap_max; ++s) d.T[s] = MAP_EMPTY;
+  const u32 q_max = orc_map_empty_q(K, cap_max);
+  for (u32 s = 0; s < cap_max; ++s) d.T[s] = empty_of(s, q_max);
   for (u32 i = 0; i < L; ++i) {
     if (orc_map_find(d.T, cap_max, d.hm, z[i]) < 0) ++n0;
     orc_map_inc(d.T, cap_max, d.hm, z[i]);
   }
   d.cap = orc_map_capacity(n0, K);
-  map_regrow(d.T, cap_max, d.cap, d.hm, d.scratch); /* the same key set at the starting capacity */
+  map_regrow(d.T, cap_max, d.cap, K, d.scratch); /* the same key set at the starting capacity */
   d.keys = n0;
   return d;
 }
@@ -481,7 +488,7 @@ static u64 doc_pass(OrcDoc* d, u32* z, const u32* w, u32 doc, u32 iter, u32 tag,
     draw4(i, doc, iter, tag, 0, seed, r);
     long at = orc_map_find(d->T, d->cap, d->hm, zi);
     if ((d->T[at] & 0xFFFFu) == 1) --d->keys;
-    orc_map_dec(d->T, d->cap, d->hm, zi);
+    orc_map_dec(d->T, d->cap, d->hm, orc_map_empty_q(K, d->cap), zi);
     float sa = sigma_a[wi];
     u32 znew = zi;
     if (sa > 0.0f) {
@@ -494,8 +501,8 @@ static u64 doc_pass(OrcDoc* d, u32* z, const u32* w, u32 doc, u32 iter, u32 tag,
       for (u32 j = 0; j < lines; ++j) {
         for (u32 l = 0; l < 64; ++l) {
           u32 e = d->T[j * 64 + l];
-          visited += e != MAP_EMPTY;
-          float m = (e == MAP_EMPTY) ? 0.0f : (float)(e & 0xFFFFu) * prow[e >> 16];
+          visited += !IS_EMPTY(e);
+          float m = IS_EMPTY(e) ? 0.0f : (float)(e & 0xFFFFu) * prow[e >> 16];
           d->mass[j * 64 + l] = m;
           G[j % 4][l] = G[j % 4][l] + m;
         }
@@ -550,7 +557,7 @@ static u64 doc_pass(OrcDoc* d, u32* z, const u32* w, u32 doc, u32 iter, u32 tag,
       if (orc_map_capacity(d->keys + 1, K) > d->cap) {
         u32 want = 2 * (d->keys + 1) < d->kmax ? 2 * (d->keys + 1) : d->kmax;
         u32 ncap = orc_map_capacity(want, K);
-        map_regrow(d->T, d->cap, ncap, d->hm, d->scratch);
+        map_regrow(d->T, d->cap, ncap, K, d->scratch);
         d->cap = ncap;
       }
       ++d->keys;
@@ -585,7 +592,7 @@ u64 orc_foldin_doc(u32* z, const u32* w, u32 L, u32 req, u32 sweeps, u64 seed, u
      * rebuilt at the capacity its keys need, orc_map_capacity(keys, K) */
     if (s > 1 && orc_map_capacity(d.keys, K) < d.cap) {
       u32 ncap = orc_map_capacity(d.keys, K);
-      map_regrow(d.T, d.cap, ncap, d.hm, d.scratch);
+      map_regrow(d.T, d.cap, ncap, K, d.scratch);
       d.cap = ncap;
     }
     visited += doc_pass(&d, z, w, req, s, TAG_FOLD_SWEEP, seed, phi, sigma_a, alias);
@@ -666,10 +673,10 @@ int orc_threads(void) {
 /* Map layout after building from a list of keys (below K) with increments (+1) and decrements
  * (-1), for the table-layout tests. ops[i] = key | (is_dec << 31). */
 void orc_map_apply(u32* T, u32 cap, u32 K, const u32* ops, u32 n_ops) {
-  const u32 hm = orc_map_home_mult(K);
-  for (u32 s = 0; s < cap; ++s) T[s] = MAP_EMPTY;
+  const u32 hm = orc_map_home_mult(K), q = orc_map_empty_q(K, cap);
+  for (u32 s = 0; s < cap; ++s) T[s] = empty_of(s, q);
   for (u32 i = 0; i < n_ops; ++i) {
-    if (ops[i] >> 31) orc_map_dec(T, cap, hm, ops[i] & 0x7FFFFFFFu);
+    if (ops[i] >> 31) orc_map_dec(T, cap, hm, q, ops[i] & 0x7FFFFFFFu);
     else orc_map_inc(T, cap, hm, ops[i]);
   }
 }

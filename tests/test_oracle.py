@@ -202,11 +202,14 @@ def _check_map(T, cap, K, counts: dict, orc):
     assert orc.map_home_mult(K) == hm
     home = lambda k: ((k * hm) * cap) >> 32  # noqa: E731
     assert all(k * hm < 2**32 for k in range(K))  # no wrap: homes are non-decreasing in the key
-    EMPTY = 0  # an empty slot; a stored entry has count >= 1
+    # an empty slot s holds count 0 at the topic (s * q) >> 16, q = floor(K * 2^16 / cap)
+    q = (K << 16) // cap
+    empty = lambda e: e & 0xFFFF == 0  # noqa: E731
     seen = {}
     for s in range(cap):
         e = int(T[s])
-        if e == EMPTY:
+        if empty(e):
+            assert e == ((s * q) >> 16) << 16 and (s * q) >> 16 < K
             continue
         k, c = e >> 16, e & 0xFFFF
         assert k not in seen
@@ -214,7 +217,7 @@ def _check_map(T, cap, K, counts: dict, orc):
         h = home(k)
         d = (s - h) % cap
         for j in range(d):  # every slot between home and position is occupied
-            assert T[(h + j) % cap] != EMPTY
+            assert not empty(int(T[(h + j) % cap]))
         if d > 0:  # Robin Hood order against the previous slot
             p = int(T[(s - 1) % cap])
             pk = p >> 16
@@ -253,7 +256,7 @@ def test_map_random_ops(orc, K, L):
         assert np.array_equal(orc.map_apply(order, cap, K), T)
     # order-preserving homes and Robin Hood order make the table sorted by key, read from the
     # slot after the last wrapped run (at most one descent in the cyclic sequence of keys)
-    keys = [int(e) >> 16 for e in T if e != 0]
+    keys = [int(e) >> 16 for e in T if int(e) & 0xFFFF]
     descents = sum(1 for i in range(len(keys)) if keys[i] > keys[(i + 1) % len(keys)])
     assert len(keys) < 2 or descents == 1
 
@@ -262,12 +265,12 @@ def test_map_reference_known_answers(orc):
     # count_topics (test_topics.cu:134-189): 100 tokens, z = i % 25 -> each count 4
     cap = orc.map_capacity(100, 25)
     T = orc.map_apply([i % 25 for i in range(100)], cap, 25)
-    got = {int(e) >> 16: int(e) & 0xFFFF for e in T if e != 0}
+    got = {int(e) >> 16: int(e) & 0xFFFF for e in T if int(e) & 0xFFFF}
     assert got == {k: 4 for k in range(25)}
     # test_hashmap.cu:224-266: 90 inserts over 9 keys -> every count is 10
     cap = orc.map_capacity(90, 9)
     T = orc.map_apply([i % 9 for i in range(90)], cap, 9)
-    got = {int(e) >> 16: int(e) & 0xFFFF for e in T if e != 0}
+    got = {int(e) >> 16: int(e) & 0xFFFF for e in T if int(e) & 0xFFFF}
     assert got == {k: 10 for k in range(9)}
     assert orc.map_capacity(100, 1024) == 128 and orc.map_capacity(50, 20) == 64
     assert orc.map_capacity(5000, 4096) == 64 * math.ceil((4096 + 1024 + 1) / 64)
