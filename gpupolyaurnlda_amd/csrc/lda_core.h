@@ -147,10 +147,11 @@ LDA_HD u32 map_home(u32 key, u32 cap, u32 hm) { return (u32)(((u64)(key * hm) * 
 LDA_HD u32 map_disp(u32 s, u32 h, u32 cap) { return s >= h ? s - h : s + cap - h; }
 
 // Slots for a document of length L over K topics: room for every distinct topic it can hold,
-// load factor <= 0.8, at least one free slot, rounded up to whole 64-slot lines.
+// load factor <= 32/33, at least one free slot, rounded up to whole 64-slot lines.  Dense tables
+// keep the lines each token's sum reads few; Robin Hood order keeps their probes short.
 LDA_HD u32 map_capacity(u32 L, u32 K) {
   u32 need = L < K ? L : K;
-  u32 c = need + need / 4 + 1;
+  u32 c = need + need / 32 + 1;
   return ((c + kMapLine - 1) / kMapLine) * kMapLine;
 }
 

@@ -338,7 +338,7 @@ static u32 empty_of(u32 s, u32 q) { return (s * q) & 0xFFFF0000u; }
 
 u32 orc_map_capacity(u32 L, u32 K) {
   u32 need = L < K ? L : K;
-  u32 c = need + need / 4 + 1;
+  u32 c = need + need / 32 + 1;
   return ((c + 63) / 64) * 64;
 }
 /* home slot of a topic below K: key * hm, hm = floor((2^32 - 1) / K), scaled to [0, cap) by

@@ -273,7 +273,9 @@ def test_map_reference_known_answers(orc):
     got = {int(e) >> 16: int(e) & 0xFFFF for e in T if int(e) & 0xFFFF}
     assert got == {k: 10 for k in range(9)}
     assert orc.map_capacity(100, 1024) == 128 and orc.map_capacity(50, 20) == 64
-    assert orc.map_capacity(5000, 4096) == 64 * math.ceil((4096 + 1024 + 1) / 64)
+    # load factor <= 32/33, rounded up to whole lines
+    assert orc.map_capacity(5000, 4096) == 64 * math.ceil((4096 + 128 + 1) / 64)
+    assert orc.map_capacity(600, 1024) == 640 and orc.map_capacity(4096, 1024) == 1088
 
 
 def _two_token_joint(phi, alpha, w, z):
