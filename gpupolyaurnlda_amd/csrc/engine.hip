@@ -309,6 +309,11 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
     std::vector<u64> cur(start.begin(), start.end() - 1);
     for (u64 i = 0; i < n_tokens; ++i) perm[cur[w[i]]++] = (u32)i;
   }
+  // count tasks in the order of their first token: the tasks running together then gather z from
+  // the same stretch of the corpus, so a z line fetched for one word serves the other words of
+  // that stretch from cache (in word order the tasks of one frequent word span the whole corpus)
+  std::sort(tasks.begin(), tasks.end(),
+            [&](const CountTask& a, const CountTask& b) { return perm[a.p0] < perm[b.p0]; });
   // sweep buckets: by register-resident line count, largest first; longest documents first
   std::vector<u32> order(n_docs);
   std::vector<int> lmax_of(n_docs);
