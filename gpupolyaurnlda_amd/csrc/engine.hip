@@ -111,6 +111,7 @@ int max_lds_per_block() {
 int launch_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
                       AliasSlot* out, hipStream_t st) {
   if (n_rows == 0) return 0;
+  if (m > kMaxTopics) return fail("alias rows of %u entries (limit %u: slot indices are 16-bit in LDS)", m, kMaxTopics);
   u64 per = alias_lds_per_wave(m);
   u64 lim = (u64)max_lds_per_block();
   if (per > lim) return fail("row length %u needs %llu B of LDS (limit %llu)", m, (unsigned long long)per,

@@ -118,6 +118,6 @@ __global__ void k_stage_alias_draw(const AliasSlot* row, u32 m, const u32* r_slo
                                    u32* out);
 
 // LDS bytes per alias row of length m (one wavefront).
-inline u64 alias_lds_per_wave(u32 m) { return (u64)((m + 1) & ~1u) * 3 * sizeof(u32); }
+inline u64 alias_lds_per_wave(u32 m) { return (u64)((m + 3) & ~3u) * 10; }
 
 }  // namespace lda

@@ -125,9 +125,10 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
   const u32 wave = threadIdx.x >> 6;
   const u32 row = blockIdx.x * rows_per_block_waves + wave;
   if (row >= n_rows) return;
-  const u32 m_even = (m + 1) & ~1u;
-  u32* hidx = reinterpret_cast<u32*>(alias_smem) + (u64)wave * m_even * 3;
-  u64* hE = reinterpret_cast<u64*>(hidx + m_even);  // heavy j's E at hE[j]
+  // per wavefront: m4 heavy indices (u16: rows are at most kMaxTopics long) and m4 u64 prefixes
+  const u32 m4 = (m + 3) & ~3u;
+  u16* hidx = reinterpret_cast<u16*>(alias_smem + (u64)wave * m4 * 10);
+  u64* hE = reinterpret_cast<u64*>(hidx + m4);  // heavy j's E at hE[j]
   u64* lDr = hE + (m - 1);                           // light b's D at lDr[-b]
 
   f32* x = rows + (u64)row * m;
@@ -184,7 +185,7 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
     u64 incD = wave_scan_u64(def);
     if (heavy) {
       u32 pos = nh + (u32)__popcll(hb & lt);
-      hidx[pos] = k;
+      hidx[pos] = (u16)k;
       hE[pos] = sE + incS;
     }
     if (light) {
