@@ -114,6 +114,9 @@ _SIGS = {
     "gpulda_stage_map": (C.c_int, [u32p, C.c_uint32, C.c_uint32, C.c_uint32, u32p]),
     "gpulda_stage_alias": (C.c_int, [f32p, C.c_uint32, C.c_uint32, u64p, C.c_float, f32p, f32p, f32p, u32p]),
     "gpulda_stage_alias_draw": (C.c_int, [f32p, u32p, C.c_uint32, u32p, u32p, C.c_uint32, u32p]),
+    "gpulda_stage_pick": (C.c_int, [u32p, C.c_uint32, C.c_uint32, f32p, C.c_float, f32p, f32p, C.c_uint32, C.c_int,
+                                    u32p]),
+    "gpulda_stage_doc_offsets": (C.c_int, [u32p, C.c_uint32, u64p]),
     "gpulda_stage_poisson": (C.c_int, [u32p, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, u32p]),
     "gpulda_stage_phi_draw": (C.c_int, [u32p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, C.c_uint64, f32p, u64p]),
     "gpulda_stage_sweep": (C.c_int, [u32p, u32p, u64p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,

@@ -1867,6 +1867,47 @@ extern "C" int gpulda_stage_alias_draw(const float* prob, const uint32_t* alias,
   return 0;
 }
 
+extern "C" int gpulda_stage_pick(const uint32_t* table, uint32_t cap, uint32_t K, const float* phi_row,
+                                 float sigma_a, const float* u_decide, const float* u_target, uint32_t count,
+                                 int use_regs, uint32_t* slot_out) {
+  if (cap == 0 || cap % kMapLine) return fail("map capacity must be a positive multiple of 64");
+  if (K == 0 || K > kMaxTopics) return fail("K = %u out of range", K);
+  if ((u64)cap * 4 > (u64)max_lds_per_block()) return fail("map capacity exceeds LDS");
+  for (u32 s = 0; s < cap; ++s)
+    if (map_count(table[s]) != 0 && map_key(table[s]) >= K) return fail("slot %u names topic %u >= K", s, map_key(table[s]));
+  if (use_regs && cap / kMapLine > 8) return fail("register variants hold at most 8 lines");
+  Scratch sc;
+  u32 *dt, *dout;
+  f32 *drow, *dux, *duw;
+  CK(sc.upload(&dt, table, cap));
+  CK(sc.upload(&drow, phi_row, K));
+  CK(sc.upload(&dux, u_decide, count));
+  CK(sc.upload(&duw, u_target, count));
+  CK(sc.alloc(&dout, count));
+  if (count) {
+    k_stage_pick<<<count, 64, (u64)cap * 4>>>(dt, cap, drow, sigma_a, dux, duw, count, use_regs ? 1u : 0u, dout);
+    HCK(hipGetLastError());
+    HCK(hipMemcpy(slot_out, dout, (u64)count * 4, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+// Offsets of the documents of a slice as the sweeps read them (compute_d_idx, topics.cu:5-32):
+// out[d] = first token of document d, out[n_docs] = total tokens.
+extern "C" int gpulda_stage_doc_offsets(const uint32_t* d_len, uint32_t n_docs, uint64_t* out) {
+  u64 N = 0;
+  for (u32 i = 0; i < n_docs; ++i) N += d_len[i];
+  std::vector<u32> w(std::max<u64>(N, 1), 0u);
+  Segment* sg = new Segment();
+  struct Guard {
+    Segment* s;
+    ~Guard() { free_segment(s); }
+  } guard{sg};
+  CK(build_segment(sg, w.data(), d_len, n_docs, N, 1, 1));
+  HCK(hipMemcpy(out, sg->doc_off, ((u64)n_docs + 1) * sizeof(u64), hipMemcpyDeviceToHost));
+  return 0;
+}
+
 extern "C" int gpulda_stage_poisson(const uint32_t* n, uint32_t count, float beta, uint32_t c1, uint32_t c2,
                                     uint32_t tag, uint64_t seed, uint32_t* out) {
   Scratch sc;

@@ -441,7 +441,7 @@ __device__ __forceinline__ u32 column_pick(const u32* T, u32 lines, u32 ls, cons
 // Sparse/dense pick with the NL lines of the map held in registers (one variant per line count):
 // all phi gathers of the token are issued before any arithmetic on them.
 template <int NL>
-__device__ __forceinline__ Pick draw_topic_regs(const u32* T, const f32* __restrict__ prow, f32 sa, u32 rx, u32 rw,
+__device__ __forceinline__ Pick draw_topic_regs(const u32* T, const f32* __restrict__ prow, f32 sa, f32 ux, f32 uw,
                                                 u32 dense) {
   const u32 lane = lane_id();
   u32 e[NL];
@@ -463,8 +463,8 @@ __device__ __forceinline__ Pick draw_topic_regs(const u32* T, const f32* __restr
   pk.topic = dense;
   pk.slot = kNoSlot;
   pk.entry = 0;
-  if (!(u24(rx) * (sa + sb) > sa)) return pk;
-  const f32 tgt = u24(rw) * sb;
+  if (!(ux * (sa + sb) > sa)) return pk;
+  const f32 tgt = uw * sb;
   f32 E;
   const u32 ls = pick_lane(A, P, tgt, E);
   // every lane scans its own column in registers with the canonical network (NL <= 16 lines lie
@@ -503,7 +503,7 @@ __device__ __forceinline__ Pick draw_topic_regs(const u32* T, const f32* __restr
 // Sparse/dense pick for any number of lines: one pass for the column sums, then for a sparse draw
 // the in-column search over column l* read transposed.
 __device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa,
-                                                u32 rx, u32 rw, u32 dense) {
+                                                f32 ux, f32 uw, u32 dense) {
   const u32 lane = lane_id();
   f32 G0 = 0.0f, G1 = 0.0f, G2 = 0.0f, G3 = 0.0f;
   // eight lines per step, so eight gathers are in flight at once; full steps read without a
@@ -542,8 +542,8 @@ __device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const f
   pk.topic = dense;
   pk.slot = kNoSlot;
   pk.entry = 0;
-  if (!(u24(rx) * (sa + sb) > sa)) return pk;
-  const f32 tgt = u24(rw) * sb;
+  if (!(ux * (sa + sb) > sa)) return pk;
+  const f32 tgt = uw * sb;
   f32 E;
   const u32 ls = pick_lane(A, P, tgt, E);
   pk.slot = column_pick(T, lines, ls, prow, E, tgt) * 64 + ls;
@@ -557,20 +557,20 @@ __device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const f
 // use the line-loop form.  All variants evaluate the same operations in the same order: a
 // variant for fewer lines drops only lines that would hold no mass.
 template <int LMAX>
-__device__ __forceinline__ Pick draw_topic(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa, u32 rx,
-                                           u32 rw, u32 dense) {
+__device__ __forceinline__ Pick draw_topic(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa, f32 ux,
+                                           f32 uw, u32 dense) {
   switch (lines) {  // a map always has at least one line
-    case 1: if (LMAX >= 1) return draw_topic_regs<1>(T, prow, sa, rx, rw, dense); break;
-    case 2: if (LMAX >= 2) return draw_topic_regs<2>(T, prow, sa, rx, rw, dense); break;
-    case 3: if (LMAX >= 4) return draw_topic_regs<3>(T, prow, sa, rx, rw, dense); break;
-    case 4: if (LMAX >= 4) return draw_topic_regs<4>(T, prow, sa, rx, rw, dense); break;
-    case 5: if (LMAX >= 8) return draw_topic_regs<5>(T, prow, sa, rx, rw, dense); break;
-    case 6: if (LMAX >= 8) return draw_topic_regs<6>(T, prow, sa, rx, rw, dense); break;
-    case 7: if (LMAX >= 8) return draw_topic_regs<7>(T, prow, sa, rx, rw, dense); break;
-    case 8: if (LMAX >= 8) return draw_topic_regs<8>(T, prow, sa, rx, rw, dense); break;
+    case 1: if (LMAX >= 1) return draw_topic_regs<1>(T, prow, sa, ux, uw, dense); break;
+    case 2: if (LMAX >= 2) return draw_topic_regs<2>(T, prow, sa, ux, uw, dense); break;
+    case 3: if (LMAX >= 4) return draw_topic_regs<3>(T, prow, sa, ux, uw, dense); break;
+    case 4: if (LMAX >= 4) return draw_topic_regs<4>(T, prow, sa, ux, uw, dense); break;
+    case 5: if (LMAX >= 8) return draw_topic_regs<5>(T, prow, sa, ux, uw, dense); break;
+    case 6: if (LMAX >= 8) return draw_topic_regs<6>(T, prow, sa, ux, uw, dense); break;
+    case 7: if (LMAX >= 8) return draw_topic_regs<7>(T, prow, sa, ux, uw, dense); break;
+    case 8: if (LMAX >= 8) return draw_topic_regs<8>(T, prow, sa, ux, uw, dense); break;
     default: break;
   }
-  return draw_topic_loop(T, lines, prow, sa, rx, rw, dense);
+  return draw_topic_loop(T, lines, prow, sa, ux, uw, dense);
 }
 
 // Adds the drawn topic to the map: a sparse pick increments the slot it came from; otherwise the
@@ -629,8 +629,8 @@ __device__ __forceinline__ void sweep_pass(const SweepParams& P, u32* T, DocStat
       if (sa > 0.0f) {
         wave_sync();
         visited += ds.occupied;
-        pk = draw_topic<LMAX>(T, uniform_u32(ds.cap / kMapLine), P.phi + (u64)wt * K, sa, readlane_u32(r.x, t),
-                              readlane_u32(r.w, t), readlane_u32(dense_l, t));
+        pk = draw_topic<LMAX>(T, uniform_u32(ds.cap / kMapLine), P.phi + (u64)wt * K, sa, u24(readlane_u32(r.x, t)),
+                              u24(readlane_u32(r.w, t)), readlane_u32(dense_l, t));
       }
       add_topic(P, T, ds, pk, c0, t, znew_lane, zz);
       const u32 znew = pk.topic;
@@ -1095,6 +1095,23 @@ __global__ void k_stage_map(const u32* ops, u32 n_ops, u32 cap, u32 K, u32* out)
   }
   wave_sync();
   for (u32 s = lane; s < cap; s += 64) out[s] = map_smem[s];
+}
+
+// One wavefront per draw: the map `table` (cap slots, any content) is copied to LDS and draw i is
+// made with the uniforms ux[i], uw[i] by the sweep's own device functions -- the register
+// variants when use_regs (maps of up to eight lines), else the line-loop form.  out[i] is the
+// picked slot, or kNoSlot for a dense draw.
+__global__ __launch_bounds__(64) void k_stage_pick(const u32* table, u32 cap, const f32* prow, f32 sa, const f32* ux,
+                                                   const f32* uw, u32 count, u32 use_regs, u32* out) {
+  extern __shared__ u32 pick_smem[];
+  const u32 i = blockIdx.x;
+  if (i >= count) return;
+  for (u32 s = lane_id(); s < cap; s += 64) pick_smem[s] = table[s];
+  wave_sync();
+  const u32 lines = uniform_u32(cap / kMapLine);
+  const Pick pk = use_regs ? draw_topic<8>(pick_smem, lines, prow, sa, ux[i], uw[i], kNoSlot)
+                           : draw_topic<0>(pick_smem, lines, prow, sa, ux[i], uw[i], kNoSlot);
+  if (lane_id() == 0) out[i] = pk.slot;
 }
 
 __global__ void k_stage_poisson(const u32* nval, u32 count, f32 beta, const AliasSlot* tables, u32 c1, u32 c2,

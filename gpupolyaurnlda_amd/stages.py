@@ -79,6 +79,32 @@ def alias_draw(prob, al, r_slot, r_keep) -> np.ndarray:
     return out
 
 
+def pick(table, K: int, phi_row, sigma_a: float, u_decide, u_target, use_regs: bool = True) -> np.ndarray:
+    """The sweep's sparse/dense decision and sparse pick on a given map: the picked slot for each
+    (u_decide, u_target) pair, or -1 for a dense draw."""
+    table = _u32(table)
+    phi_row = _f32(phi_row)
+    ux = _f32(np.atleast_1d(u_decide))
+    uw = _f32(np.atleast_1d(u_target))
+    assert len(ux) == len(uw) and len(phi_row) == K
+    out = np.empty(len(ux), dtype=np.uint32)
+    check(lib().gpulda_stage_pick(ptr(table, C.c_uint32), len(table), K, ptr(phi_row, C.c_float), C.c_float(sigma_a),
+                                  ptr(ux, C.c_float), ptr(uw, C.c_float), len(ux), 1 if use_regs else 0,
+                                  ptr(out, C.c_uint32)), "stage_pick")
+    res = out.astype(np.int64)
+    res[out == 0xFFFFFFFF] = -1
+    return res
+
+
+def doc_offsets(d_len) -> np.ndarray:
+    """First token of each document (and the total) as the sweeps read them."""
+    d_len = _u32(d_len)
+    out = np.empty(len(d_len) + 1, dtype=np.uint64)
+    check(lib().gpulda_stage_doc_offsets(ptr(d_len, C.c_uint32), len(d_len), ptr(out, C.c_uint64)),
+          "stage_doc_offsets")
+    return out
+
+
 def poisson(n, beta: float, c1: int = 0, c2: int = 0, tag: int = 3, seed: int = 0) -> np.ndarray:
     """Poisson(n_i + beta) draws with Philox counters (i, c1, c2, tag)."""
     n = _u32(n)

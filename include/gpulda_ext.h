@@ -128,8 +128,22 @@ int gpulda_stage_map(const uint32_t* ops, uint32_t n_ops, uint32_t cap, uint32_t
 /* Row normalisation (by divisor[k] when divisor != NULL), row sums and alias tables. */
 int gpulda_stage_alias(const float* rows, uint32_t n_rows, uint32_t m, const uint64_t* divisor, float alpha,
                        float* rows_out, float* sums_out, float* prob_out, uint32_t* alias_out);
+/* draw_alias (src/cu/topics.cuh:63-83): slot floor(r_slot * m / 2^32), kept when
+ * u(r_keep) < prob[slot], else alias[slot]. */
 int gpulda_stage_alias_draw(const float* prob, const uint32_t* alias, uint32_t m, const uint32_t* r_slot,
                             const uint32_t* r_keep, uint32_t count, uint32_t* out);
+/* Sparse/dense decision and sparse pick of the z-sweep on a given doc-topic map (cap slots of
+ * entries key << 16 | count) for a word whose phi row is phi_row (K values) and sigma_a > 0:
+ * draw i is sparse when u_decide[i] * (sigma_a + sigma_b) > sigma_a and then picks the slot whose
+ * interval holds u_target[i] * sigma_b (compute_product_cumsum + draw_wary_search,
+ * src/cu/topics.cuh:163-192, 88-137).  slot_out[i] = that slot, or 0xFFFFFFFF for a dense draw.
+ * use_regs selects the register variants (cap <= 512), else the line-loop form. */
+int gpulda_stage_pick(const uint32_t* table, uint32_t cap, uint32_t K, const float* phi_row, float sigma_a,
+                      const float* u_decide, const float* u_target, uint32_t count, int use_regs,
+                      uint32_t* slot_out);
+/* Document offsets as the sweeps read them (compute_d_idx, src/cu/topics.cu:5-32): out[d] = first
+ * token of document d, out[n_docs] = total (n_docs + 1 values). */
+int gpulda_stage_doc_offsets(const uint32_t* d_len, uint32_t n_docs, uint64_t* out);
 /* Poisson draws with counters (i, c1, c2, tag) for i = 0..count-1, tables built for beta. */
 int gpulda_stage_poisson(const uint32_t* n, uint32_t count, float beta, uint32_t c1, uint32_t c2, uint32_t tag,
                          uint64_t seed, uint32_t* out);

@@ -244,10 +244,14 @@ void orc_alias_build(const float* w, float total, u32 n, OrcAlias* out) {
   free(a); free(heavy); free(E); free(D);
 }
 
+/* draw_alias (topics.cuh:63-83) with the slot and the keep test on two independent words: slot =
+ * floor(r_slot * n / 2^32), kept when u(r_keep) < prob[slot], else its alias. */
 static u32 draw_alias(const OrcAlias* row, u32 n, u32 r_slot, u32 r_keep) {
   u32 slot = below(r_slot, n);
   return unif24(r_keep) < row[slot].prob ? slot : row[slot].alias;
 }
+
+u32 orc_alias_draw(const OrcAlias* row, u32 n, u32 r_slot, u32 r_keep) { return draw_alias(row, n, r_slot, r_keep); }
 
 /* Poisson tables.  Restates Poisson::Poisson + build_poisson_prob, poisson.cu:9-34: table n is */
 /* the PMF of Poisson(n + beta) over 0..199 (computed in double, stored as f32).               */
@@ -475,13 +479,80 @@ static OrcDoc doc_open(const u32* z, u32 L, u32 K, u32* scratch) {
   return d;
 }
 
+/* Sparse/dense decision and sparse pick for one token over a map of `lines` 64-slot lines (slot
+ * j*64+l is line j, lane l) and the Phi row prow of the token's word (sigma_a = sa > 0):
+ *   m[j][l] = count * prow[key] (0 when empty), written to mass[];
+ *   G_g[l]  = m[g][l] + m[g+4][l] + ... left to right (line group g = j % 4);
+ *   A[l]    = ((G_0 + G_1) + G_2) + G_3;  P = scan64(A);  sigma_b = P[63];
+ *   sparse iff ux * (sa + sigma_b) > sa; then t = uw * sigma_b and the pick is the slot whose
+ *   interval holds t when the slots are laid end to end column by column (lane 0's lines in line
+ *   order, then lane 1's, ...): draw_wary_search (topics.cuh:88-137) restated two-level.
+ *   lane l* = first l with A[l] > 0 and P[l] > t (else the last with A[l] > 0); E = P[l*-1]
+ *   (0 for l* = 0); Q = scan64 of column l*'s masses in line order (64 lines per scan, carry =
+ *   running total of earlier scans); the line is the first with m > 0 and E + (carry + Q_j) > t
+ *   (else the last line with m > 0).
+ * Returns the picked slot, or -1 for a dense draw (or a map without mass).  *visited, when not
+ * NULL, is increased by the number of occupied slots the sum read. */
+long orc_sparse_pick(const u32* T, u32 lines, const float* prow, float sa, float ux, float uw, float* mass,
+                     u64* visited) {
+  float G[4][64];
+  for (int g = 0; g < 4; ++g)
+    for (int l = 0; l < 64; ++l) G[g][l] = 0.0f;
+  for (u32 j = 0; j < lines; ++j) {
+    for (u32 l = 0; l < 64; ++l) {
+      u32 e = T[j * 64 + l];
+      if (visited) *visited += !IS_EMPTY(e);
+      float m = IS_EMPTY(e) ? 0.0f : (float)(e & 0xFFFFu) * prow[e >> 16];
+      mass[j * 64 + l] = m;
+      G[j % 4][l] = G[j % 4][l] + m;
+    }
+  }
+  float A[64], P[64];
+  for (int l = 0; l < 64; ++l) {
+    A[l] = ((G[0][l] + G[1][l]) + G[2][l]) + G[3][l];
+    P[l] = A[l];
+  }
+  orc_scan64(P);
+  float sb = P[63];
+  if (!(ux * (sa + sb) > sa)) return -1;
+  float t = uw * sb;
+  long ls = -1, lpos = -1;
+  for (int l = 0; l < 64; ++l) {
+    if (A[l] > 0.0f) {
+      lpos = l;
+      if (ls < 0 && P[l] > t) ls = l;
+    }
+  }
+  if (ls < 0) ls = lpos;
+  if (ls < 0) return -1;
+  /* in column l*: canonical scans of its masses in line order, 64 lines at a time */
+  float E = ls > 0 ? P[ls - 1] : 0.0f;
+  float carry = 0.0f;
+  long hit = -1, last = -1;
+  for (u32 c0 = 0; c0 < lines && hit < 0; c0 += 64) {
+    float Q[64];
+    for (u32 k = 0; k < 64; ++k) Q[k] = c0 + k < lines ? mass[(c0 + k) * 64 + (u32)ls] : 0.0f;
+    float mc[64];
+    memcpy(mc, Q, sizeof(Q));
+    orc_scan64(Q);
+    for (u32 k = 0; k < 64 && c0 + k < lines; ++k) {
+      if (mc[k] > 0.0f) {
+        if (hit < 0 && E + (carry + Q[k]) > t) hit = (long)(c0 + k);
+        last = (long)(c0 + k);
+      }
+    }
+    carry = carry + Q[63];
+  }
+  long js = hit >= 0 ? hit : last;
+  return js * 64 + ls;
+}
+
 /* One pass over the document's tokens with counters (token, doc, iter, tag); returns the number
  * of map entries the sparse sums visited. */
 static u64 doc_pass(OrcDoc* d, u32* z, const u32* w, u32 doc, u32 iter, u32 tag, u64 seed, const float* phi,
                     const float* sigma_a, const OrcAlias* alias) {
   const u32 K = d->K;
   u64 visited = 0;
-  float P[64];
   for (u32 i = 0; i < d->L; ++i) {
     u32 wi = w[i], zi = z[i];
     u32 r[4];
@@ -492,66 +563,10 @@ static u64 doc_pass(OrcDoc* d, u32* z, const u32* w, u32 doc, u32 iter, u32 tag,
     float sa = sigma_a[wi];
     u32 znew = zi;
     if (sa > 0.0f) {
-      const float* prow = phi + (u64)wi * K;
-      const u32 lines = d->cap / 64;
-      /* column sums in four interleaved line groups (line j in group j % 4, lines in order) */
-      float G[4][64];
-      for (int g = 0; g < 4; ++g)
-        for (int l = 0; l < 64; ++l) G[g][l] = 0.0f;
-      for (u32 j = 0; j < lines; ++j) {
-        for (u32 l = 0; l < 64; ++l) {
-          u32 e = d->T[j * 64 + l];
-          visited += !IS_EMPTY(e);
-          float m = IS_EMPTY(e) ? 0.0f : (float)(e & 0xFFFFu) * prow[e >> 16];
-          d->mass[j * 64 + l] = m;
-          G[j % 4][l] = G[j % 4][l] + m;
-        }
-      }
-      float A[64];
-      for (int l = 0; l < 64; ++l) {
-        A[l] = ((G[0][l] + G[1][l]) + G[2][l]) + G[3][l];
-        P[l] = A[l];
-      }
-      orc_scan64(P);
-      float sb = P[63];
-      float u1 = unif24(r[0]);
-      if (u1 * (sa + sb) > sa) {
-        /* draw_wary_search restated two-level: the lane (column) whose prefix holds the target,
-         * then the slot inside that column */
-        float t = unif24(r[3]) * sb;
-        long ls = -1, lpos = -1;
-        for (int l = 0; l < 64; ++l) {
-          if (A[l] > 0.0f) {
-            lpos = l;
-            if (ls < 0 && P[l] > t) ls = l;
-          }
-        }
-        if (ls < 0) ls = lpos;
-        if (ls >= 0) {
-          /* in column l*: canonical scans of its masses in line order, 64 lines at a time */
-          float E = ls > 0 ? P[ls - 1] : 0.0f;
-          float carry = 0.0f;
-          long hit = -1, last = -1;
-          for (u32 c0 = 0; c0 < lines && hit < 0; c0 += 64) {
-            float Q[64];
-            for (u32 k = 0; k < 64; ++k) Q[k] = c0 + k < lines ? d->mass[(c0 + k) * 64 + (u32)ls] : 0.0f;
-            float mc[64];
-            memcpy(mc, Q, sizeof(Q));
-            orc_scan64(Q);
-            for (u32 k = 0; k < 64 && c0 + k < lines; ++k) {
-              if (mc[k] > 0.0f) {
-                if (hit < 0 && E + (carry + Q[k]) > t) hit = (long)(c0 + k);
-                last = (long)(c0 + k);
-              }
-            }
-            carry = carry + Q[63];
-          }
-          long js = hit >= 0 ? hit : last;
-          znew = d->T[(u32)js * 64 + (u32)ls] >> 16;
-        }
-      } else {
-        znew = draw_alias(alias + (u64)wi * K, K, r[1], r[2]);
-      }
+      long s = orc_sparse_pick(d->T, d->cap / 64, phi + (u64)wi * K, sa, unif24(r[0]), unif24(r[3]), d->mass,
+                               &visited);
+      if (s >= 0) znew = d->T[s] >> 16;
+      else znew = draw_alias(alias + (u64)wi * K, K, r[1], r[2]);
     }
     if (orc_map_find(d->T, d->cap, d->hm, znew) < 0) {
       if (orc_map_capacity(d->keys + 1, K) > d->cap) {

@@ -48,6 +48,8 @@ _SIGS = {
     "orc_count": (None, [u32p, u32p, C.c_uint64, C.c_uint32, u32p, C.c_uint64]),
     "orc_threads": (C.c_int, []),
     "orc_loglik": (C.c_double, [u32p, C.c_uint32, C.c_uint32, u32p, u64p, C.c_uint32, C.c_double, C.c_double]),
+    "orc_sparse_pick": (C.c_long, [u32p, C.c_uint32, f32p, C.c_float, C.c_float, C.c_float, f32p, u64p]),
+    "orc_alias_draw": (C.c_uint32, [C.POINTER(OrcAlias), C.c_uint32, C.c_uint32, C.c_uint32]),
 }
 
 _lib = None
@@ -118,6 +120,28 @@ def poisson(n, beta, c1=0, c2=0, tag=3, seed=0, tables=None):
         tables = poisson_tables(beta)[2]
     n = np.asarray(n, dtype=np.uint32)
     return np.array([lib().orc_poisson(int(v), C.c_float(beta), tables, i, c1, c2, tag, seed) for i, v in enumerate(n)],
+                    dtype=np.uint32)
+
+
+def sparse_pick(T, prow, sa, ux, uw):
+    """Sparse/dense decision and sparse pick of one token over map T (len a multiple of 64) and the
+    Phi row of its word: the picked slot for each (ux, uw) pair, or -1 for a dense draw."""
+    T = np.ascontiguousarray(T, dtype=np.uint32)
+    prow = np.ascontiguousarray(prow, dtype=np.float32)
+    ux, uw = np.broadcast_arrays(np.atleast_1d(np.asarray(ux, dtype=np.float32)),
+                                 np.atleast_1d(np.asarray(uw, dtype=np.float32)))
+    assert len(T) % 64 == 0
+    mass = np.empty(len(T), dtype=np.float32)
+    L = lib()
+    return np.array([L.orc_sparse_pick(_p(T, C.c_uint32), len(T) // 64, _p(prow, C.c_float), C.c_float(sa),
+                                       C.c_float(float(a)), C.c_float(float(b)), _p(mass, C.c_float), None)
+                     for a, b in zip(ux, uw)], dtype=np.int64)
+
+
+def alias_draw(prob, al, r_slot, r_keep):
+    packed = _AliasPacked(np.asarray(prob, dtype=np.float32), np.asarray(al, dtype=np.uint32))
+    n = len(np.asarray(prob).reshape(-1))
+    return np.array([lib().orc_alias_draw(packed.arr, n, int(a), int(b)) for a, b in zip(r_slot, r_keep)],
                     dtype=np.uint32)
 
 

@@ -1,0 +1,134 @@
+"""Every workload shape the sampler is measured on, checked against the CPU oracle on the GPU.
+
+  medium  V=50k Zipf(1.07), K=256, D=200k lognormal(100, 0.8) <= 2000 -- the whole corpus: two
+          iterations bit-exact (z, n_wk, phi), then a 200-iteration log-likelihood trace
+  large   V=100k Zipf(1.07), K=1024, lognormal(200, 1.0) <= 4096 -- a 20,000-document slice under
+          the persistent schedule the benchmark runs: two iterations, z, n_wk and the full V x K
+          phi bit-exact
+  batched V=100k, K=1024, power-law lengths on [4, 20000] -- a 20,000-document slice under the
+          persistent + block-per-document schedule, eagerly and as a replayed hipGraph, against
+          the oracle at every iteration
+  serving fold-in against a K=1024, V=100k model: 50 sweeps per request, lognormal(300, 1.2)
+          lengths clipped to [10, 5000]; the server's results, mixed and alone, equal the oracle's
+          fold-in of the same request ids
+
+(The tiny shape runs 100 iterations through the five-call ABI in test_gpu_parity.py.)
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-configs] {msg}", flush=True)
+
+
+def _model(orc, spec, c, seed):
+    return orc.Model(c["w"], c["doc_off"], c["z"], spec.V, spec.K, spec.alpha, spec.beta, seed=seed, counts="z",
+                     threads=16)
+
+
+def _same_state(rc, model, it, phi=True):
+    assert np.array_equal(rc.z(), model.z), f"z differs at iteration {it}"
+    assert np.array_equal(rc.counts(), model.n), f"n_wk differs at iteration {it}"
+    if phi:
+        assert np.array_equal(rc.phi().view(np.uint32), model.phi.view(np.uint32)), f"phi differs at iteration {it}"
+
+
+def test_medium_full_corpus_bit_exact_and_ll_trace(orc):
+    from gpupolyaurnlda_amd import corpus
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+    spec = corpus.MEDIUM
+    c = corpus.generate(spec)
+    N = len(c["w"])
+    say(f"medium full corpus: D={spec.D} N={N}")
+    assert spec.D == 200000 and N > 20_000_000
+    model = _model(orc, spec, c, seed=1)
+    with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=1, z=c["z"]) as rc:
+        rc.schedule(1, 32)
+        lls = [rc.loglik()]
+        for it in (1, 2):
+            rc.iterate_untimed(1)
+            model.iterate()
+            _same_state(rc, model, it)
+            lls.append(rc.loglik())
+            say(f"medium iteration {it} bit-exact")
+        ll_c = orc.loglik(model.n, model.z, c["doc_off"], float(np.float32(spec.alpha)), float(np.float32(spec.beta)))
+        assert abs(lls[-1] - ll_c) <= 1e-9 * abs(ll_c)
+        for it in range(3, 201):
+            rc.iterate_untimed(1)
+            lls.append(rc.loglik())
+    tail = np.array(lls[-10:])
+    say(f"medium LL: start {lls[0]:.6e} it2 {lls[2]:.6e} it200 {lls[-1]:.6e}; last-10 spread "
+        f"{np.abs(tail - tail.mean()).max() / abs(tail.mean()):.2e}")
+    assert lls[-1] > lls[0]
+    assert np.abs(tail - tail.mean()).max() <= 0.005 * abs(tail.mean())
+
+
+def test_large_slice_bit_exact_full_phi(orc):
+    from gpupolyaurnlda_amd import corpus
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+    spec = corpus.LARGE.scaled(20000)
+    c = corpus.generate(spec)
+    say(f"large slice: D={spec.D} N={len(c['w'])} V={spec.V} K={spec.K}")
+    model = _model(orc, spec, c, seed=2)
+    with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=2, z=c["z"]) as rc:
+        rc.schedule(1, 32)  # the benchmark's schedule: one persistent launch, 32 wavefronts per CU
+        for it in (1, 2):
+            rc.iterate_untimed(1)
+            model.iterate()
+            _same_state(rc, model, it)
+            p, a = rc.alias()
+            assert np.array_equal(p.view(np.uint32), model.prob.view(np.uint32)) and np.array_equal(a, model.al)
+            assert np.array_equal(rc.sigma_a().view(np.uint32), model.sigma_a.view(np.uint32))
+            say(f"large slice iteration {it}: z, n_wk, phi, sigma_a, alias bit-exact")
+        assert rc.visited() == model.visited
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batched_slice_block_schedule_vs_oracle(orc, graph):
+    from gpupolyaurnlda_amd import corpus
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+    spec = corpus.BATCHED.scaled(20000)
+    c = corpus.generate(spec)
+    longest = int(c["d_len"].max())
+    say(f"batched slice: D={spec.D} N={len(c['w'])} longest={longest} graph={graph}")
+    block_min = 1000
+    assert (c["d_len"] >= block_min).sum() > 10  # the block-per-document kernel gets work
+    model = _model(orc, spec, c, seed=3)
+    with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=3, z=c["z"]) as rc:
+        rc.schedule(2, 16, block_min)
+        rc.use_graph(graph)
+        for it in (1, 2, 3):  # graph replays with eager reads in between
+            rc.iterate_untimed(1)
+            model.iterate()
+            _same_state(rc, model, it, phi=(it == 3))
+            say(f"batched slice iteration {it} bit-exact (graph={graph})")
+
+
+def test_serving_large_model_vs_oracle(orc):
+    from gpupolyaurnlda_amd import corpus, stages
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+    from gpupolyaurnlda_amd.serve import FoldInServer, run_serial, run_stream
+    spec = corpus.LARGE.scaled(20000)
+    c = corpus.generate(spec)
+    with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=2, z=c["z"]) as rc:
+        rc.iterate_untimed(3)
+        phi = rc.phi()
+        sig = rc.sigma_a()
+        prob, al = rc.alias()
+        d_len, arr, w = corpus.requests(phi, 30, median=300.0, sigma=1.2, lo=10, hi=5000, rate=2000.0, seed=4)
+        say(f"serving: V={spec.V} K={spec.K}, 30 requests, lengths {int(d_len.min())}..{int(d_len.max())}, 50 sweeps")
+        with FoldInServer(hbm_budget_bytes=16 << 30, sweeps=50, waves_per_cu=2, max_slots=1024) as srv:
+            mixed = run_stream(srv, d_len, arr, w)
+            alone = run_serial(srv, d_len, w)
+    off = np.concatenate([[0], np.cumsum(d_len)]).astype(np.uint64)
+    ids = np.arange(len(d_len), dtype=np.uint32)
+    zc, vc = orc.foldin(w, off, ids, 50, 2, spec.K, phi, sig, prob, al, threads=16)
+    zs, vs = stages.foldin(w, off, ids, 50, 2, spec.K, phi, sig, prob, al)
+    assert vs == vc and np.array_equal(zs, zc)
+    want = [zc[off[i]:off[i + 1]] for i in range(len(d_len))]
+    bad_mixed = [i for i in range(len(d_len)) if not np.array_equal(mixed["results"][i], want[i])]
+    bad_alone = [i for i in range(len(d_len)) if not np.array_equal(alone["results"][i], want[i])]
+    assert not bad_mixed and not bad_alone, f"mixed wrong {bad_mixed}, alone wrong {bad_alone}"

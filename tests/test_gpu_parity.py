@@ -481,3 +481,15 @@ def test_medium_config_parity_and_ll_trace(orc):
     assert lls[-1] > lls[0]
     tail = np.array(lls[-10:])
     assert np.abs(tail - tail.mean()).max() <= 0.005 * abs(tail.mean())
+
+
+def test_ppu_row_shapes_match_oracle(S, orc):
+    """The reference's Polya-urn row-shape inputs (test_polyaurn.cu:76-118) through the device
+    Poisson draw and normalisation, bit for bit against the oracle over 50 iterations."""
+    n = np.ascontiguousarray(np.array([[1, 10, 100], [1, 1, 1], [1000, 1000, 1000]], np.uint32).T)
+    for it in range(1, 51):
+        pt, ts = S.phi_draw(n, 0.01, it, seed=0)
+        pt2, ts2 = orc.phi_draw(n, 0.01, it, 0)
+        assert np.array_equal(pt, pt2) and np.array_equal(ts, ts2), it
+        rows, _, _, _ = S.alias(pt, divisor=ts, alpha=0.1)
+        assert np.array_equal(rows.view(np.uint32), orc.phi_finish(pt2, ts2, 0.1)[0].view(np.uint32)), it

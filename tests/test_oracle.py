@@ -448,3 +448,44 @@ def test_foldin_recovers_topic(orc):
     off = np.concatenate([[0], np.cumsum([len(r) for r in reqs])]).astype(np.uint64)
     z, _ = orc.foldin(np.concatenate(reqs), off, np.arange(20, dtype=np.uint32), 20, 3, K, phi, sig, prob, al)
     assert (z == 2).mean() > 0.9
+
+
+PPU_COUNTS_REF = np.array([[1, 10, 100], [1, 1, 1], [1000, 1000, 1000]], np.uint32)  # [topic][word]
+
+
+def ppu_draws(orc, iters):
+    """Normalised Polya-urn draws of test_polyaurn.cu:76-118's counts over `iters` iterations."""
+    n = np.ascontiguousarray(PPU_COUNTS_REF.T)  # word-major here
+    tables = orc.poisson_tables(0.01)[2]
+    out = []
+    for it in range(1, iters + 1):
+        pt, ts = orc.phi_draw(n, 0.01, it, 0, tables)
+        out.append(orc.phi_finish(pt, ts, 0.1)[0])
+    return np.array(out)  # (iters, V, K)
+
+
+def test_ppu_row_shapes_reference(orc):
+    """test_polyaurn.cu:76-118 restated: topic 0 with counts (1, 10, 100) over three words comes out
+    near (0.01, 0.09, 0.9), topic 1 with (1, 1, 1) as ratios of small integers, topic 2 with
+    (1000, 1000, 1000) near 1/3 each.  The reference asserts single seed-dependent draws (its
+    0/0.5/1 set for topic 1 rejects valid draws such as 1/3); here every draw is checked for exactness
+    and the averages over 400 independent iterations against the Poisson means."""
+    phis = ppu_draws(orc, 400).astype(np.float64)
+    pt_all = []
+    for it in (1, 2, 3):
+        pt, ts = orc.phi_draw(np.ascontiguousarray(PPU_COUNTS_REF.T), 0.01, it, 0)
+        assert np.all(pt == np.floor(pt))
+        pt_all.append((pt, ts))
+    colsum = phis.sum(axis=1)  # (iters, K): each topic column sums to 1 (or is all zero)
+    assert np.all((np.abs(colsum - 1) < 1e-6) | (colsum == 0))
+    lam = PPU_COUNTS_REF.T.astype(np.float64) + 0.01
+    mean = phis.mean(axis=0)
+    assert np.allclose(mean[:, 0], lam[:, 0] / lam[:, 0].sum(), atol=0.006)  # ~ (0.009, 0.090, 0.901)
+    p_any = 1 - np.exp(-lam[:, 1].sum())
+    assert np.allclose(mean[:, 1], p_any / 3, atol=0.03)
+    assert np.all(np.abs(phis[:, :, 2] - 1 / 3) < 0.05) and np.allclose(mean[:, 2], 1 / 3, atol=0.004)
+    # topic 1: every value is x / S for Poisson integers x
+    for pt, ts in pt_all:
+        phi = orc.phi_finish(pt, ts, 0.1)[0]
+        if ts[1]:
+            assert np.array_equal(phi[:, 1], (pt[:, 1] / np.float32(ts[1])).astype(np.float32))
