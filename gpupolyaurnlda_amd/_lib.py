@@ -87,6 +87,9 @@ _SIGS = {
     "gpulda_last_error": (C.c_char_p, []),
     "gpulda_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), u64p, C.POINTER(C.c_int)]),
     "gpulda_set_slice_cache": (C.c_int, [C.c_int]),
+    "gpulda_set_slice_cache_budget": (C.c_int, [C.c_uint64]),
+    "gpulda_slice_cache_stats": (C.c_int, [u32p, u64p, u64p]),
+    "gpulda_slice_hash": (C.c_uint64, [u32p, u32p, C.c_uint32, C.c_uint64]),
     "gpulda_iteration": (C.c_uint32, []),
     "gpulda_copy_phi": (C.c_int, [f32p]),
     "gpulda_copy_alias": (C.c_int, [f32p, u32p]),

@@ -8,10 +8,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fcntl.h>
 #include <fstream>
 #include <numeric>
 #include <sstream>
 #include <thread>
+#include <unistd.h>
 #include <unordered_map>
 
 #include "lda_core.h"
@@ -334,45 +336,24 @@ std::vector<uint32_t> counts_from_files(const TempPaths& t, uint32_t V, uint32_t
   return n;
 }
 
-uint64_t file_hash(const std::string& path) {
-  // fixed 64 MiB chunks hashed in parallel, then combined in chunk order: the value does not
-  // depend on the number of threads
-  FILE* f = fopen(path.c_str(), "rb");
-  if (!f) die("cannot open " + path + ": " + strerror(errno));
-  fseek(f, 0, SEEK_END);
-  const uint64_t bytes = (uint64_t)ftell(f);
-  fclose(f);
-  const uint64_t chunk = 64ull << 20;
-  const uint64_t n_chunks = (bytes + chunk - 1) / chunk;
-  std::vector<uint64_t> part(n_chunks, 0);
-  const unsigned n_threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
-  auto work = [&](unsigned tid) {
-    FILE* g = fopen(path.c_str(), "rb");
-    if (!g) return;
-    std::vector<unsigned char> buf(chunk);
-    for (uint64_t c = tid; c < n_chunks; c += n_threads) {
-      const uint64_t len = std::min(chunk, bytes - c * chunk);
-      fseek(g, (long)(c * chunk), SEEK_SET);
-      if (fread(buf.data(), 1, len, g) != len) {
-        part[c] = ~0ull;
-        continue;
-      }
-      uint64_t h = 0x9E3779B97F4A7C15ull ^ (c * 0xD6E8FEB86659FD93ull) ^ len;
-      uint64_t i = 0;
-      for (; i + 8 <= len; i += 8) {
-        uint64_t x;
-        memcpy(&x, buf.data() + i, 8);
-        h = (h ^ x) * 0x100000001B3ull;
-        h ^= h >> 29;
-      }
-      for (; i < len; ++i) h = (h ^ buf[i]) * 0x100000001B3ull;
-      part[c] = h;
-    }
-    fclose(g);
-  };
-  std::vector<std::thread> th;
-  for (unsigned i = 0; i < n_threads; ++i) th.emplace_back(work, i);
-  for (auto& x : th) x.join();
+namespace {
+// 64-bit hash of one chunk (position c in the sequence, len bytes).
+uint64_t chunk_hash(const unsigned char* p, uint64_t len, uint64_t c) {
+  uint64_t h = 0x9E3779B97F4A7C15ull ^ (c * 0xD6E8FEB86659FD93ull) ^ len;
+  uint64_t i = 0;
+  for (; i + 8 <= len; i += 8) {
+    uint64_t x;
+    memcpy(&x, p + i, 8);
+    h = (h ^ x) * 0x100000001B3ull;
+    h ^= h >> 29;
+  }
+  for (; i < len; ++i) h = (h ^ p[i]) * 0x100000001B3ull;
+  return h;
+}
+
+constexpr uint64_t kHashChunk = 64ull << 20;
+
+uint64_t combine_chunks(const std::vector<uint64_t>& part, uint64_t bytes) {
   uint64_t h = 0xCBF29CE484222325ull ^ bytes;
   for (uint64_t p : part) {
     h = (h ^ p) * 0x100000001B3ull;
@@ -381,16 +362,120 @@ uint64_t file_hash(const std::string& path) {
   return h;
 }
 
+unsigned hash_threads() { return std::max(1u, std::min(8u, std::thread::hardware_concurrency())); }
+}  // namespace
+
+uint64_t bytes_hash(const void* data, uint64_t bytes) {
+  // fixed 64 MiB chunks hashed in parallel, then combined in chunk order: the value does not
+  // depend on the number of threads
+  const unsigned char* p = static_cast<const unsigned char*>(data);
+  const uint64_t n_chunks = (bytes + kHashChunk - 1) / kHashChunk;
+  std::vector<uint64_t> part(n_chunks, 0);
+  const unsigned n_threads = hash_threads();
+  auto work = [&](unsigned tid) {
+    for (uint64_t c = tid; c < n_chunks; c += n_threads)
+      part[c] = chunk_hash(p + c * kHashChunk, std::min(kHashChunk, bytes - c * kHashChunk), c);
+  };
+  std::vector<std::thread> th;
+  for (unsigned i = 0; i < n_threads; ++i) th.emplace_back(work, i);
+  for (auto& x : th) x.join();
+  return combine_chunks(part, bytes);
+}
+
+uint64_t file_hash(const std::string& path) {
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) die("cannot open " + path + ": " + strerror(errno));
+  fseek(f, 0, SEEK_END);
+  const uint64_t bytes = (uint64_t)ftell(f);
+  fclose(f);
+  const uint64_t n_chunks = (bytes + kHashChunk - 1) / kHashChunk;
+  std::vector<uint64_t> part(n_chunks, 0);
+  const unsigned n_threads = hash_threads();
+  auto work = [&](unsigned tid) {
+    FILE* g = fopen(path.c_str(), "rb");
+    if (!g) return;
+    std::vector<unsigned char> buf(kHashChunk);
+    for (uint64_t c = tid; c < n_chunks; c += n_threads) {
+      const uint64_t len = std::min(kHashChunk, bytes - c * kHashChunk);
+      fseek(g, (long)(c * kHashChunk), SEEK_SET);
+      part[c] = fread(buf.data(), 1, len, g) == len ? chunk_hash(buf.data(), len, c) : ~0ull;
+    }
+    fclose(g);
+  };
+  std::vector<std::thread> th;
+  for (unsigned i = 0; i < n_threads; ++i) th.emplace_back(work, i);
+  for (auto& x : th) x.join();
+  return combine_chunks(part, bytes);
+}
+
+namespace {
+// Flushes a written file to stable storage and closes it.
+void close_durable(FILE* f, const std::string& path) {
+  if (fflush(f) != 0 || fsync(fileno(f)) != 0) die("write failed: " + path + ": " + strerror(errno));
+  close_text(f, path);
+}
+
+std::string dir_of(const std::string& path) {
+  const size_t sl = path.rfind('/');
+  return sl == std::string::npos ? std::string(".") : path.substr(0, sl);
+}
+
+std::string base_of(const std::string& path) {
+  const size_t sl = path.rfind('/');
+  return sl == std::string::npos ? path : path.substr(sl + 1);
+}
+
+void sync_dir(const std::string& dir) {
+  int fd = open(dir.c_str(), O_RDONLY);
+  if (fd >= 0) {
+    (void)fsync(fd);
+    close(fd);
+  }
+}
+}  // namespace
+
+std::string snapshot_name(const std::string& record_path, uint32_t iterations) {
+  return base_of(record_path) + ".z" + std::to_string(iterations);
+}
+
+std::string snapshot_file(const std::string& record_path, const std::string& name) {
+  return dir_of(record_path) + "/" + name;
+}
+
+void write_snapshot(const std::string& path, const void* z, uint64_t bytes) {
+  const std::string tmp = path + ".tmp";
+  FILE* f = create_text(tmp);
+  if (bytes && fwrite(z, 1, bytes, f) != bytes) die("write failed: " + tmp);
+  close_durable(f, tmp);
+  if (rename(tmp.c_str(), path.c_str()) != 0) die("cannot rename " + tmp + ": " + strerror(errno));
+  sync_dir(dir_of(path));
+}
+
+std::vector<unsigned char> read_snapshot(const Checkpoint& c, const std::string& record_path) {
+  const std::string path = snapshot_file(record_path, c.z_snapshot);
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) die("checkpoint snapshot " + path + " is missing: " + strerror(errno));
+  std::vector<unsigned char> buf(c.tokens * sizeof(uint32_t));
+  const size_t got = buf.empty() ? 0 : fread(buf.data(), 1, buf.size(), f);
+  const bool extra = fgetc(f) != EOF;
+  fclose(f);
+  if (got != buf.size() || extra) die("checkpoint snapshot " + path + " does not hold " + std::to_string(c.tokens) + " topics");
+  if (bytes_hash(buf.data(), buf.size()) != c.z_hash) die("checkpoint snapshot " + path + " does not match its record");
+  return buf;
+}
+
 void write_checkpoint(const std::string& path, const Checkpoint& c) {
   const std::string tmp = path + ".tmp";
   FILE* f = create_text(tmp);
   fprintf(f, "gpulda-checkpoint 1\n");
+  if (!c.z_snapshot.empty()) fprintf(f, "z_snapshot %s\n", c.z_snapshot.c_str());
   fprintf(f, "iterations %u\nseed %" PRIu64 "\nK %u\nV %u\ntokens %" PRIu64 "\ndocuments %" PRIu64 "\n", c.iterations,
           c.seed, c.K, c.V, c.tokens, c.docs);
   fprintf(f, "alpha_bits %08x\nbeta_bits %08x\nbuffer_size %u\nmax_D %u\nz_hash %016" PRIx64 "\n", c.alpha_bits,
           c.beta_bits, c.buffer_size, c.max_D, c.z_hash);
-  close_text(f, tmp);
+  close_durable(f, tmp);
   if (rename(tmp.c_str(), path.c_str()) != 0) die("cannot rename " + tmp + ": " + strerror(errno));
+  sync_dir(dir_of(path));
 }
 
 bool read_checkpoint(const std::string& path, Checkpoint* c) {
@@ -424,6 +509,9 @@ bool read_checkpoint(const std::string& path, Checkpoint* c) {
   c->buffer_size = (uint32_t)get("buffer_size", 10);
   c->max_D = (uint32_t)get("max_D", 10);
   c->z_hash = get("z_hash", 16);
+  auto snap = kv.find("z_snapshot");
+  c->z_snapshot = snap == kv.end() ? std::string() : snap->second;
+  if (c->z_snapshot.find('/') != std::string::npos) die("bad checkpoint snapshot name: " + c->z_snapshot);
   return true;
 }
 

@@ -50,10 +50,13 @@ void write_phi(const std::string& path, const float* phi, uint32_t V, uint32_t K
 // n_wk of a z.bin / w.bin pair, V x K word-major.
 std::vector<uint32_t> counts_from_files(const TempPaths& t, uint32_t V, uint32_t K);
 
-// Hash of a file's bytes (order-dependent, 64-bit).
+// Hash of a file's bytes (order-dependent, 64-bit); bytes_hash of the same bytes in memory is equal.
 uint64_t file_hash(const std::string& path);
+uint64_t bytes_hash(const void* data, uint64_t bytes);
 
-// State after a completed iteration: the run's parameters, the corpus shape and a hash of z.bin.
+// State after a completed iteration: the run's parameters, the corpus shape, and a snapshot of z
+// (a file beside the record, named in it) with its hash.  The next iteration rewrites z.bin in
+// place, so a run stopped at any point resumes from the snapshot, not from z.bin.
 struct Checkpoint {
   uint32_t iterations = 0;
   uint64_t seed = 0;
@@ -62,12 +65,23 @@ struct Checkpoint {
   uint32_t alpha_bits = 0, beta_bits = 0;
   uint32_t buffer_size = 0, max_D = 0;
   uint64_t z_hash = 0;
+  std::string z_snapshot;  // file name of the z snapshot, in the record's directory
 };
 
-// Written to path + ".tmp" and renamed, so a reader sees either the old or the new record.
+// Written to path + ".tmp", flushed to disk and renamed, so a reader sees either the old or the new
+// record.  The snapshot it names is written first (write_snapshot), so every record names a
+// complete snapshot.
 void write_checkpoint(const std::string& path, const Checkpoint& c);
 // false if the file does not exist; exits with a message if it is malformed.
 bool read_checkpoint(const std::string& path, Checkpoint* c);
+// Name of the snapshot of the record at `record_path` after `iterations` iterations, and the file
+// a snapshot name refers to (in the record's directory).
+std::string snapshot_name(const std::string& record_path, uint32_t iterations);
+std::string snapshot_file(const std::string& record_path, const std::string& name);
+// z as little-endian u32, written to path + ".tmp", flushed to disk and renamed.
+void write_snapshot(const std::string& path, const void* z, uint64_t bytes);
+// The record's snapshot, checked against its length and hash (exits with a message otherwise).
+std::vector<unsigned char> read_snapshot(const Checkpoint& c, const std::string& record_path);
 
 [[noreturn]] void die(const std::string& msg);
 

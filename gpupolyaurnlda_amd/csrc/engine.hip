@@ -184,7 +184,9 @@ struct Segment {
   const u32* key_d = nullptr;
   u32 n_docs = 0;
   u64 n_tokens = 0;
-  u64 fingerprint = 0;
+  u64 fingerprint = 0;  // content hash of w and d (the cache key with the sizes)
+  u64 bytes = 0;        // device bytes this segment holds
+  u64 last_use = 0;     // cache clock at its last use (least recently used is evicted first)
   u32 max_len = 0;
   u32 n_tasks = 0;
   u32* w = nullptr;
@@ -229,8 +231,8 @@ int launch_buckets(SweepParams P, const std::vector<Segment::Bucket>& buckets, c
       case 2: k_sweep_lines<2><<<bk.count, 64, lds, st>>>(P); break;
       case 4: k_sweep_lines<4><<<bk.count, 64, lds, st>>>(P); break;
       case 8: k_sweep_lines<8><<<bk.count, 64, lds, st>>>(P); break;
-      case 16: k_sweep_lines<16><<<bk.count, 64, lds, st>>>(P); break;
-      case 32: k_sweep_lines<32><<<bk.count, 64, lds, st>>>(P); break;
+      // 16, 32 and larger: maps beyond eight lines take the line-loop draw (the buckets still
+      // order the launches and size their LDS)
       default: k_sweep_lines<0><<<bk.count, 64, lds, st>>>(P); break;
     }
     HCK(hipGetLastError());
@@ -261,7 +263,8 @@ u64 mix64(u64 h, u64 v) {
   return h * 0xBF58476D1CE4E5B9ull;
 }
 
-// Cheap content fingerprint of a slice: all document lengths and 4096 sampled word ids.
+// Sampled fingerprint of a slice: all document lengths and 4096 sampled word ids (the trusted
+// cache mode, for callers whose w and d never change, such as read-only mappings).
 u64 slice_fingerprint(const u32* w, const u32* d, u32 n_docs, u64 n_tokens) {
   u64 h = mix64(n_docs, n_tokens);
   for (u32 i = 0; i < n_docs; ++i) h = mix64(h, d[i]);
@@ -270,6 +273,34 @@ u64 slice_fingerprint(const u32* w, const u32* d, u32 n_docs, u64 n_tokens) {
     for (u64 i = 0; i < n_tokens; i += step) h = mix64(h, w[i]);
     h = mix64(h, w[n_tokens - 1]);
   }
+  return h;
+}
+
+// 64-bit hash of n u32 words, in chunks of kHashChunk words hashed independently (four
+// multiply-rotate lanes each, after xxHash64's round) and combined in chunk order, so it does
+// not depend on how the chunks are spread over threads.
+constexpr u64 kHashChunk = 1ull << 18;
+constexpr u64 kHashP1 = 0x9E3779B185EBCA87ull, kHashP2 = 0xC2B2AE3D27D4EB4Full, kHashP3 = 0x165667B19E3779F9ull;
+inline u64 rotl64(u64 x, int r) { return (x << r) | (x >> (64 - r)); }
+inline u64 hash_round(u64 acc, u64 x) { return rotl64(acc + x * kHashP2, 31) * kHashP1; }
+
+u64 hash_chunk(const u32* p, u64 n, u64 index) {
+  u64 a0 = kHashP1 + index, a1 = kHashP2 ^ index, a2 = kHashP3 + (index << 1), a3 = ~index;
+  u64 i = 0;
+  for (; i + 8 <= n; i += 8) {
+    u64 x[4];
+    memcpy(x, p + i, 32);
+    a0 = hash_round(a0, x[0]);
+    a1 = hash_round(a1, x[1]);
+    a2 = hash_round(a2, x[2]);
+    a3 = hash_round(a3, x[3]);
+  }
+  u64 h = rotl64(a0, 1) + rotl64(a1, 7) + rotl64(a2, 12) + rotl64(a3, 18);
+  for (; i < n; ++i) h = hash_round(h, p[i]);
+  h ^= n;
+  h ^= h >> 33;
+  h *= kHashP2;
+  h ^= h >> 29;
   return h;
 }
 
@@ -403,6 +434,7 @@ struct BufferState {
   // work left for sync_buffer: widen the parts' topics into the caller's z as they arrive
   Segment* pending = nullptr;
   u32* pending_z = nullptr;
+  Buffer* owner = nullptr;  // the caller's struct (its gpu_w / gpu_d_len name the segment in use)
 };
 
 // Threads for host-side conversions of z (the caller's thread takes the first share).
@@ -411,11 +443,11 @@ u32 host_threads() {
   return std::max<u32>(1, std::min<u32>(16, hc ? hc : 1));
 }
 
-// f(begin, end) over [0, n), split across host_threads() threads.
+// f(begin, end) over [0, n), split across host_threads() threads (n of at least min_split).
 template <class F>
-void host_parallel(u64 n, F f) {
+void host_parallel(u64 n, F f, u64 min_split = 1ull << 16) {
   const u32 T = host_threads();
-  if (T == 1 || n < (1ull << 16)) {
+  if (T == 1 || n < min_split) {
     f((u64)0, n);
     return;
   }
@@ -451,6 +483,22 @@ void widen_topics(const u16* in, u32* out, u64 n) {
   });
 }
 
+// Hash of a slice's whole content: its sizes, every document length and every word id.
+u64 slice_content_hash(const u32* w, const u32* d, u32 n_docs, u64 n_tokens) {
+  const u64 nc = (n_tokens + kHashChunk - 1) / kHashChunk;
+  std::vector<u64> part(nc);
+  host_parallel(
+      nc,
+      [&](u64 a, u64 b) {
+        for (u64 c = a; c < b; ++c)
+          part[c] = hash_chunk(w + c * kHashChunk, std::min(kHashChunk, n_tokens - c * kHashChunk), c);
+      },
+      2);
+  u64 h = mix64(mix64(n_docs, n_tokens), hash_chunk(d, n_docs, ~0ull));
+  for (u64 x : part) h = mix64(h, x);
+  return h;
+}
+
 enum Schedule : int { kSchedBuckets = 0, kSchedPersistent = 1, kSchedPersistentBlock = 2, kSchedPersistentSplit = 3 };
 
 struct State {
@@ -461,7 +509,16 @@ struct State {
   u64 seed = 0;
   u32 iter = 0;
   u32 doc_counter = 0;
-  bool slice_cache = true;
+  // device copies of buffer slices: 0 = none (every call uploads, as the reference), 1 = reused
+  // when the whole content matches (default), 2 = reused when the host pointers, sizes and a
+  // sampled fingerprint match (for callers whose w and d never change)
+  int slice_cache = 1;
+  u64 cache_budget = 0;       // bytes the cached slices may hold (0: half the free HBM at initialize)
+  u64 cache_auto_budget = 0;
+  u64 cache_bytes = 0;
+  u64 cache_clock = 0;
+  u64 cache_builds = 0;       // slices built (uploads of w and d)
+  std::vector<BufferState*> buffers;
   hipStream_t stream = nullptr;  // Phi stream / resident stream
   f32* phi = nullptr;
   AliasSlot* alias = nullptr;
@@ -624,48 +681,98 @@ int launch_part_sweep(const SweepParams& P0, const Segment* sg, u32 p, u32* head
   return 0;
 }
 
-Segment* find_cached(const u32* w, const u32* d, u32 n_docs, u64 n_tokens, u64 fp) {
-  for (Segment* s : S.cache)
-    if (s->key_w == w && s->key_d == d && s->n_docs == n_docs && s->n_tokens == n_tokens && s->fingerprint == fp)
-      return s;
-  return nullptr;
+// A segment some buffer still has in flight (between its sample_z_async and sync_buffer).
+bool segment_pinned(const Segment* s) {
+  for (const BufferState* b : S.buffers)
+    if (b->pending == s) return true;
+  return false;
+}
+
+u64 segment_bytes(const Segment* s, u64 n_tasks) {
+  return s->n_tokens * (sizeof(u32) + sizeof(u16) + sizeof(u32)) + ((u64)s->n_docs + 1) * sizeof(u64) +
+         (u64)s->n_docs * 3 * sizeof(u32) + n_tasks * sizeof(CountTask);
+}
+
+// Frees cached segment i.  No work is queued on it: it is not pinned, and a buffer's work ends at
+// its sync_buffer.  Callers' Buffer structs that still name it forget it.
+void evict_segment(size_t i) {
+  Segment* s = S.cache[i];
+  for (BufferState* b : S.buffers)
+    if (b->owner && b->owner->gpu_w == (void*)s->w) {
+      b->owner->gpu_w = nullptr;
+      b->owner->gpu_d_len = nullptr;
+    }
+  S.cache_bytes -= s->bytes;
+  free_segment(s);
+  S.cache.erase(S.cache.begin() + (long)i);
+}
+
+// Evicts the least recently used segments that no buffer has in flight until `need` more bytes
+// fit the budget (or only pinned segments are left).
+void make_room(u64 need) {
+  const u64 budget = S.cache_budget ? S.cache_budget : S.cache_auto_budget;
+  while (!S.cache.empty() && S.cache_bytes + need > budget) {
+    long victim = -1;
+    for (size_t i = 0; i < S.cache.size(); ++i)
+      if (!segment_pinned(S.cache[i]) && (victim < 0 || S.cache[i]->last_use < S.cache[(size_t)victim]->last_use))
+        victim = (long)i;
+    if (victim < 0) return;
+    evict_segment((size_t)victim);
+  }
 }
 
 int get_segment(BufferState* bs, const Buffer* B, Segment** out) {
   const u32* w = B->w;
   const u32* d = B->d;
-  if (S.slice_cache) {
-    u64 fp = slice_fingerprint(w, d, B->n_docs, B->n_tokens);
-    Segment* s = find_cached(w, d, B->n_docs, B->n_tokens, fp);
-    if (!s) {
-      s = new Segment();
-      int rc = build_segment(s, w, d, B->n_docs, B->n_tokens, S.V, S.K);
-      if (rc != 0 && g_err.find("out of memory") != std::string::npos) {
-        // drop every cached slice and retry once
-        HCK(hipDeviceSynchronize());
-        for (Segment* c : S.cache) free_segment(c);
-        S.cache.clear();
-        free_segment(s);
-        s = new Segment();
-        rc = build_segment(s, w, d, B->n_docs, B->n_tokens, S.V, S.K);
-      }
-      if (rc != 0) {
-        free_segment(s);
-        return rc;
-      }
-      s->key_w = w;
-      s->key_d = d;
-      s->fingerprint = fp;
-      S.cache.push_back(s);
-    }
-    *out = s;
+  if (S.slice_cache == 0) {
+    HCK(hipStreamSynchronize(bs->stream));
+    free_segment(bs->seg);
+    bs->seg = new Segment();
+    ++S.cache_builds;
+    CK(build_segment(bs->seg, w, d, B->n_docs, B->n_tokens, S.V, S.K));
+    *out = bs->seg;
     return 0;
   }
-  HCK(hipStreamSynchronize(bs->stream));
-  free_segment(bs->seg);
-  bs->seg = new Segment();
-  CK(build_segment(bs->seg, w, d, B->n_docs, B->n_tokens, S.V, S.K));
-  *out = bs->seg;
+  const bool trusted = S.slice_cache == 2;
+  const u64 fp = trusted ? slice_fingerprint(w, d, B->n_docs, B->n_tokens)
+                         : slice_content_hash(w, d, B->n_docs, B->n_tokens);
+  const u32* key_w = trusted ? w : nullptr;  // a content-verified entry is not tied to the pointers
+  const u32* key_d = trusted ? d : nullptr;
+  for (Segment* s : S.cache)
+    if (s->key_w == key_w && s->key_d == key_d && s->n_docs == B->n_docs && s->n_tokens == B->n_tokens &&
+        s->fingerprint == fp && !segment_pinned(s)) {  // (another buffer in flight on it: a copy of its own)
+      s->last_use = ++S.cache_clock;
+      *out = s;
+      return 0;
+    }
+  Segment probe;
+  probe.n_docs = B->n_docs;
+  probe.n_tokens = B->n_tokens;
+  make_room(segment_bytes(&probe, (u64)S.V + B->n_tokens / kCountTask + 1));
+  Segment* s = new Segment();
+  ++S.cache_builds;
+  int rc = build_segment(s, w, d, B->n_docs, B->n_tokens, S.V, S.K);
+  if (rc != 0 && g_err.find("out of memory") != std::string::npos) {
+    // evict every slice no buffer has in flight and retry once
+    free_segment(s);
+    HCK(hipDeviceSynchronize());
+    for (size_t i = S.cache.size(); i-- > 0;)
+      if (!segment_pinned(S.cache[i])) evict_segment(i);
+    s = new Segment();
+    rc = build_segment(s, w, d, B->n_docs, B->n_tokens, S.V, S.K);
+  }
+  if (rc != 0) {
+    free_segment(s);
+    return rc;
+  }
+  s->key_w = key_w;
+  s->key_d = key_d;
+  s->fingerprint = fp;
+  s->bytes = segment_bytes(s, s->n_tasks);
+  s->last_use = ++S.cache_clock;
+  S.cache_bytes += s->bytes;
+  S.cache.push_back(s);
+  *out = s;
   return 0;
 }
 
@@ -673,6 +780,8 @@ void release_all() {
   if (S.stream) (void)hipDeviceSynchronize();
   for (Segment* s : S.cache) free_segment(s);
   S.cache.clear();
+  S.cache_bytes = 0;
+  S.buffers.clear();
   free_segment(S.resident);
   S.resident = nullptr;
   dfree(S.phi);
@@ -779,6 +888,13 @@ int do_initialize(Args* args, Buffer* buffers, u32 n_buffers) {
     B->gpu_K_d = nullptr;
     B->gpu_rng = bs;
     B->stream = &bs->stream;
+    bs->owner = B;
+    S.buffers.push_back(bs);
+  }
+  {
+    size_t free_b = 0, total_b = 0;
+    HCK(hipMemGetInfo(&free_b, &total_b));
+    S.cache_auto_budget = free_b / 2;
   }
   HCK(hipStreamSynchronize(S.stream));
   S.ready = true;
@@ -800,6 +916,7 @@ int do_sample_z_async(Buffer* B) {
   if (B->n_docs > S.args->max_D) return fail("buffer holds %u documents, max_D is %u", B->n_docs, S.args->max_D);
   if (B->n_tokens > S.args->buffer_size)
     return fail("buffer holds %u tokens, buffer_size is %u", B->n_tokens, S.args->buffer_size);
+  if (bs->pending) return fail("sample_z_async: the previous call on this buffer was not synchronised");
   Segment* sg = nullptr;
   CK(get_segment(bs, B, &sg));
   hipStream_t st = bs->stream;
@@ -811,7 +928,6 @@ int do_sample_z_async(Buffer* B) {
   const SweepParams P = sweep_params(sg, S.doc_counter);
   const u32 np = (u32)sg->parts.size();
   if (np > kMaxParts) return fail("slice split into %u parts (limit %u)", np, kMaxParts);
-  if (bs->pending) return fail("sample_z_async: the previous call on this buffer was not synchronised");
   bs->host_bad = 0;
   for (u32 p = 0; p < np; ++p) {
     const Segment::Part& pt = sg->parts[p];
@@ -982,9 +1098,31 @@ extern "C" int gpulda_set_part_tokens(uint64_t min_tokens) {
   return 0;
 }
 
-extern "C" int gpulda_set_slice_cache(int enabled) {
-  S.slice_cache = enabled != 0;
+extern "C" int gpulda_set_slice_cache(int mode) {
+  if (mode < 0 || mode > 2) return fail("slice cache mode %d (0 off, 1 content-verified, 2 trusted)", mode);
+  if (mode != S.slice_cache) {
+    // entries are keyed differently in each mode: drop those no buffer has in flight
+    for (size_t i = S.cache.size(); i-- > 0;)
+      if (!segment_pinned(S.cache[i])) evict_segment(i);
+  }
+  S.slice_cache = mode;
   return 0;
+}
+
+extern "C" int gpulda_set_slice_cache_budget(uint64_t bytes) {
+  S.cache_budget = bytes;
+  return 0;
+}
+
+extern "C" int gpulda_slice_cache_stats(uint32_t* segments, uint64_t* bytes, uint64_t* builds) {
+  if (segments) *segments = (uint32_t)S.cache.size();
+  if (bytes) *bytes = S.cache_bytes;
+  if (builds) *builds = S.cache_builds;
+  return 0;
+}
+
+extern "C" uint64_t gpulda_slice_hash(const uint32_t* w, const uint32_t* d_len, uint32_t n_docs, uint64_t n_tokens) {
+  return slice_content_hash(w, d_len, n_docs, n_tokens);
 }
 
 extern "C" uint32_t gpulda_iteration(void) { return S.iter; }

@@ -98,7 +98,9 @@ template <int LMAX>
 __global__ void k_sweep_persistent(SweepParams P, u32* head, u32 n_items);
 // Block-per-document sweep (4 wavefronts per document); head == null: block b takes item b.
 __global__ void k_sweep_block(SweepParams P, u32* head, u32 n_items);
-// Register-resident line counts of the specialised sweep kernels (0 = any size).
+// Launch buckets of the per-bucket schedule, by map lines (0 = any size).  Buckets 1-8 run the
+// kernel with that many register-resident lines; 16, 32 and 0 run the line-loop kernel
+// (k_sweep_lines<0>) with the LDS of their largest map.
 constexpr int kSweepBuckets[] = {1, 2, 4, 8, 16, 32};
 // Kernel family member for a map of `lines` 64-slot lines.
 inline int sweep_bucket_lmax(u32 lines) {

@@ -681,13 +681,13 @@ __global__ __launch_bounds__(64) void k_sweep_lines(SweepParams P) {
   sweep_doc<LMAX>(P, doc, sweep_smem);
 }
 
+// LMAX = 1, 2, 4, 8: the register variants up to that many lines; 0: any map (register variants up
+// to 8 lines are then not compiled in, so larger maps keep the registers of the line-loop form).
 template __global__ void k_sweep_lines<0>(SweepParams);
 template __global__ void k_sweep_lines<1>(SweepParams);
 template __global__ void k_sweep_lines<2>(SweepParams);
 template __global__ void k_sweep_lines<4>(SweepParams);
 template __global__ void k_sweep_lines<8>(SweepParams);
-template __global__ void k_sweep_lines<16>(SweepParams);
-template __global__ void k_sweep_lines<32>(SweepParams);
 
 // Persistent form: a fixed grid of wavefronts takes documents from a queue (P.doc_order, longest
 // first) through an atomic head.  Every wavefront leaves once the head passes n_items, so the

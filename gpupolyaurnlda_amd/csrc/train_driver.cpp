@@ -12,11 +12,17 @@
 //   --stages LIST          which of preprocess,train,output run (default: all three, in order)
 //   --phi_file PATH        after training, Phi of the last iteration (V x K f32, word-major)
 //   --theta_file PATH      after training, per-document topic counts (`k:n` pairs per line)
-//   --checkpoint_file PATH after every --checkpoint_every iterations, a record of the completed
-//                          iterations, the parameters and a hash of z.bin
-//   --resume               continue from --checkpoint_file: n_wk is rebuilt from z.bin and the
+//   --checkpoint_file PATH after every --checkpoint_every iterations, a snapshot of z (PATH.z<it>,
+//                          written and flushed first) and a record of the completed iterations,
+//                          the parameters and the snapshot's hash (written, flushed, renamed)
+//   --resume               continue from --checkpoint_file: z.bin is restored from the snapshot
+//                          (the iterations after the checkpoint rewrite z.bin in place, so a run
+//                          stopped anywhere leaves it part-way), n_wk is rebuilt from it and the
 //                          iteration counter restored, so the run continues the uninterrupted
 //                          chain bit for bit (any buffer_size / max_D)
+//   --stop_after_buffers N stop the run once N buffers have been synchronised (all device work
+//                          finished, z.bin flushed, no checkpoint written): a stop in the middle
+//                          of an iteration, for tests of --resume
 //   --n_buffers N          buffers in flight (default 1, as train.rs): buffer i + 1 is packed and
 //                          submitted while buffer i is swept, each synchronised before reuse;
 //                          results do not depend on N
@@ -62,6 +68,7 @@ struct Options {
   uint32_t checkpoint_every = 1;
   bool resume = false;
   uint32_t n_buffers = 1;
+  uint64_t stop_after_buffers = 0;
 };
 
 unsigned parse_stages(const std::string& v) {
@@ -117,6 +124,7 @@ Options parse(int argc, char** argv) {
     else if (a == "--checkpoint_every") o.checkpoint_every = parse_u32(a, val());
     else if (a == "--resume") o.resume = true;
     else if (a == "--n_buffers") o.n_buffers = parse_u32(a, val());
+    else if (a == "--stop_after_buffers") o.stop_after_buffers = parse_u32(a, val());
     else die("unknown flag " + a);
   }
   if (o.resume && o.checkpoint_file.empty()) die("--resume needs --checkpoint_file");
@@ -170,8 +178,21 @@ gpulda_io::Checkpoint checkpoint_of(const Options& o, uint32_t V, const Map& z, 
   c.beta_bits = f32_bits(o.beta);
   c.buffer_size = o.buffer_size;
   c.max_D = o.max_D;
-  c.z_hash = gpulda_io::file_hash(o.tmp.z);
   return c;
+}
+
+// Snapshot of z after `iterations` iterations, then the record that names it; the previous
+// snapshot is removed once the new record is in place.
+void write_checkpoint(const Options& o, uint32_t V, const Map& z, const Map& d, uint32_t iterations,
+                      std::string* prev_snapshot) {
+  gpulda_io::Checkpoint c = checkpoint_of(o, V, z, d, iterations);
+  c.z_hash = gpulda_io::bytes_hash(z.p, z.bytes);
+  c.z_snapshot = gpulda_io::snapshot_name(o.checkpoint_file, iterations);
+  gpulda_io::write_snapshot(gpulda_io::snapshot_file(o.checkpoint_file, c.z_snapshot), z.p, z.bytes);
+  gpulda_io::write_checkpoint(o.checkpoint_file, c);
+  if (!prev_snapshot->empty() && *prev_snapshot != c.z_snapshot)
+    (void)unlink(gpulda_io::snapshot_file(o.checkpoint_file, *prev_snapshot).c_str());
+  *prev_snapshot = c.z_snapshot;
 }
 
 // train.rs:60-105, plus resume, checkpoints and exports.
@@ -196,6 +217,7 @@ void train(const Options& o) {
   // the checkpoint is checked before the device is touched
   uint32_t start = 0;
   std::vector<uint32_t> n_resume;
+  std::string snapshot;  // snapshot named by the newest record of this run
   if (o.resume) {
     gpulda_io::Checkpoint ck;
     if (!gpulda_io::read_checkpoint(o.checkpoint_file, &ck)) die("no checkpoint at " + o.checkpoint_file);
@@ -203,7 +225,19 @@ void train(const Options& o) {
     if (ck.seed != now.seed || ck.K != now.K || ck.V != now.V || ck.tokens != now.tokens || ck.docs != now.docs ||
         ck.alpha_bits != now.alpha_bits || ck.beta_bits != now.beta_bits)
       die("checkpoint " + o.checkpoint_file + " was written for other parameters or another corpus");
-    if (ck.z_hash != now.z_hash) die("z.bin changed after checkpoint " + o.checkpoint_file + " was written");
+    if (ck.z_snapshot.empty()) {
+      // a record without a snapshot: z.bin itself must still hold the checkpointed topics
+      if (ck.z_hash != gpulda_io::file_hash(o.tmp.z))
+        die("z.bin changed after checkpoint " + o.checkpoint_file + " was written");
+    } else {
+      const std::vector<unsigned char> snap = gpulda_io::read_snapshot(ck, o.checkpoint_file);
+      if (z.bytes && memcmp(z.p, snap.data(), z.bytes) != 0) {
+        memcpy(z.p, snap.data(), z.bytes);
+        if (msync(z.p, z.bytes, MS_SYNC) != 0) die("cannot flush " + o.tmp.z);
+        printf("restored z.bin from %s\n", ck.z_snapshot.c_str());
+      }
+      snapshot = ck.z_snapshot;
+    }
     n_resume = gpulda_io::counts_from_files(o.tmp, args.V, o.K);
     start = ck.iterations;
   }
@@ -220,6 +254,18 @@ void train(const Options& o) {
     printf("resuming after iteration %u\n", start);
   }
 
+  uint64_t synced = 0;  // buffers synchronised in this run (--stop_after_buffers)
+  auto synchronise = [&](Buffer* b) {
+    sync_buffer(b);
+    if (o.stop_after_buffers && ++synced == o.stop_after_buffers) {
+      for (uint32_t i = 0; i < o.n_buffers; ++i)
+        if (in_flight[i] && &buffers[i] != b) sync_buffer(&buffers[i]);
+      if (z.bytes) msync(z.p, z.bytes, MS_SYNC);
+      printf("stopped after %llu buffers\n", (unsigned long long)synced);
+      fflush(stdout);
+      exit(3);
+    }
+  };
   for (uint32_t it = start; it < o.n_mc; ++it) {
     sample_phi();
     size_t doc = 0, tok = 0;
@@ -236,7 +282,10 @@ void train(const Options& o) {
       if (n_docs == 0) break;
       if (tok + n_tokens > z.n) die("d.bin lengths exceed the tokens in w.bin");
       Buffer& b = buffers[slot];
-      if (in_flight[slot]) sync_buffer(&b);
+      if (in_flight[slot]) {
+        in_flight[slot] = 0;
+        synchronise(&b);
+      }
       b.z = z.p + tok;
       b.w = w.p + tok;
       b.d = d.p + doc;
@@ -246,8 +295,8 @@ void train(const Options& o) {
       sample_z_async(&b);
       in_flight[slot] = 1;
       if (o.n_buffers == 1) {
-        sync_buffer(&b);
         in_flight[slot] = 0;
+        synchronise(&b);
       }
       // empty_buffer (train.rs:47-58)
       doc += n_docs;
@@ -255,13 +304,13 @@ void train(const Options& o) {
     }
     for (uint32_t i = 0; i < o.n_buffers; ++i)
       if (in_flight[i]) {
-        sync_buffer(&buffers[i]);
         in_flight[i] = 0;
+        synchronise(&buffers[i]);
       }
     if (z.bytes) msync(z.p, z.bytes, MS_SYNC);  // z.get_mut().flush() (train.rs:99)
     printf("finished iteration: %u\n", it);
     if (!o.checkpoint_file.empty() && ((it + 1) % o.checkpoint_every == 0 || it + 1 == o.n_mc))
-      gpulda_io::write_checkpoint(o.checkpoint_file, checkpoint_of(o, args.V, z, d, it + 1));
+      write_checkpoint(o, args.V, z, d, it + 1, &snapshot);
   }
   if (!o.phi_file.empty()) {
     if (gpulda_iteration() == 0) die("--phi_file: no iteration has run");

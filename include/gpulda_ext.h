@@ -43,9 +43,21 @@ uint64_t gpulda_get_seed(void);
 const char* gpulda_last_error(void);
 /* Device name, compute units, HBM bytes, max LDS bytes per workgroup of the current device. */
 int gpulda_device_info(char* name, int name_len, int* cu_count, uint64_t* hbm_bytes, int* lds_per_block);
-/* 1 = cache device copies of buffer slices keyed by their host pointers (default), 0 = re-upload
- * w and d on every sample_z_async (for callers that refill one host buffer). */
-int gpulda_set_slice_cache(int enabled);
+/* Device copies of buffer slices (w, d and what the sweep derives from them) across
+ * sample_z_async calls.  The reference uploads w and d on every call (src/cu/train.cu:160-163).
+ *   0  no cache: every call uploads its slice;
+ *   1  (default) a slice is reused when its whole content matches -- every document length and
+ *      word id, by a 64-bit hash computed on host threads at each call -- whatever its pointers;
+ *   2  trusted: reused when the host pointers, sizes and a sampled fingerprint match, for callers
+ *      whose w and d never change between calls (read-only mappings, as the reference's host).
+ * Cached slices are held within a byte budget (default: half the free HBM at initialize), least
+ * recently used first out; a slice a buffer has in flight is never evicted. */
+int gpulda_set_slice_cache(int mode);
+int gpulda_set_slice_cache_budget(uint64_t bytes);  /* 0 = the default budget */
+/* Slices cached now, their device bytes, and slices built (uploads of w and d) so far. */
+int gpulda_slice_cache_stats(uint32_t* segments, uint64_t* bytes, uint64_t* builds);
+/* The content hash of cache mode 1 (host only; no device needed). */
+uint64_t gpulda_slice_hash(const uint32_t* w, const uint32_t* d_len, uint32_t n_docs, uint64_t n_tokens);
 /* sample_z_async splits a slice into up to 8 parts of whole documents and at least min_tokens
  * tokens each (default 4 Mi), and overlaps the copies of z for one part with the sweep of another.
  * Applies to slices first seen after the call.  Results do not depend on it. */

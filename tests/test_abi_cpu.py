@@ -120,3 +120,37 @@ def test_generator_is_deterministic_and_shaped():
     assert med.max() <= 2000 and abs(np.median(med) - 100) <= 3
     bat = corpus.lengths(corpus.BATCHED.scaled(200000))
     assert bat.min() >= 4 and bat.max() <= 20000
+
+
+def test_slice_hash_sees_every_word():
+    """The slice cache's default mode reuses a device copy only when the whole content matches: a
+    refilled host buffer that differs in one word -- one the old sampled fingerprint never read --
+    or in one document length must hash differently; equal content at another address hashes
+    the same.  (The reference re-uploads every call, train.cu:160-163.)"""
+    import ctypes as C
+    import time
+    from gpupolyaurnlda_amd._lib import lib, ptr
+    rng = np.random.default_rng(0)
+    n, D = 3_000_000, 30_000
+    w = rng.integers(0, 50_000, n).astype(np.uint32)
+    d = np.full(D, n // D, np.uint32)
+
+    def h(w, d):
+        return lib().gpulda_slice_hash(ptr(w, C.c_uint32), ptr(d, C.c_uint32), len(d), len(w))
+
+    base = h(w, d)
+    assert h(w.copy(), d.copy()) == base
+    step = n // 4096
+    for pos in (1, step + 1, n // 2 + 3, n - 2):  # none of them on the sampled grid
+        w2 = w.copy()
+        w2[pos] ^= 1
+        assert h(w2, d) != base, pos
+    d2 = d.copy()
+    d2[5] += 1
+    d2[6] -= 1
+    assert h(w, d2) != base
+    assert h(w[:-1], d) != base
+    big = rng.integers(0, 100_000, 50_000_000).astype(np.uint32)
+    t0 = time.perf_counter()
+    h(big, np.array([len(big)], np.uint32))
+    print(f"slice hash: {len(big) * 4 / (time.perf_counter() - t0) / 1e9:.1f} GB/s")

@@ -162,6 +162,22 @@ def test_resume_checks_run_before_the_device(tmp_path):
     assert out.returncode != 0 and "z.bin changed" in out.stderr
     out = _run(tmp, "--stages", "train", "--K", 5, "--resume", "--checkpoint_file", ck, check=False)
     assert out.returncode != 0 and "other parameters" in out.stderr
+    # a record naming a z snapshot: the snapshot must exist, hold every token and match the hash
+    rec = open(ck).read().replace("gpulda-checkpoint 1\n", "gpulda-checkpoint 1\nz_snapshot ck.txt.z2\n")
+    with open(ck, "w") as f:
+        f.write(rec)
+    out = _run(tmp, "--stages", "train", "--K", 4, "--resume", "--checkpoint_file", ck, check=False)
+    assert out.returncode != 0 and "snapshot" in out.stderr and "missing" in out.stderr
+    np.array([1, 2, 3], "<u4").tofile(f"{tmp}/ck.txt.z2")
+    out = _run(tmp, "--stages", "train", "--K", 4, "--resume", "--checkpoint_file", ck, check=False)
+    assert out.returncode != 0 and "does not hold 4 topics" in out.stderr
+    np.array([1, 2, 3, 0], "<u4").tofile(f"{tmp}/ck.txt.z2")
+    out = _run(tmp, "--stages", "train", "--K", 4, "--resume", "--checkpoint_file", ck, check=False)
+    assert out.returncode != 0 and "does not match its record" in out.stderr
+    with open(ck, "w") as f:
+        f.write(rec.replace("z_snapshot ck.txt.z2", "z_snapshot ../ck.txt.z2"))
+    out = _run(tmp, "--stages", "train", "--K", 4, "--resume", "--checkpoint_file", ck, check=False)
+    assert out.returncode != 0 and "bad checkpoint snapshot name" in out.stderr
 
 
 def test_unknown_stage_and_flag_rejected(tmp_path):

@@ -130,54 +130,93 @@ def test_driver_full_pipeline(tmp_path, orc):
     assert open(f"{tmp}/theta.txt").read() == "".join(s + "\n" for s in want_theta)
 
 
-def test_driver_checkpoint_resume(tmp_path):
-    """Two iterations, a checkpoint, then a resumed run to four (with another buffer size) give
-    the z.bin of four uninterrupted iterations, bit for bit."""
+def _buffers_per_iteration(d_len, buffer_size, max_D):
+    from gpupolyaurnlda_amd.train import fill_buffer
+    n, doc = 0, 0
+    while True:
+        nd, _ = fill_buffer(d_len, doc, buffer_size, max_D)
+        if nd == 0:
+            return n
+        n += 1
+        doc += nd
+
+
+def test_driver_checkpoint_resume(tmp_path, orc):
+    """Resume from a checkpoint, after a stop at an iteration boundary and after a stop in the
+    middle of an iteration (z.bin then holds part of the next iteration's topics; the resumed run
+    restores it from the checkpoint's snapshot).  Both resumed runs, with another buffer size, end
+    with the z of the oracle replaying four uninterrupted iterations."""
     import shutil
     from gpupolyaurnlda_amd import corpus
     spec = corpus.TINY.scaled(300)
     c = corpus.generate(spec)
-    base, a, b = (str(tmp_path / n) for n in ("base", "a", "b"))
+    base, b, m = (str(tmp_path / n) for n in ("base", "b", "m"))
     os.makedirs(base)
     _mallet_file(f"{base}/in.txt", c)
-    common = ("--K", spec.K, "--alpha", spec.alpha, "--beta", spec.beta, "--seed", 5)
+    seed, n_mc = 5, 4
+    common = ("--K", spec.K, "--alpha", spec.alpha, "--beta", spec.beta, "--seed", seed)
     _driver(base, "--stages", "preprocess", "--input", f"{base}/in.txt", *common)
-    shutil.copytree(base, a)
     shutil.copytree(base, b)
-    _driver(a, "--stages", "train", "--n_mc", 4, "--buffer_size", 3000, "--max_D", 40, *common)
+    shutil.copytree(base, m)
+    z0 = np.fromfile(f"{base}/z.bin", "<u4")
+    w = np.fromfile(f"{base}/w.bin", "<u4")
+    d_len = np.fromfile(f"{base}/d.bin", "<u4")
+    V = len(open(f"{base}/c.txt").read().splitlines())
+    want = _replay(orc, w, d_len, z0, V, spec.K, spec.alpha, spec.beta, seed, n_mc, 3000, 40).z
+    assert not np.array_equal(want, z0)
+    # stop at the boundary after iteration 2, resume to 4
     ck = f"{b}/ck.txt"
     _driver(b, "--stages", "train", "--n_mc", 2, "--buffer_size", 3000, "--max_D", 40, "--checkpoint_file", ck,
             *common)
-    assert "iterations 2" in open(ck).read()
-    out = _driver(b, "--stages", "train", "--n_mc", 4, "--buffer_size", 1000, "--max_D", 17, "--checkpoint_file", ck,
-                  "--resume", *common)
+    assert "iterations 2" in open(ck).read() and os.path.exists(f"{ck}.z2")
+    out = _driver(b, "--stages", "train", "--n_mc", n_mc, "--buffer_size", 1000, "--max_D", 17,
+                  "--checkpoint_file", ck, "--resume", *common)
     assert "resuming after iteration 2" in out and "finished iteration: 3" in out
-    assert "finished iteration: 1" not in out
-    assert "iterations 4" in open(ck).read()
-    za = np.fromfile(f"{a}/z.bin", "<u4")
-    zb = np.fromfile(f"{b}/z.bin", "<u4")
-    z0 = np.fromfile(f"{base}/z.bin", "<u4")
-    assert not np.array_equal(za, z0)
-    assert np.array_equal(za, zb)
+    assert "finished iteration: 1" not in out and "restored" not in out
+    assert "iterations 4" in open(ck).read() and os.path.exists(f"{ck}.z4") and not os.path.exists(f"{ck}.z2")
+    assert np.array_equal(np.fromfile(f"{b}/z.bin", "<u4"), want)
+    # stop in the middle of the third iteration (checkpoints every 2 iterations), resume to 4
+    per_it = _buffers_per_iteration(d_len, 3000, 40)
+    assert per_it >= 4
+    ck = f"{m}/ck.txt"
+    exe = os.path.join(ROOT, "gpupolyaurnlda_amd", "gpulda_train")
+    cmd = [exe, "--z_temp_file", f"{m}/z.bin", "--w_temp_file", f"{m}/w.bin", "--d_temp_file", f"{m}/d.bin",
+           "--K_d_temp_file", f"{m}/k_d.bin", "--c_temp_file", f"{m}/c.txt", "--stages", "train", "--n_mc", n_mc,
+           "--buffer_size", 3000, "--max_D", 40, "--checkpoint_file", ck, "--checkpoint_every", 2,
+           "--stop_after_buffers", 2 * per_it + per_it // 2, *common]
+    stop = subprocess.run([str(x) for x in cmd], capture_output=True, text=True, timeout=120)
+    assert stop.returncode == 3 and "stopped after" in stop.stdout, stop.stdout + stop.stderr
+    assert "iterations 2" in open(ck).read()
+    z_stopped = np.fromfile(f"{m}/z.bin", "<u4")
+    z_ck = np.fromfile(f"{ck}.z2", "<u4")
+    assert not np.array_equal(z_stopped, z_ck)  # z.bin moved on past the checkpoint
+    out = _driver(m, "--stages", "train", "--n_mc", n_mc, "--buffer_size", 1000, "--max_D", 17,
+                  "--checkpoint_file", ck, "--resume", *common)
+    assert "restored z.bin from ck.txt.z2" in out and "resuming after iteration 2" in out
+    assert np.array_equal(np.fromfile(f"{m}/z.bin", "<u4"), want)
 
 
-def test_driver_buffers_in_flight(tmp_path):
-    """Several buffers in flight (each synchronised before reuse) give the z.bin of one buffer
-    synchronised after every call, as train.rs runs it."""
+def test_driver_buffers_in_flight(tmp_path, orc):
+    """Three buffers in flight (each synchronised before reuse) give the z of the oracle replaying
+    train.rs's one-buffer loop."""
     import shutil
     from gpupolyaurnlda_amd import corpus
     spec = corpus.TINY.scaled(400)
     c = corpus.generate(spec)
-    base, a, b = (str(tmp_path / n) for n in ("base", "a", "b"))
+    base, b = (str(tmp_path / n) for n in ("base", "b"))
     os.makedirs(base)
     _mallet_file(f"{base}/in.txt", c)
-    common = ("--K", spec.K, "--alpha", spec.alpha, "--beta", spec.beta, "--seed", 9, "--n_mc", 3,
-              "--buffer_size", 1500, "--max_D", 30)
+    seed, n_mc, buffer_size, max_D = 9, 3, 1500, 30
+    common = ("--K", spec.K, "--alpha", spec.alpha, "--beta", spec.beta, "--seed", seed, "--n_mc", n_mc,
+              "--buffer_size", buffer_size, "--max_D", max_D)
     _driver(base, "--stages", "preprocess", "--input", f"{base}/in.txt", *common)
-    shutil.copytree(base, a)
     shutil.copytree(base, b)
-    _driver(a, "--stages", "train", *common)
     _driver(b, "--stages", "train", "--n_buffers", 3, *common)
-    za = np.fromfile(f"{a}/z.bin", "<u4")
-    assert not np.array_equal(za, np.fromfile(f"{base}/z.bin", "<u4"))
-    assert np.array_equal(za, np.fromfile(f"{b}/z.bin", "<u4"))
+    z0 = np.fromfile(f"{base}/z.bin", "<u4")
+    w = np.fromfile(f"{base}/w.bin", "<u4")
+    d_len = np.fromfile(f"{base}/d.bin", "<u4")
+    V = len(open(f"{base}/c.txt").read().splitlines())
+    assert _buffers_per_iteration(d_len, buffer_size, max_D) > 3
+    want = _replay(orc, w, d_len, z0, V, spec.K, spec.alpha, spec.beta, seed, n_mc, buffer_size, max_D).z
+    assert not np.array_equal(want, z0)
+    assert np.array_equal(np.fromfile(f"{b}/z.bin", "<u4"), want)

@@ -493,3 +493,113 @@ def test_ppu_row_shapes_match_oracle(S, orc):
         assert np.array_equal(pt, pt2) and np.array_equal(ts, ts2), it
         rows, _, _, _ = S.alias(pt, divisor=ts, alpha=0.1)
         assert np.array_equal(rows.view(np.uint32), orc.phi_finish(pt2, ts2, 0.1)[0].view(np.uint32)), it
+
+
+def _cache_stats():
+    from gpupolyaurnlda_amd._lib import lib, ptr
+    seg = np.zeros(1, np.uint32)
+    by = np.zeros(1, np.uint64)
+    builds = np.zeros(1, np.uint64)
+    assert lib().gpulda_slice_cache_stats(ptr(seg, C.c_uint32), ptr(by, C.c_uint64), ptr(builds, C.c_uint64)) == 0
+    return int(seg[0]), int(by[0]), int(builds[0])
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_slice_cache_refilled_buffer(orc, mode):
+    """One host buffer refilled in place with a slice that differs from the previous one in a single
+    word: the default cache (whole-content hash) and no cache both upload it again, and z equals the
+    oracle's.  (The reference uploads every call, train.cu:160-163.)"""
+    from gpupolyaurnlda_amd._lib import check, lib
+    from gpupolyaurnlda_amd.train import Sampler
+    rng = np.random.default_rng(31)
+    V, K, D, L = 3000, 64, 60, 300
+    n = D * L
+    wA = rng.integers(0, V, n).astype(np.uint32)
+    wB = wA.copy()
+    wB[1] = (wB[1] + 1) % V  # off the sampled fingerprint's grid
+    zA = rng.integers(0, K, n).astype(np.uint32)
+    zB = rng.integers(0, K, n).astype(np.uint32)
+    d = np.full(D, L, np.uint32)
+    w_all = np.concatenate([wA, wB])
+    C_counts = np.bincount(w_all, minlength=V).astype(np.uint32)
+    off = np.arange(0, 2 * n + 1, L, dtype=np.uint64)
+    model = orc.Model(w_all, off, np.concatenate([zA, zB]), V, K, 0.1, 0.01, seed=6, counts="C", threads=8)
+    model.phi_step()
+    model.z_step(doc_base=0, lo=0, hi=D)
+    model.z_step(doc_base=D, lo=D, hi=2 * D)
+    wbuf, zbuf = wA.copy(), zA.copy()
+    kd = np.zeros(D, np.uint32)
+    check(lib().gpulda_set_slice_cache(mode), "set_slice_cache")
+    try:
+        with Sampler(0.1, 0.01, K, C_counts, n, D, seed=6) as smp:
+            smp.sample_phi()
+            smp.set_buffer(0, zbuf, wbuf, d, kd)
+            smp.sample_z_async(0)
+            smp.sync_buffer(0)
+            out_a = zbuf.copy()
+            b0 = _cache_stats()[2]
+            wbuf[:] = wB
+            zbuf[:] = zB
+            smp.sample_z_async(0)
+            smp.sync_buffer(0)
+            assert _cache_stats()[2] == b0 + 1  # w uploaded again
+    finally:
+        check(lib().gpulda_set_slice_cache(1), "set_slice_cache")
+    assert np.array_equal(out_a, model.z[:n])
+    assert np.array_equal(zbuf, model.z[n:])
+
+
+def test_slice_cache_tiny_budget_two_buffers_in_flight(orc):
+    """A one-byte cache budget with two buffers in flight: each new slice evicts every cached slice
+    except the one the other buffer is still sweeping, so slices are uploaded again and again, and
+    z equals the oracle replaying the one-buffer loop of train.rs."""
+    from gpupolyaurnlda_amd._lib import check, lib
+    from gpupolyaurnlda_amd.corpus import TINY
+    from gpupolyaurnlda_amd.train import Sampler, fill_buffer
+    c = _tiny_corpus()
+    w, d, z = c["w"], c["d_len"], c["z"].copy()
+    buffer_size, max_D, n_mc = 3000, 40, 3
+    k_d = np.zeros(len(d), np.uint32)
+    model = orc.Model(w, c["doc_off"], c["z"], TINY.V, TINY.K, TINY.alpha, TINY.beta, seed=8, counts="C", threads=8)
+    check(lib().gpulda_set_slice_cache_budget(1), "budget")
+    try:
+        with Sampler(TINY.alpha, TINY.beta, TINY.K, c["C"], buffer_size, max_D, n_buffers=2, seed=8) as smp:
+            b0 = _cache_stats()[2]
+            n_calls = 0
+            for _ in range(n_mc):
+                smp.sample_phi()
+                doc = tok = 0
+                busy = [None, None]
+                slot = 0
+                while True:
+                    nd, nt = fill_buffer(d, doc, buffer_size, max_D)
+                    if nd == 0:
+                        break
+                    if busy[slot] is not None:
+                        smp.sync_buffer(slot)
+                    smp.set_buffer(slot, z[tok:tok + nt], w[tok:tok + nt], d[doc:doc + nd], k_d[doc:doc + nd])
+                    smp.sample_z_async(slot)
+                    busy[slot] = True
+                    n_calls += 1
+                    doc += nd
+                    tok += nt
+                    slot ^= 1
+                for s in (0, 1):
+                    if busy[s] is not None:
+                        smp.sync_buffer(s)
+                segs, _, builds = _cache_stats()
+                assert segs <= 2
+            assert builds - b0 == n_calls  # nothing was kept: every call uploaded its slice
+    finally:
+        check(lib().gpulda_set_slice_cache_budget(0), "budget")
+    for _ in range(n_mc):
+        model.phi_step()
+        doc = 0
+        while True:
+            nd, _ = fill_buffer(d, doc, buffer_size, max_D)
+            if nd == 0:
+                break
+            model.z_step(doc_base=doc, lo=doc, hi=doc + nd)
+            doc += nd
+        model.n = orc.count(model.w, model.z, TINY.V, TINY.K)
+    assert np.array_equal(z, model.z)
