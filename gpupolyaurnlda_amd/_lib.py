@@ -127,6 +127,7 @@ _SIGS = {
     "gpulda_stage_count": (C.c_int, [u32p, u32p, C.c_uint64, C.c_uint32, C.c_uint32, u32p]),
     "gpulda_stage_foldin": (C.c_int, [u32p, u64p, C.c_uint32, u32p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
                                       f32p, f32p, f32p, u32p, u32p, u64p]),
+    "gpulda_foldin_noalloc": (C.c_int, [u32p, u64p, C.c_uint32, u32p, C.c_uint32, u32p, u64p]),
     "gpulda_serve_open": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
     "gpulda_serve_submit": (C.c_int, [u32p, C.c_uint32, C.c_uint32, u32p]),
     "gpulda_serve_flush": (C.c_int, []),

@@ -417,6 +417,32 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
 }
 
 // ---------------------------------------------------------------------------------------------
+// Working memory of the allocation-free entry point (gpulda_foldin_noalloc): one static device
+// array per GPU, held until the process ends.  Host code never uses the array's name as an address
+// (on the host it is only a stand-in); its device address on each GPU comes from
+// hipGetSymbolAddress, taken once per device with that device current and kept here.
+// ---------------------------------------------------------------------------------------------
+constexpr u64 kNoAllocScratchBytes = 16ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on its first use there).
+int scratch_on_current_device(unsigned char** out) {
+  int dev = 0;
+  HCK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail("device %d beyond the %d this library tracks", dev, kMaxDevices);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HCK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail("no device address for the working-memory array");
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Library state.
 // ---------------------------------------------------------------------------------------------
 struct BufferState {
@@ -837,6 +863,10 @@ int do_initialize(Args* args, Buffer* buffers, u32 n_buffers) {
     int dev = 0;
     HCK(hipGetDevice(&dev));
     HCK(hipDeviceGetAttribute(&S.cu_count, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  {
+    unsigned char* scr = nullptr;
+    CK(scratch_on_current_device(&scr));  // set-up of the allocation-free entry point
   }
   CK(dalloc(&S.iter_dev, 1));
   CK(dalloc(&S.heads, 2));
@@ -2261,5 +2291,108 @@ extern "C" int gpulda_stage_foldin(const uint32_t* w, const uint64_t* doc_off, u
   if (N) HCK(hipMemcpy(hz.data(), dz, N * 2, hipMemcpyDeviceToHost));
   for (u64 i = 0; i < N; ++i) z_out[i] = hz[i];
   if (visited) HCK(hipMemcpy(visited, dvis, 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// ---- allocation-free fold-in ------------------------------------------------------------------
+namespace {
+__global__ void k_span_lengths(const u64* __restrict__ off, u32 n, u32* __restrict__ len) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) len[i] = (u32)(off[i + 1] - off[i]);
+}
+
+// Carves 256-byte aligned pieces off the working-memory array.
+struct Carver {
+  unsigned char* base;
+  u64 used = 0;
+  template <class T>
+  T* take(u64 count) {
+    T* p = reinterpret_cast<T*>(base + used);
+    used += (count * sizeof(T) + 255) & ~(u64)255;
+    return p;
+  }
+};
+
+u64 noalloc_bytes(u64 n_tokens, u64 n_req) {
+  auto r = [](u64 b) { return (b + 255) & ~(u64)255; };
+  return r(n_tokens * 4) + r(n_tokens * 2) + r((n_req + 1) * 8) + r(n_req * 4) + r(n_req * 4) + r(8);
+}
+}  // namespace
+
+// The fold-in of gpulda_stage_foldin (same kernel, same results) against the library's current
+// model -- phi, sigma_a and the alias tables of the last Phi step, and the library's seed -- with
+// no device allocation: words, topics, offsets, lengths, request ids and the visit counter are
+// staged through the static working-memory array.  Host arrays in and out; synchronous.
+extern "C" int gpulda_foldin_noalloc(const uint32_t* w, const uint64_t* doc_off, uint32_t n_req,
+                                     const uint32_t* req_ids, uint32_t sweeps, uint32_t* z_out, uint64_t* visited) {
+  CK(require_ready());
+  if (S.iter == 0) return fail("foldin_noalloc needs a trained model (run at least one Phi step)");
+  if (!doc_off || (n_req && (!req_ids || !z_out))) return fail("foldin_noalloc: missing array");
+  const u64 N = doc_off[n_req];
+  u32 maxL = 0;
+  for (u32 r = 0; r < n_req; ++r) {
+    if (doc_off[r + 1] < doc_off[r]) return fail("request offsets decrease at %u", r);
+    const u64 L = doc_off[r + 1] - doc_off[r];
+    if (L > kMaxDocLen) return fail("request %u longer than %u tokens", r, kMaxDocLen);
+    maxL = std::max(maxL, (u32)L);
+  }
+  if (doc_off[0] != 0) return fail("request offsets must start at 0");
+  if (N && !w) return fail("foldin_noalloc: missing words");
+  for (u64 i = 0; i < N; ++i)
+    if (w[i] >= S.V) return fail("word id %u at token %llu is >= V = %u", w[i], (unsigned long long)i, S.V);
+  const u64 need = noalloc_bytes(N, n_req);
+  if (need > kNoAllocScratchBytes)
+    return fail("foldin_noalloc: %llu tokens in %u requests need %llu bytes of working memory (limit %llu: 6 bytes "
+                "per token, 16 per request)",
+                (unsigned long long)N, n_req, (unsigned long long)need, (unsigned long long)kNoAllocScratchBytes);
+  const u64 lds = ((u64)map_capacity(maxL, S.K) + (S.K + 31) / 32) * 4;
+  if (lds > (u64)max_lds_per_block()) return fail("doc-topic map exceeds LDS");
+  unsigned char* base = nullptr;
+  CK(scratch_on_current_device(&base));
+  Carver cv{base};
+  u32* dw = cv.take<u32>(N);
+  u16* dz = cv.take<u16>(N);
+  u64* doff = cv.take<u64>((u64)n_req + 1);
+  u32* dlen = cv.take<u32>(n_req);
+  u32* drid = cv.take<u32>(n_req);
+  unsigned long long* dvis = cv.take<unsigned long long>(1);
+  hipStream_t st = S.stream;
+  if (N) HCK(hipMemcpyAsync(dw, w, N * sizeof(u32), hipMemcpyHostToDevice, st));
+  HCK(hipMemcpyAsync(doff, doc_off, ((u64)n_req + 1) * sizeof(u64), hipMemcpyHostToDevice, st));
+  if (n_req) {
+    HCK(hipMemcpyAsync(drid, req_ids, (u64)n_req * sizeof(u32), hipMemcpyHostToDevice, st));
+    k_span_lengths<<<(n_req + 255) / 256, 256, 0, st>>>(doff, n_req, dlen);
+    HCK(hipGetLastError());
+  }
+  CK(fill_u64(dvis, 1, 0, st));  // the array keeps the previous call's contents
+  SweepParams P = serve_params();
+  P.w = dw;
+  P.z = dz;
+  P.n_docs = n_req;
+  P.visited = dvis;
+  FoldParams F;
+  F.slots = nullptr;
+  F.n_slots = n_req;
+  F.span_off = doff;
+  F.span_len = dlen;
+  F.req_id = drid;
+  F.sweeps = sweeps;
+  F.z_out = nullptr;
+  F.done = nullptr;
+  F.done_value = 0;
+  if (n_req) {
+    k_foldin<8><<<n_req, 64, lds, st>>>(P, F);
+    HCK(hipGetLastError());
+  }
+  if (N) {
+    // the words are no longer needed: their space takes the topics widened to u32
+    k_z_widen<<<grid_for(N, 256), 256, 0, st>>>(dz, dw, N);
+    HCK(hipGetLastError());
+    HCK(hipMemcpyAsync(z_out, dw, N * sizeof(u32), hipMemcpyDeviceToHost, st));
+  }
+  unsigned long long hv = 0;
+  HCK(hipMemcpyAsync(&hv, dvis, sizeof(hv), hipMemcpyDeviceToHost, st));
+  HCK(hipStreamSynchronize(st));
+  if (visited) *visited = hv;
   return 0;
 }

@@ -63,6 +63,20 @@ class FoldInServer:
         self.close()
 
 
+def foldin_noalloc(w, doc_off, req_ids, sweeps: int):
+    """Fold-in of each request (documents of w delimited by doc_off) against the library's current
+    model, through the allocation-free entry point.  Returns (topics, map entries visited)."""
+    w = np.ascontiguousarray(w, dtype=np.uint32)
+    doc_off = np.ascontiguousarray(doc_off, dtype=np.uint64)
+    req_ids = np.ascontiguousarray(req_ids, dtype=np.uint32)
+    z = np.empty(len(w), dtype=np.uint32)
+    visited = np.zeros(1, dtype=np.uint64)
+    check(lib().gpulda_foldin_noalloc(ptr(w, C.c_uint32), ptr(doc_off, C.c_uint64), len(req_ids),
+                                      ptr(req_ids, C.c_uint32), sweeps, ptr(z, C.c_uint32),
+                                      ptr(visited, C.c_uint64)), "foldin_noalloc")
+    return z, int(visited[0])
+
+
 def _split(d_len, w):
     off = np.zeros(len(d_len) + 1, dtype=np.int64)
     np.cumsum(d_len, out=off[1:])

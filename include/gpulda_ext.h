@@ -172,6 +172,21 @@ int gpulda_stage_foldin(const uint32_t* w, const uint64_t* doc_off, uint32_t n_r
                         const float* sigma_a, const float* prob, const uint32_t* alias, uint32_t* z_out,
                         uint64_t* visited);
 
+/* ---- allocation-free fold-in ---------------------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc): the fold-in of gpulda_stage_foldin (same kernel, same
+ * results) against the library's current model -- phi, sigma_a and the alias tables of the last
+ * Phi step, and the seed set by gpulda_set_seed -- that calls no hipMalloc, hipFree or other
+ * allocating routine.  Words, topics, offsets and counters are staged through one static device
+ * array of GPULDA_NOALLOC_SCRATCH_BYTES on each GPU, held until the process ends; a call needing
+ * more (6 bytes per token + 16 per request, each piece rounded to 256 bytes) is rejected with an
+ * error.  Every call on a GPU shares that GPU's array: calls that overlap on one GPU race.
+ * Requests are the documents of w delimited by doc_off (doc_off[0] = 0, n_req + 1 offsets);
+ * z_out receives their topics, *visited the map entries the sparse sums read. */
+#define GPULDA_NOALLOC_SCRATCH_BYTES (16u << 20)
+int gpulda_foldin_noalloc(const uint32_t* w, const uint64_t* doc_off, uint32_t n_req, const uint32_t* req_ids,
+                          uint32_t sweeps, uint32_t* z_out, uint64_t* visited);
+
 #ifdef __cplusplus
 }
 #endif
