@@ -2053,7 +2053,7 @@ extern "C" int gpulda_stage_pick(const uint32_t* table, uint32_t cap, uint32_t K
   CK(sc.upload(&duw, u_target, count));
   CK(sc.alloc(&dout, count));
   if (count) {
-    k_stage_pick<<<count, 64, (u64)cap * 4>>>(dt, cap, drow, sigma_a, dux, duw, count, use_regs ? 1u : 0u, dout);
+    k_stage_pick<<<count, 64, (u64)cap * 4>>>(dt, cap, K, drow, sigma_a, dux, duw, count, use_regs ? 1u : 0u, dout);
     HCK(hipGetLastError());
     HCK(hipMemcpy(slot_out, dout, (u64)count * 4, hipMemcpyDeviceToHost));
   }

@@ -116,8 +116,8 @@ __global__ void k_stage_scan(const f32* in, f32* out, u32 n_groups);
 __global__ void k_stage_map(const u32* ops, u32 n_ops, u32 cap, u32 K, u32* out);
 __global__ void k_stage_poisson(const u32* nval, u32 count, f32 beta, const AliasSlot* tables, u32 c1, u32 c2,
                                 u32 tag, u32 k0, u32 k1, u32* out);
-__global__ void k_stage_pick(const u32* table, u32 cap, const f32* prow, f32 sa, const f32* ux, const f32* uw,
-                             u32 count, u32 use_regs, u32* out);
+__global__ void k_stage_pick(const u32* table, u32 cap, u32 K, const f32* phi_row_k, f32 sa, const f32* ux,
+                             const f32* uw, u32 count, u32 use_regs, u32* out);
 __global__ void k_stage_alias_draw(const AliasSlot* row, u32 m, const u32* r_slot, const u32* r_keep, u32 count,
                                    u32* out);
 

@@ -262,12 +262,31 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
 // ---------------------------------------------------------------------------------------------
 constexpr u32 kNoSlot = 0xFFFFFFFFu;
 
-// Mass of one map entry for row prow.  An empty slot reads as count 0 (at a topic below K), so the
-// gather and the product are unconditional (no branch, all gathers of a token in flight together)
-// and its mass is 0 * phi[w][key] = +0, the value the definition gives it.
-__device__ __forceinline__ f32 slot_mass(u32 e, const f32* __restrict__ prow) {
-  static_assert(kMapEmpty == 0u, "reads past a map read count 0 at key 0");
-  return (f32)map_count(e) * prow[map_key(e)];
+// The Phi row of a token's word as a buffer resource of K floats: a read at a topic past K (the key
+// of an empty slot) returns 0 without touching memory, and the address path of a buffer load (a
+// 32-bit offset under a scalar base) processes a wavefront's scattered reads faster than a global
+// load does.
+struct PhiRow {
+  __amdgpu_buffer_rsrc_t rs;
+};
+constexpr int kBufferWord3 = 0x00020000;  // raw 32-bit buffer access (gfx9 family)
+
+__device__ __forceinline__ PhiRow phi_row(const f32* prow, u32 K) {
+  PhiRow r;
+  r.rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<f32*>(prow), (short)0, (int)(K * sizeof(f32)), kBufferWord3);
+  return r;
+}
+
+__device__ __forceinline__ f32 row_at(const PhiRow& row, u32 key) {
+  return __builtin_bit_cast(f32, __builtin_amdgcn_raw_buffer_load_b32(row.rs, (int)(key * sizeof(f32)), 0, 0));
+}
+
+// Mass of one map entry.  An empty slot holds count 0 at key 0xFFFF (kMapEmpty), so the read and the
+// product are unconditional (no branch, all reads of a token in flight together): the read returns 0
+// without a memory access, and the mass is 0 * 0 = +0, the value the definition gives it.
+__device__ __forceinline__ f32 slot_mass(u32 e, const PhiRow& row) {
+  static_assert((kMapEmpty & 0xFFFFu) == 0u && (kMapEmpty >> 16) == 0xFFFFu, "empty slots read past the row");
+  return (f32)map_count(e) * row_at(row, map_key(e));
 }
 
 __device__ __forceinline__ f32 combine_groups(f32 g0, f32 g1, f32 g2, f32 g3) { return ((g0 + g1) + g2) + g3; }
@@ -298,8 +317,7 @@ __device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u32* bits, 
                                               const u16* __restrict__ z, u64 b, u32 L) {
   const u32 lane = lane_id();
   const u32 hm = map_home_mult(K);
-  const u32 eq = map_empty_q(K, cap);
-  for (u32 s = lane; s < cap; s += 64) T[s] = map_empty(s, eq);
+  for (u32 s = lane; s < cap; s += 64) T[s] = kMapEmpty;
   // 64 bitmap words per step; only words with a topic set are visited
   const u32 words = (K + 31) / 32;
   for (u32 w0 = 0; w0 < words; w0 += 64) {
@@ -353,9 +371,8 @@ __device__ __noinline__ void regrow_doc_map(u32* T, u32 ncap, u32 K, const u16* 
                                             u32 c0, u32 t, u32 znew_lane, u32 zz) {
   const u32 lane = lane_id();
   const u32 hm = map_home_mult(K);
-  const u32 eq = map_empty_q(K, ncap);
   wave_sync();
-  for (u32 s = lane; s < ncap; s += 64) T[s] = map_empty(s, eq);
+  for (u32 s = lane; s < ncap; s += 64) T[s] = kMapEmpty;
   for (u32 c = 0; c < L; c += 64) {
     u32 nt = L - c < 64 ? L - c : 64;
     u32 v;
@@ -376,7 +393,6 @@ struct DocState {
   u32 occupied;   // distinct topics in the map
   u32 gdoc;       // global document id (Philox counter word 1)
   u32 hm;         // map_home_mult(K)
-  u32 eq;         // map_empty_q(K, cap)
 };
 
 // Capacity rule: start sized for the n0 distinct topics; grow (rebuild) when a new topic would
@@ -391,7 +407,6 @@ __device__ __forceinline__ DocState open_span(const SweepParams& P, u64 b, u32 L
   ds.hm = map_home_mult(P.K);
   ds.occupied = uniform_u32(distinct_topics(T + ds.cap_max, P.K, P.z, ds.b, ds.L));
   ds.cap = map_capacity(ds.occupied, P.K);
-  ds.eq = map_empty_q(P.K, ds.cap);
   build_doc_map(T, ds.cap, T + ds.cap_max, P.K, ds.occupied, P.z, ds.b, ds.L);
   return ds;
 }
@@ -420,7 +435,7 @@ __device__ __forceinline__ u32 pick_lane(f32 A, f32 P, f32 tgt, f32& E) {
 
 // In-column search over lines read transposed (lane k holds line c0 + k of column ls), one scan
 // per 64 lines.  Returns the picked line.
-__device__ __forceinline__ u32 column_pick(const u32* T, u32 lines, u32 ls, const f32* __restrict__ prow, f32 E,
+__device__ __forceinline__ u32 column_pick(const u32* T, u32 lines, u32 ls, const PhiRow& prow, f32 E,
                                            f32 tgt) {
   const u32 lane = lane_id();
   f32 carry = 0.0f;
@@ -441,7 +456,7 @@ __device__ __forceinline__ u32 column_pick(const u32* T, u32 lines, u32 ls, cons
 // Sparse/dense pick with the NL lines of the map held in registers (one variant per line count):
 // all phi gathers of the token are issued before any arithmetic on them.
 template <int NL>
-__device__ __forceinline__ Pick draw_topic_regs(const u32* T, const f32* __restrict__ prow, f32 sa, f32 ux, f32 uw,
+__device__ __forceinline__ Pick draw_topic_regs(const u32* T, const PhiRow& prow, f32 sa, f32 ux, f32 uw,
                                                 u32 dense) {
   const u32 lane = lane_id();
   u32 e[NL];
@@ -502,7 +517,7 @@ __device__ __forceinline__ Pick draw_topic_regs(const u32* T, const f32* __restr
 
 // Sparse/dense pick for any number of lines: one pass for the column sums, then for a sparse draw
 // the in-column search over column l* read transposed.
-__device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa,
+__device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const PhiRow& prow, f32 sa,
                                                 f32 ux, f32 uw, u32 dense) {
   const u32 lane = lane_id();
   f32 G0 = 0.0f, G1 = 0.0f, G2 = 0.0f, G3 = 0.0f;
@@ -557,7 +572,7 @@ __device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const f
 // use the line-loop form.  All variants evaluate the same operations in the same order: a
 // variant for fewer lines drops only lines that would hold no mass.
 template <int LMAX>
-__device__ __forceinline__ Pick draw_topic(const u32* T, u32 lines, const f32* __restrict__ prow, f32 sa, f32 ux,
+__device__ __forceinline__ Pick draw_topic(const u32* T, u32 lines, const PhiRow& prow, f32 sa, f32 ux,
                                            f32 uw, u32 dense) {
   switch (lines) {  // a map always has at least one line
     case 1: if (LMAX >= 1) return draw_topic_regs<1>(T, prow, sa, ux, uw, dense); break;
@@ -587,8 +602,7 @@ __device__ __forceinline__ void add_topic(const SweepParams& P, u32* T, DocState
     if (map_capacity(ds.occupied + 1, P.K) > ds.cap) {
       const u32 want = 2 * (ds.occupied + 1) < ds.kmax ? 2 * (ds.occupied + 1) : ds.kmax;
       ds.cap = map_capacity(want, P.K);
-      ds.eq = map_empty_q(P.K, ds.cap);
-      regrow_doc_map(T, ds.cap, P.K, P.z, ds.b, ds.L, c0, t, znew_lane, zz);
+          regrow_doc_map(T, ds.cap, P.K, P.z, ds.b, ds.L, c0, t, znew_lane, zz);
       pz = map_probe(T, ds.cap, ds.hm, pk.topic);
     }
     ++ds.occupied;
@@ -621,7 +635,7 @@ __device__ __forceinline__ void sweep_pass(const SweepParams& P, u32* T, DocStat
     for (u32 t = 0; t < nt; ++t) {
       const u32 wt = readlane_u32(ww, t);
       const u32 zt = readlane_u32(zz, t);
-      ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, ds.eq, zt));
+      ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, zt));
       const f32 sa = readlane_f32(sa_l, t);
       Pick pk;
       pk.topic = zt;
@@ -629,7 +643,7 @@ __device__ __forceinline__ void sweep_pass(const SweepParams& P, u32* T, DocStat
       if (sa > 0.0f) {
         wave_sync();
         visited += ds.occupied;
-        pk = draw_topic<LMAX>(T, uniform_u32(ds.cap / kMapLine), P.phi + (u64)wt * K, sa, u24(readlane_u32(r.x, t)),
+        pk = draw_topic<LMAX>(T, uniform_u32(ds.cap / kMapLine), phi_row(P.phi + (u64)wt * K, K), sa, u24(readlane_u32(r.x, t)),
                               u24(readlane_u32(r.w, t)), readlane_u32(dense_l, t));
       }
       add_topic(P, T, ds, pk, c0, t, znew_lane, zz);
@@ -647,8 +661,7 @@ __device__ __forceinline__ void shrink_doc_map(const SweepParams& P, u32* T, Doc
   const u32 want = map_capacity(ds.occupied, P.K);
   if (want < ds.cap) {
     ds.cap = want;
-    ds.eq = map_empty_q(P.K, ds.cap);
-    wave_sync();
+      wave_sync();
     const u32 distinct = uniform_u32(distinct_topics(T + ds.cap_max, P.K, P.z, ds.b, ds.L));
     build_doc_map(T, ds.cap, T + ds.cap_max, P.K, distinct, P.z, ds.b, ds.L);
   }
@@ -693,8 +706,13 @@ template __global__ void k_sweep_lines<8>(SweepParams);
 // first) through an atomic head.  Every wavefront leaves once the head passes n_items, so the
 // grid always drains.  Results are identical to the per-bucket launches: the randomness of a
 // document depends on its global id only.
+// Registers are budgeted for eight wavefronts per SIMD (the compiler would otherwise stop at seven
+// for its scalar registers, and spilling some of them to vector lanes costs less than the lost
+// wavefront: the sweep waits on its gathers and LDS more than it issues).
 template <int LMAX>
-__global__ __launch_bounds__(64) void k_sweep_persistent(SweepParams P, u32* head, u32 n_items) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_sweep_persistent(SweepParams P,
+                                                                                                u32* head,
+                                                                                                u32 n_items) {
   extern __shared__ u32 sweep_smem[];
   for (;;) {
     u32 item = 0;
@@ -764,7 +782,7 @@ __global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u
         const u32 zt = readlane_u32(zz, t);
         const f32 sa = readlane_f32(sa_l, t);
         if (wave == 0) {
-          ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, ds.eq, zt));
+          ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, zt));
           if (lane == 0) sh_cap = ds.cap;
         }
         __syncthreads();
@@ -774,7 +792,7 @@ __global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u
         pk.entry = 0;
         if (sa > 0.0f) {
           const u32 lines = sh_cap / kMapLine;
-          const f32* prow = P.phi + (u64)wt * K;
+          const PhiRow prow = phi_row(P.phi + (u64)wt * K, K);
           // wave g sums line group g of every column (four of its lines per step, gathers in
           // flight together, added in line order)
           f32 G = 0.0f;
@@ -1086,11 +1104,10 @@ __global__ void k_stage_map(const u32* ops, u32 n_ops, u32 cap, u32 K, u32* out)
   extern __shared__ u32 map_smem[];
   const u32 lane = lane_id();
   const u32 hm = map_home_mult(K);
-  const u32 eq = map_empty_q(K, cap);
-  for (u32 s = lane; s < cap; s += 64) map_smem[s] = map_empty(s, eq);
+  for (u32 s = lane; s < cap; s += 64) map_smem[s] = kMapEmpty;
   for (u32 i = 0; i < n_ops; ++i) {
     u32 op = ops[i];
-    if (op >> 31) map_dec(map_smem, cap, hm, eq, op & 0x7FFFFFFFu);
+    if (op >> 31) map_dec(map_smem, cap, hm, op & 0x7FFFFFFFu);
     else map_inc(map_smem, cap, hm, op);
   }
   wave_sync();
@@ -1101,14 +1118,15 @@ __global__ void k_stage_map(const u32* ops, u32 n_ops, u32 cap, u32 K, u32* out)
 // made with the uniforms ux[i], uw[i] by the sweep's own device functions -- the register
 // variants when use_regs (maps of up to eight lines), else the line-loop form.  out[i] is the
 // picked slot, or kNoSlot for a dense draw.
-__global__ __launch_bounds__(64) void k_stage_pick(const u32* table, u32 cap, const f32* prow, f32 sa, const f32* ux,
-                                                   const f32* uw, u32 count, u32 use_regs, u32* out) {
+__global__ __launch_bounds__(64) void k_stage_pick(const u32* table, u32 cap, u32 K, const f32* phi_row_k, f32 sa,
+                                                   const f32* ux, const f32* uw, u32 count, u32 use_regs, u32* out) {
   extern __shared__ u32 pick_smem[];
   const u32 i = blockIdx.x;
   if (i >= count) return;
   for (u32 s = lane_id(); s < cap; s += 64) pick_smem[s] = table[s];
   wave_sync();
   const u32 lines = uniform_u32(cap / kMapLine);
+  const PhiRow prow = phi_row(phi_row_k, K);
   const Pick pk = use_regs ? draw_topic<8>(pick_smem, lines, prow, sa, ux[i], uw[i], kNoSlot)
                            : draw_topic<0>(pick_smem, lines, prow, sa, ux[i], uw[i], kNoSlot);
   if (lane_id() == 0) out[i] = pk.slot;

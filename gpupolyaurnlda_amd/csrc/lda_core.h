@@ -109,9 +109,11 @@ LDA_HD u32 bounded(u32 r, u32 n) { return (u32)(((u64)r * (u64)n) >> 32); }
 // backward shift, so its layout is a function of the current key set only.
 // ---------------------------------------------------------------------------------------------
 // An empty slot holds count 0 (a stored entry has a count of at least 1), so its mass
-// count * phi[w][key] is +0 whatever its key field, with no test.  kMapEmpty, count 0 at key 0,
-// stands for reads past a map's last line; the slots of a map hold map_empty(s, q) below.
-constexpr u32 kMapEmpty = 0u;
+// count * phi[w][key] is +0 whatever its key field, with no test.  Every empty slot -- and every
+// read past a map's last line -- is kMapEmpty: count 0 at key 0xFFFF, beyond the K <= 65535 topics
+// of any row, so the sweep's row reads (buffer loads bounded by the row) skip it without a memory
+// access.  The content is the same for every empty slot, so a table stays a function of its keys.
+constexpr u32 kMapEmpty = 0xFFFF0000u;
 constexpr u32 kMapLine = 64;          // one wavefront probes 64 consecutive slots
 constexpr u32 kMaxDocLen = 65535;     // counts are 16-bit
 constexpr u32 kMaxTopics = 65536;     // keys are 16-bit
@@ -126,15 +128,6 @@ LDA_HD u32 map_key(u32 e) { return e >> 16; }
 LDA_HD u32 map_count(u32 e) { return e & 0xFFFFu; }
 LDA_HD u32 map_entry(u32 key, u32 count) { return (key << 16) | count; }
 LDA_HD bool map_is_empty(u32 e) { return map_count(e) == 0u; }
-
-// Content of empty slot s of a map of capacity cap over K topics: count 0 at the topic
-// (s * q) >> 16, q = floor(K * 2^16 / cap), the topic whose home is near s.  With homes ordered
-// by topic, that is close to the keys of the neighbouring slots, so the (zero-mass) gather of an
-// empty slot falls in the cache lines its neighbours gather anyway (key 0 would add a line to
-// every group of lanes holding an empty slot).  s * q < K * 2^16 <= 2^32 for s < cap, and the
-// content depends on the slot alone, so the table stays a function of its key set.
-LDA_HD u32 map_empty_q(u32 K, u32 cap) { return (u32)(((u64)K << 16) / cap); }
-LDA_HD u32 map_empty(u32 s, u32 q) { return (s * q) & 0xFFFF0000u; }
 
 // Home slot of a topic below K: key * hm with hm = floor((2^32 - 1) / K) stays below 2^32, and the
 // multiply-shift maps it to [0, C), non-decreasing in the key.  A line of the table then holds a

@@ -93,8 +93,8 @@ __device__ __forceinline__ u64 wave_scan_u64(u64 v) {
 
 // ---------------------------------------------------------------------------------------------
 // Doc-topic map operations.  T points at `cap` LDS slots (cap a multiple of 64, never full); hm is
-// map_home_mult(K) for keys below K, eq is map_empty_q(K, cap).  A slot is empty when its count is
-// 0.  All 64 lanes of the wave call these with the same (uniform) arguments.
+// map_home_mult(K) for keys below K.  A slot is empty when its count is 0 (it then holds
+// kMapEmpty).  All 64 lanes of the wave call these with the same (uniform) arguments.
 // ---------------------------------------------------------------------------------------------
 // x mod cap for x < 2 * cap: x - cap wraps past 2^32 when x < cap, so the minimum is the slot.
 __device__ __forceinline__ u32 wrap_slot(u32 x, u32 cap) { return min(x, x - cap); }
@@ -217,7 +217,7 @@ __device__ __forceinline__ u32 map_inc(u32* T, u32 cap, u32 hm, u32 key) {
 
 // Count of `key` -1 (the key must be present); deletes it by backward shift at zero.
 // Returns 1 if the key was removed.
-__device__ __forceinline__ u32 map_dec(u32* T, u32 cap, u32 hm, u32 eq, u32 key) {
+__device__ __forceinline__ u32 map_dec(u32* T, u32 cap, u32 hm, u32 key) {
   const u32 lane = lane_id();
   MapProbe p = map_find(T, cap, hm, key);
   if (!p.found) return 0;  // not reached: every decremented topic is in the map
@@ -234,7 +234,7 @@ __device__ __forceinline__ u32 map_dec(u32* T, u32 cap, u32 hm, u32 eq, u32 key)
     u64 bs = ballot(map_is_empty(e)) | ballot(map_disp(s, map_home(map_key(e), cap, hm), cap) == 0);
     u32 fs = bs ? first_lane(bs) : 64;
     u32 dst = s == 0 ? cap - 1 : s - 1;
-    if (lane <= fs) T[dst] = lane < fs ? e : map_empty(dst, eq);
+    if (lane <= fs) T[dst] = lane < fs ? e : kMapEmpty;
     if (bs) break;
   }
   return 1;

@@ -333,12 +333,10 @@ void orc_phi_finish(float* phi /* in: phi~, out: phi */, const u64* topic_sum, u
 /* Doc-topic map: textbook sequential Robin Hood linear probing, ties broken by key, backward  */
 /* shift deletion.  Restates the HashMap of hashmap.cuh:13-334 (entry = key<<16 | count).     */
 /* ------------------------------------------------------------------------------------------ */
-/* A stored entry has count >= 1; an empty slot s holds count 0 at the topic (s * q) >> 16,
- * q = floor(K * 2^16 / cap) -- a function of the slot alone (its mass 0 * phi is +0 whatever the
- * topic). */
+/* A stored entry has count >= 1; every empty slot holds count 0 at key 0xFFFF (its mass 0 * phi
+ * is +0 whatever the key). */
 #define IS_EMPTY(e) (((e) & 0xFFFFu) == 0u)
-u32 orc_map_empty_q(u32 K, u32 cap) { return (u32)(((u64)K << 16) / cap); }
-static u32 empty_of(u32 s, u32 q) { return (s * q) & 0xFFFF0000u; }
+#define EMPTY_SLOT 0xFFFF0000u
 
 u32 orc_map_capacity(u32 L, u32 K) {
   u32 need = L < K ? L : K;
@@ -406,15 +404,15 @@ static void map_put(u32* T, u32 cap, u32 hm, u32 cur) {
 /* Moves every entry of a table of capacity `cap` into a cleared table of capacity `ncap`
  * (same array, ncap >= cap). */
 static void map_regrow(u32* T, u32 cap, u32 ncap, u32 K, u32* scratch) {
-  const u32 hm = orc_map_home_mult(K), q = orc_map_empty_q(K, ncap);
+  const u32 hm = orc_map_home_mult(K);
   u32 n = 0;
   for (u32 s = 0; s < cap; ++s)
     if (!IS_EMPTY(T[s])) scratch[n++] = T[s];
-  for (u32 s = 0; s < ncap; ++s) T[s] = empty_of(s, q);
+  for (u32 s = 0; s < ncap; ++s) T[s] = EMPTY_SLOT;
   for (u32 i = 0; i < n; ++i) map_put(T, ncap, hm, scratch[i]);
 }
 
-void orc_map_dec(u32* T, u32 cap, u32 hm, u32 q, u32 key) {
+void orc_map_dec(u32* T, u32 cap, u32 hm, u32 key) {
   long s = orc_map_find(T, cap, hm, key);
   if (s < 0) abort();
   if ((T[s] & 0xFFFFu) > 1) { T[s] -= 1; return; }
@@ -422,7 +420,7 @@ void orc_map_dec(u32* T, u32 cap, u32 hm, u32 q, u32 key) {
   for (;;) {
     u32 nx = (hole + 1) % cap;
     u32 e = T[nx];
-    if (IS_EMPTY(e) || dist(nx, home_of(e >> 16, cap, hm), cap) == 0) { T[hole] = empty_of(hole, q); return; }
+    if (IS_EMPTY(e) || dist(nx, home_of(e >> 16, cap, hm), cap) == 0) { T[hole] = EMPTY_SLOT; return; }
     T[hole] = e;
     hole = nx;
   }
@@ -467,8 +465,7 @@ static OrcDoc doc_open(const u32* z, u32 L, u32 K, u32* scratch) {
   d.hm = orc_map_home_mult(K);
   d.kmax = L < K ? L : K;
   u32 n0 = 0;
-  const u32 q_max = orc_map_empty_q(K, cap_max);
-  for (u32 s = 0; s < cap_max; ++s) d.T[s] = empty_of(s, q_max);
+  for (u32 s = 0; s < cap_max; ++s) d.T[s] = EMPTY_SLOT;
   for (u32 i = 0; i < L; ++i) {
     if (orc_map_find(d.T, cap_max, d.hm, z[i]) < 0) ++n0;
     orc_map_inc(d.T, cap_max, d.hm, z[i]);
@@ -559,7 +556,7 @@ static u64 doc_pass(OrcDoc* d, u32* z, const u32* w, u32 doc, u32 iter, u32 tag,
     draw4(i, doc, iter, tag, 0, seed, r);
     long at = orc_map_find(d->T, d->cap, d->hm, zi);
     if ((d->T[at] & 0xFFFFu) == 1) --d->keys;
-    orc_map_dec(d->T, d->cap, d->hm, orc_map_empty_q(K, d->cap), zi);
+    orc_map_dec(d->T, d->cap, d->hm, zi);
     float sa = sigma_a[wi];
     u32 znew = zi;
     if (sa > 0.0f) {
@@ -688,10 +685,10 @@ int orc_threads(void) {
 /* Map layout after building from a list of keys (below K) with increments (+1) and decrements
  * (-1), for the table-layout tests. ops[i] = key | (is_dec << 31). */
 void orc_map_apply(u32* T, u32 cap, u32 K, const u32* ops, u32 n_ops) {
-  const u32 hm = orc_map_home_mult(K), q = orc_map_empty_q(K, cap);
-  for (u32 s = 0; s < cap; ++s) T[s] = empty_of(s, q);
+  const u32 hm = orc_map_home_mult(K);
+  for (u32 s = 0; s < cap; ++s) T[s] = EMPTY_SLOT;
   for (u32 i = 0; i < n_ops; ++i) {
-    if (ops[i] >> 31) orc_map_dec(T, cap, hm, q, ops[i] & 0x7FFFFFFFu);
+    if (ops[i] >> 31) orc_map_dec(T, cap, hm, ops[i] & 0x7FFFFFFFu);
     else orc_map_inc(T, cap, hm, ops[i]);
   }
 }
