@@ -465,13 +465,17 @@ __device__ __forceinline__ Pick draw_topic_regs(const u32* T, const PhiRow& prow
   for (int j = 0; j < NL; ++j) e[j] = T[j * 64 + lane];
 #pragma unroll
   for (int j = 0; j < NL; ++j) m[j] = slot_mass(e[j], prow);
-  // G_g starts at line g's mass (0 + m == m, masses are never -0)
+  // G_g starts at line g's mass (0 + m == m, masses are never -0); with fewer than four lines the
+  // empty groups add +0, which leaves the sum's bits unchanged, so those adds are dropped
   f32 G[4];
 #pragma unroll
   for (int g = 0; g < 4; ++g) G[g] = g < NL ? m[g] : 0.0f;
 #pragma unroll
   for (int j = 4; j < NL; ++j) G[j & 3] = G[j & 3] + m[j];
-  const f32 A = combine_groups(G[0], G[1], G[2], G[3]);
+  f32 A = G[0];
+  if (NL >= 2) A = A + G[1];
+  if (NL >= 3) A = A + G[2];
+  if (NL >= 4) A = A + G[3];
   const f32 P = wave_scan_f32(A);
   const f32 sb = readlane_f32(P, 63);
   Pick pk;
@@ -567,27 +571,6 @@ __device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const P
   return pk;
 }
 
-// Sparse/dense pick for one token (sigma_a > 0).  The register variant is chosen by the map's
-// current line count (up to 8, within the kernel's LMAX, which caps its registers); larger maps
-// use the line-loop form.  All variants evaluate the same operations in the same order: a
-// variant for fewer lines drops only lines that would hold no mass.
-template <int LMAX>
-__device__ __forceinline__ Pick draw_topic(const u32* T, u32 lines, const PhiRow& prow, f32 sa, f32 ux,
-                                           f32 uw, u32 dense) {
-  switch (lines) {  // a map always has at least one line
-    case 1: if (LMAX >= 1) return draw_topic_regs<1>(T, prow, sa, ux, uw, dense); break;
-    case 2: if (LMAX >= 2) return draw_topic_regs<2>(T, prow, sa, ux, uw, dense); break;
-    case 3: if (LMAX >= 4) return draw_topic_regs<3>(T, prow, sa, ux, uw, dense); break;
-    case 4: if (LMAX >= 4) return draw_topic_regs<4>(T, prow, sa, ux, uw, dense); break;
-    case 5: if (LMAX >= 8) return draw_topic_regs<5>(T, prow, sa, ux, uw, dense); break;
-    case 6: if (LMAX >= 8) return draw_topic_regs<6>(T, prow, sa, ux, uw, dense); break;
-    case 7: if (LMAX >= 8) return draw_topic_regs<7>(T, prow, sa, ux, uw, dense); break;
-    case 8: if (LMAX >= 8) return draw_topic_regs<8>(T, prow, sa, ux, uw, dense); break;
-    default: break;
-  }
-  return draw_topic_loop(T, lines, prow, sa, ux, uw, dense);
-}
-
 // Adds the drawn topic to the map: a sparse pick increments the slot it came from; otherwise the
 // topic is probed for and inserted when absent, the map growing first if the new key would
 // exceed its load limit.
@@ -610,48 +593,103 @@ __device__ __forceinline__ void add_topic(const SweepParams& P, u32* T, DocState
   map_inc_at(T, ds.cap, pk.topic, pz);
 }
 
-// One pass over a document's tokens with Philox counters (token, gdoc, iter, tag).
+// Per-lane values of the 64 tokens of one chunk: lane l holds token c0 + l.
+struct Chunk {
+  u32 ww;     // word
+  u32 zz;     // topic before the pass
+  u32 dense;  // alias-table (dense) draw
+  f32 sa;     // sigma_a of the word
+  f32 ux;     // uniform of the sparse/dense choice
+  f32 uw;     // uniform of the sparse target
+  u32 znew;   // topic after the pass (lanes below the current token)
+};
+
+// Tokens t .. nt-1 of a chunk while the map has NL lines (NL = 1..8: register variant; 0: line
+// loop).  Returns the next token: nt, or the token after the one whose insertion grew the map.
+// All variants evaluate the same operations in the same order: a variant for fewer lines drops
+// only lines that would hold no mass.
+template <int NL>
+__device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, u32* T, DocState& ds, Chunk& ch, u32 c0, u32 t,
+                                            u32 nt, u32& visited) {
+  const u32 lane = lane_id();
+  const u32 K = P.K;
+  const u32 cap0 = ds.cap;
+  for (; t < nt; ++t) {
+    const u32 wt = readlane_u32(ch.ww, t);
+    const u32 zt = readlane_u32(ch.zz, t);
+    ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, zt));
+    const f32 sa = readlane_f32(ch.sa, t);
+    Pick pk;
+    pk.topic = zt;
+    pk.slot = kNoSlot;
+    if (sa > 0.0f) {
+      wave_sync();
+      visited += ds.occupied;
+      const PhiRow prow = phi_row(P.phi + (u64)wt * K, K);
+      const f32 ux = readlane_f32(ch.ux, t), uw = readlane_f32(ch.uw, t);
+      const u32 dense = readlane_u32(ch.dense, t);
+      if constexpr (NL > 0) pk = draw_topic_regs<NL>(T, prow, sa, ux, uw, dense);
+      else pk = draw_topic_loop(T, uniform_u32(ds.cap / kMapLine), prow, sa, ux, uw, dense);
+    }
+    add_topic(P, T, ds, pk, c0, t, ch.znew, ch.zz);
+    if (lane == t) ch.znew = pk.topic;
+    if (ds.cap != cap0) return t + 1;
+  }
+  return nt;
+}
+
+// Runs tokens t .. nt-1 of a chunk with the variant for the map's current line count (register
+// variants up to 8 lines, within the kernel's LMAX, which caps its registers; larger maps use
+// the line loop).  The choice is made again only when an insertion grows the map.
+template <int LMAX>
+__device__ __forceinline__ u32 sweep_tokens_any(const SweepParams& P, u32* T, DocState& ds, Chunk& ch, u32 c0,
+                                                u32 t, u32 nt, u32& visited) {
+  switch (ds.cap / kMapLine) {  // a map always has at least one line
+    case 1: if (LMAX >= 1) return sweep_tokens<1>(P, T, ds, ch, c0, t, nt, visited); break;
+    case 2: if (LMAX >= 2) return sweep_tokens<2>(P, T, ds, ch, c0, t, nt, visited); break;
+    case 3: if (LMAX >= 4) return sweep_tokens<3>(P, T, ds, ch, c0, t, nt, visited); break;
+    case 4: if (LMAX >= 4) return sweep_tokens<4>(P, T, ds, ch, c0, t, nt, visited); break;
+    case 5: if (LMAX >= 8) return sweep_tokens<5>(P, T, ds, ch, c0, t, nt, visited); break;
+    case 6: if (LMAX >= 8) return sweep_tokens<6>(P, T, ds, ch, c0, t, nt, visited); break;
+    case 7: if (LMAX >= 8) return sweep_tokens<7>(P, T, ds, ch, c0, t, nt, visited); break;
+    case 8: if (LMAX >= 8) return sweep_tokens<8>(P, T, ds, ch, c0, t, nt, visited); break;
+    default: break;
+  }
+  return sweep_tokens<0>(P, T, ds, ch, c0, t, nt, visited);
+}
+
+// One pass over a document's tokens with Philox counters (token, gdoc, iter, tag).  visited gains
+// the sum over sampled tokens of the map's distinct topics (at most 65535 * 65535 per pass, so
+// the pass counts in 32 bits).
 template <int LMAX>
 __device__ __forceinline__ void sweep_pass(const SweepParams& P, u32* T, DocState& ds, u32 iter, u32 tag,
                                            u64& visited) {
   const u32 lane = lane_id();
   const u32 K = P.K;
+  u32 pass_visited = 0;
   for (u32 c0 = 0; c0 < ds.L; c0 += 64) {
     const u32 i = c0 + lane;
     const bool valid = i < ds.L;
-    const u32 ww = valid ? P.w[ds.b + i] : 0;
-    const u32 zz = valid ? (u32)P.z[ds.b + i] : 0;
+    Chunk ch;
+    ch.ww = valid ? P.w[ds.b + i] : 0;
+    ch.zz = valid ? (u32)P.z[ds.b + i] : 0;
     const u32x4 r = philox_tagged(i, ds.gdoc, iter, tag, 0, P.k0, P.k1);
-    // per-token values that do not depend on the map: prefetched for the whole chunk
-    const f32 sa_l = valid ? P.sigma_a[ww] : 0.0f;
+    // per-token values that do not depend on the map: computed for the whole chunk
+    ch.sa = valid ? P.sigma_a[ch.ww] : 0.0f;
     const u32 slot_l = bounded(r.y, K);
     AliasSlot al_l;
     al_l.prob = 0.0f;
     al_l.alias = 0;
-    if (valid) al_l = P.alias[(u64)ww * K + slot_l];
-    const u32 dense_l = u24(r.z) < al_l.prob ? slot_l : al_l.alias;
-    u32 znew_lane = zz;
+    if (valid) al_l = P.alias[(u64)ch.ww * K + slot_l];
+    ch.dense = u24(r.z) < al_l.prob ? slot_l : al_l.alias;
+    ch.ux = u24(r.x);
+    ch.uw = u24(r.w);
+    ch.znew = ch.zz;
     const u32 nt = ds.L - c0 < 64 ? ds.L - c0 : 64;
-    for (u32 t = 0; t < nt; ++t) {
-      const u32 wt = readlane_u32(ww, t);
-      const u32 zt = readlane_u32(zz, t);
-      ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, zt));
-      const f32 sa = readlane_f32(sa_l, t);
-      Pick pk;
-      pk.topic = zt;
-      pk.slot = kNoSlot;
-      if (sa > 0.0f) {
-        wave_sync();
-        visited += ds.occupied;
-        pk = draw_topic<LMAX>(T, uniform_u32(ds.cap / kMapLine), phi_row(P.phi + (u64)wt * K, K), sa, u24(readlane_u32(r.x, t)),
-                              u24(readlane_u32(r.w, t)), readlane_u32(dense_l, t));
-      }
-      add_topic(P, T, ds, pk, c0, t, znew_lane, zz);
-      const u32 znew = pk.topic;
-      if (lane == t) znew_lane = znew;
-    }
-    if (valid) P.z[ds.b + i] = (u16)znew_lane;
+    for (u32 t = 0; t < nt;) t = sweep_tokens_any<LMAX>(P, T, ds, ch, c0, t, nt, pass_visited);
+    if (valid) P.z[ds.b + i] = (u16)ch.znew;
   }
+  visited += pass_visited;
 }
 
 // Fold-in carries the map from sweep to sweep.  Once the document's topics have thinned out, the
@@ -1127,8 +1165,21 @@ __global__ __launch_bounds__(64) void k_stage_pick(const u32* table, u32 cap, u3
   wave_sync();
   const u32 lines = uniform_u32(cap / kMapLine);
   const PhiRow prow = phi_row(phi_row_k, K);
-  const Pick pk = use_regs ? draw_topic<8>(pick_smem, lines, prow, sa, ux[i], uw[i], kNoSlot)
-                           : draw_topic<0>(pick_smem, lines, prow, sa, ux[i], uw[i], kNoSlot);
+  Pick pk;
+  if (use_regs && lines <= 8) {
+    switch (lines) {
+      case 1: pk = draw_topic_regs<1>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
+      case 2: pk = draw_topic_regs<2>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
+      case 3: pk = draw_topic_regs<3>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
+      case 4: pk = draw_topic_regs<4>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
+      case 5: pk = draw_topic_regs<5>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
+      case 6: pk = draw_topic_regs<6>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
+      case 7: pk = draw_topic_regs<7>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
+      default: pk = draw_topic_regs<8>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
+    }
+  } else {
+    pk = draw_topic_loop(pick_smem, lines, prow, sa, ux[i], uw[i], kNoSlot);
+  }
   if (lane_id() == 0) out[i] = pk.slot;
 }
 
