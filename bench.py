@@ -51,11 +51,12 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=4000)
     ap.add_argument("--serve-serial", type=int, default=100, help="requests in the serial baseline sample")
     ap.add_argument("--serve-waves", type=int, default=2, help="worker wavefronts per CU of the fold-in server")
-    ap.add_argument("--abi-rate", action="store_true",
-                    help="also time the five-call ABI path (host buffers, z over PCIe each iteration)")
-    ap.add_argument("--compare-graph", action="store_true",
-                    help="also time the iteration as a replayed hipGraph against plain per-bucket launches "
-                         "(identical Philox streams, z compared)")
+    ap.add_argument("--no-abi", action="store_true",
+                    help="skip timing the five-call ABI path (host buffers, z over PCIe each iteration)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-form measurement (power-law corpus: hipGraph replay against "
+                         "per-bucket launches, identical Philox streams, z compared)")
+    ap.add_argument("--batched-steps", type=int, default=5)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -106,18 +107,24 @@ def main():
     sweep_ms = float(stage_ms[2])
     alg_bytes = N * (18.0 + 4.0 * s_bar)
     achieved = alg_bytes / (sweep_ms * 1e-3) / 1e9
-    traffic = None
+    # HBM traffic per launch comes from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this same command
+    # (same warmup and steps, so the same chain state); it is used only when that run's S_bar is
+    # this run's, and is reported with its own S_bar and algorithmic bytes either way
+    traffic, traffic_run = None, None
     if os.path.exists(TRAFFIC_FILE):
         try:
             tr = json.load(open(TRAFFIC_FILE))
             if tr.get("workload") == spec.name:
-                traffic = tr.get("bytes_per_launch")
+                traffic_run = {k: tr.get(k) for k in ("s_bar", "alg_bytes_per_launch", "bytes_per_launch",
+                                                      "traffic_over_alg", "command")}
+                if tr.get("s_bar") is not None and abs(float(tr["s_bar"]) - s_bar) <= 1e-3 * s_bar:
+                    traffic = tr.get("bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "z-sweep (k_sweep_*: all launches of one sweep)", "kernel_ms": round(sweep_ms, 3), "alg_bytes_per_launch": int(alg_bytes),
-                "s_bar": round(s_bar, 3)}
+                "kernel": "z-sweep (k_sweep_*: all launches of one sweep)", "kernel_ms": round(sweep_ms, 3),
+                "alg_bytes_per_launch": int(alg_bytes), "s_bar": round(s_bar, 3), "traffic_run": traffic_run}
 
     cpu = None
     if not args.no_cpu:
@@ -127,12 +134,12 @@ def main():
     if not args.no_serve:
         serving = serve_bench(rc, spec, args.serve_requests, args.serve_serial, args.serve_waves)
 
-    graph = None
-    if args.compare_graph:  # last: it releases rc (the library holds one model at a time)
-        graph = compare_graph(rc, c, spec, args.steps)
+    rc.close()  # the library holds one model at a time
+    batched = None
+    if not args.no_batched:
+        batched = batched_bench(args.batched_steps)
     abi = None
-    if args.abi_rate:
-        rc.close()  # the library holds one model at a time
+    if not args.no_abi:
         abi = abi_rate(c, spec, 3)
 
     out = {
@@ -154,12 +161,12 @@ def main():
                                 "sweep": round(float(stage_ms[2]), 3), "count": round(float(stage_ms[3]), 3)},
                    "parallelism": "1 GPU",
                    "schedule": tune if tune is not None else args.schedule,
-                   "graph_vs_plain": graph, "abi_path_pcie_inclusive": abi},
+                   "abi_path_pcie_inclusive": abi},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serving,
+        "batched": batched,
     }
-    rc.close()
     print(json.dumps(out), flush=True)
 
 
@@ -222,33 +229,57 @@ def abi_rate(c, spec, iters):
     return out
 
 
-def compare_graph(rc, c, spec, steps):
-    """Batched form vs plain form on the same chain: the current schedule replayed as a hipGraph
-    against per-bucket launches, from the same state; z must come out identical."""
+def batched_bench(steps):
+    """The batched form on its own corpus (V=100k, K=1,024, D=2M, power-law lengths on [4, 20000]):
+    after two warm-up iterations the autotuner picks the schedule (persistent and block-per-document
+    launches), then `steps` iterations run as a replayed hipGraph; the same chain is replayed from
+    the start with plain per-bucket launches over the same iterations, and z must come out
+    identical."""
+    from gpupolyaurnlda_amd import corpus
     from gpupolyaurnlda_amd.resident import ResidentCorpus
 
-    z0 = rc.z()
+    spec = corpus.BATCHED
+    t0 = time.time()
+    c = corpus.generate(spec)
+    N = len(c["w"])
+    log(f"batched: generated N={N} tokens, longest document {int(c['d_len'].max())}, in {time.time() - t0:.1f}s")
+    rc = ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=spec.seed, z=c["z"])
+    rc.iterate_untimed(2)
+    rows, chosen = rc.autotune()
     it0 = rc.iteration
-    t0 = time.perf_counter()
     rc.use_graph(True)
-    rc.iterate(steps)
+    rc.iterate_untimed(1)  # capture
+    t0 = time.perf_counter()
+    stage_ms = rc.iterate(steps)
     t_graph = (time.perf_counter() - t0) / steps
-    rc.use_graph(False)
     z_graph = rc.z()
     rc.close()
-    # plain form: the same chain replayed from the start with per-bucket launches
     plain = ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=spec.seed, z=c["z"])
     plain.schedule(0)
-    plain.iterate_untimed(it0)
-    same_start = bool(np.array_equal(plain.z(), z0))
+    plain.iterate_untimed(it0 + 1)
     t0 = time.perf_counter()
     plain.iterate_untimed(steps)
     t_plain = (time.perf_counter() - t0) / steps
     same = bool(np.array_equal(plain.z(), z_graph))
     plain.close()
-    log(f"graph {t_graph * 1e3:.1f} ms/iter vs plain per-bucket {t_plain * 1e3:.1f} ms/iter; identical z: {same}")
-    return {"graph_ms_per_iter": round(t_graph * 1e3, 3), "plain_buckets_ms_per_iter": round(t_plain * 1e3, 3),
-            "identical_z": same and same_start}
+    out = {"workload": spec.name, "N": N, "schedule": rows[chosen], "steps": steps,
+           "graph_ms_per_iter": round(t_graph * 1e3, 3), "graph_tokens_per_s": round(N / t_graph, 1),
+           "graph_stage_ms": {"phi_draw": round(float(stage_ms[0]), 3), "alias": round(float(stage_ms[1]), 3),
+                              "sweep": round(float(stage_ms[2]), 3), "count": round(float(stage_ms[3]), 3)},
+           "plain_buckets_ms_per_iter": round(t_plain * 1e3, 3),
+           "plain_buckets_tokens_per_s": round(N / t_plain, 1), "identical_z": same}
+    log(f"batched: {out}")
+    return out
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(rc, spec, c, n_before, z_before, it0, steps, cpu_docs):
@@ -280,16 +311,33 @@ def cpu_baseline(rc, spec, c, n_before, z_before, it0, steps, cpu_docs):
     O.count(c["w"][:n_s], z_s, spec.V, spec.K)
     t_cnt = time.perf_counter() - t0
     t_iter = t_phi + (t_sw + t_cnt) * (N / n_s)
+    # one thread: the z-sweep of a smaller sample (the sweep is the iteration's dominant cost)
+    D_1 = min(max(cpu_docs // 8, 1), spec.D)
+    n_1 = int(off[D_1])
+    t0 = time.perf_counter()
+    z_1, _ = O.sweep(z_now[:n_1], c["w"][:n_1], off[:D_1 + 1], 0, it0 + steps + 1, spec.seed, spec.K, phi, sig,
+                     prob, al, threads=1)
+    t_1 = time.perf_counter() - t0
+    same_1 = bool(np.array_equal(z_1, z_s[:n_1]))
     same_phi = bool(np.array_equal(phi.view(np.uint32), phi_gpu.view(np.uint32)))
     same_z = bool(np.array_equal(z_s, z_gpu[:n_s]))
     log(f"cpu: phi {t_phi:.2f}s, sweep {t_sw:.2f}s + count {t_cnt:.2f}s for {n_s} tokens; "
         f"bit-exact phi={same_phi} z={same_z}")
     del n_before, z_before
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
     return {"value": round(N / t_iter, 1), "unit": "tokens/s", "cores": threads, "kind": "port",
             "sample": f"phi step over all V*K cells + z-sweep and count of the first {D_s} documents "
                       f"({n_s} tokens), scaled to N={N}; phi and sampled z bit-identical to the GPU: "
                       f"{same_phi and same_z}",
-            "bit_exact_sample": same_phi and same_z}
+            "bit_exact_sample": same_phi and same_z,
+            "model": cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "sweep_tokens_per_s": round(n_s / t_sw, 1),
+            "sweep_tokens_per_s_1thread": round(n_1 / t_1, 1),
+            "sample_1thread": f"z-sweep of the first {D_1} documents ({n_1} tokens) on one thread; "
+                              f"z equal to the {threads}-thread sweep: {same_1}"}
 
 
 if __name__ == "__main__":

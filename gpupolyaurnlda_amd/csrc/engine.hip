@@ -627,10 +627,11 @@ u64 sweep_lds_bytes(const Segment* sg) { return ((u64)map_capacity(sg->max_len, 
 //                        longest-first queue (k_sweep_persistent);
 //   persistent + block   documents of at least block_min_len tokens go to a block-per-document
 //                        persistent launch (k_sweep_block) on a second stream, the rest as above;
-//   persistent split     documents longer than block_min_len go to a persistent launch sized for
-//                        the longest map on a second stream; the others to a persistent launch
-//                        whose LDS is sized for their own longest map, so more of its wavefronts
-//                        fit on a CU.
+//   persistent split     documents longer than block_min_len go to a persistent launch of the
+//                        line-loop sweep (k_sweep_persistent_long, few registers: long documents
+//                        grow maps past the register variants) sized for the longest map, on a
+//                        second stream; the others to k_sweep_persistent<8> whose LDS is sized
+//                        for their own longest map, so more of its wavefronts fit on a CU.
 // All schedules produce bit-identical z: a document's randomness depends on its global id only.
 int launch_sweep_schedule(const SweepParams& P0, Segment* sg, hipStream_t st) {
   if (S.schedule == kSchedBuckets || sg->n_docs == 0) return launch_sweep(P0, sg, st);
@@ -647,7 +648,7 @@ int launch_sweep_schedule(const SweepParams& P0, Segment* sg, hipStream_t st) {
       HCK(hipStreamWaitEvent(S.stream2, S.fork, 0));
       const u32 per_cu = std::max<u32>(1, std::min<u32>(S.waves_per_cu, (u32)(lds_per_cu() / lds)));
       const u32 waves = std::min<u32>((u32)S.cu_count * per_cu, n_long);
-      k_sweep_persistent<8><<<waves, 64, lds, S.stream2>>>(P, S.heads + 1, n_long);
+      k_sweep_persistent_long<<<waves, 64, lds, S.stream2>>>(P, S.heads + 1, n_long);
       HCK(hipGetLastError());
       HCK(hipEventRecord(S.join, S.stream2));
     }
@@ -1406,8 +1407,8 @@ extern "C" int gpulda_corpus_autotune(float* table, uint32_t max_candidates, uin
   const u32 longest = sg->lens_desc.empty() ? 0 : sg->lens_desc[0];
   for (u32 m : {2048u, 8192u})
     if (longest >= m) cands.push_back({kSchedPersistentBlock, 16, m});
-  // (the split schedule is not a candidate: its single-sweep trial matches the others early in a
-  // run, but it falls behind the persistent queue as the documents' topics concentrate)
+  for (u32 m : {512u, 1024u, 2048u})
+    if (longest > m) cands.push_back({kSchedPersistentSplit, 32, m});
   u16* saved = nullptr;
   CK(dalloc(&saved, sg->n_tokens));
   unsigned long long vis0 = 0;

@@ -96,6 +96,7 @@ template <int LMAX>
 __global__ void k_sweep_lines(SweepParams P);
 template <int LMAX>
 __global__ void k_sweep_persistent(SweepParams P, u32* head, u32 n_items);
+__global__ void k_sweep_persistent_long(SweepParams P, u32* head, u32 n_items);
 // Block-per-document sweep (4 wavefronts per document); head == null: block b takes item b.
 __global__ void k_sweep_block(SweepParams P, u32* head, u32 n_items);
 // Launch buckets of the per-bucket schedule, by map lines (0 = any size).  Buckets 1-8 run the

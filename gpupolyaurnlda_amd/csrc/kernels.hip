@@ -762,6 +762,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
 }
 template __global__ void k_sweep_persistent<8>(SweepParams, u32*, u32);
 
+// Persistent form of the line-loop sweep for long documents (maps past the register variants),
+// without the eight-wavefront register budget: the line loop then keeps its scalar state in
+// scalar registers (spilling it to vector lanes lengthens each token's dependent chain, which is
+// what a long document's sweep time is made of).
+__global__ __launch_bounds__(64) void k_sweep_persistent_long(SweepParams P, u32* head, u32 n_items) {
+  extern __shared__ u32 sweep_smem[];
+  for (;;) {
+    u32 item = 0;
+    if (lane_id() == 0) item = atomicAdd(head, 1u);
+    item = readlane_u32(item, 0);
+    if (item >= n_items) return;
+    sweep_doc<0>(P, P.doc_order[item], sweep_smem);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Block-per-document sweep for long documents: four wavefronts share one map.  Wave 0 owns the
 // map (decrement, growth, increment) and the in-column search; wave g sums line group g (lines

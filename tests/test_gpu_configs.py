@@ -6,8 +6,8 @@
           the persistent schedule the benchmark runs: two iterations, z, n_wk and the full V x K
           phi bit-exact
   batched V=100k, K=1024, power-law lengths on [4, 20000] -- a 20,000-document slice under the
-          persistent + block-per-document schedule, eagerly and as a replayed hipGraph, against
-          the oracle at every iteration
+          persistent + block-per-document and the persistent split schedules, eagerly and as a
+          replayed hipGraph, against the oracle at every iteration
   serving fold-in against a K=1024, V=100k model: 50 sweeps per request, lognormal(300, 1.2)
           lengths clipped to [10, 5000]; the server's results, mixed and alone, equal the oracle's
           fold-in of the same request ids
@@ -86,25 +86,28 @@ def test_large_slice_bit_exact_full_phi(orc):
         assert rc.visited() == model.visited
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_batched_slice_block_schedule_vs_oracle(orc, graph):
+@pytest.mark.parametrize("sched,graph", [(2, False), (2, True), (3, False), (3, True)])
+def test_batched_slice_vs_oracle(orc, sched, graph):
+    """sched 2: persistent + block-per-document (documents of at least 1,000 tokens, four
+    wavefronts each); sched 3: persistent split (documents over 1,000 tokens in the long-document
+    persistent launch, the rest in the register-variant one)."""
     from gpupolyaurnlda_amd import corpus
     from gpupolyaurnlda_amd.resident import ResidentCorpus
     spec = corpus.BATCHED.scaled(20000)
     c = corpus.generate(spec)
     longest = int(c["d_len"].max())
-    say(f"batched slice: D={spec.D} N={len(c['w'])} longest={longest} graph={graph}")
-    block_min = 1000
-    assert (c["d_len"] >= block_min).sum() > 10  # the block-per-document kernel gets work
+    say(f"batched slice: D={spec.D} N={len(c['w'])} longest={longest} schedule={sched} graph={graph}")
+    min_len = 1000
+    assert (c["d_len"] > min_len).sum() > 10  # the long-document kernel gets work
     model = _model(orc, spec, c, seed=3)
     with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=3, z=c["z"]) as rc:
-        rc.schedule(2, 16, block_min)
+        rc.schedule(sched, 16, min_len)
         rc.use_graph(graph)
         for it in (1, 2, 3):  # graph replays with eager reads in between
             rc.iterate_untimed(1)
             model.iterate()
             _same_state(rc, model, it, phi=(it == 3))
-            say(f"batched slice iteration {it} bit-exact (graph={graph})")
+            say(f"batched slice iteration {it} bit-exact (schedule={sched}, graph={graph})")
 
 
 def test_serving_large_model_vs_oracle(orc):
