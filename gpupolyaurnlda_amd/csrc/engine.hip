@@ -142,14 +142,6 @@ int lds_per_cu() {
   return v;
 }
 
-int launch_count(const CountTask* tasks, u32 n_tasks, const u32* perm, const u16* z, u32* n, u32 K, hipStream_t st) {
-  if (n_tasks == 0) return 0;
-  const u32 wpb = 4;
-  u64 lds = (u64)wpb * K * sizeof(u32);
-  k_count<<<(n_tasks + wpb - 1) / wpb, 64 * wpb, lds, st>>>(tasks, n_tasks, perm, z, n, K, wpb);
-  HCK(hipGetLastError());
-  return 0;
-}
 
 // Poisson PMF of lambda = n + beta over 0..199, in double, stored as f32 (poisson.cu:9-20).
 void poisson_pmf_rows(f32 beta, std::vector<f32>& pmf) {
@@ -192,7 +184,11 @@ struct Segment {
   u32* w = nullptr;
   u16* z = nullptr;
   u64* doc_off = nullptr;
-  u32* perm = nullptr;
+  // n_wk reduction (launch_count): per token its rank inside its kCountTile tile in (word,
+  // position) order, per tile rank the token's place in word-sorted order, and z in that order
+  u16* crank = nullptr;
+  u32* cdst = nullptr;
+  u16* zs = nullptr;
   CountTask* tasks = nullptr;
   u32* doc_order = nullptr;  // documents grouped by sweep bucket (longest first inside a bucket)
   u32* doc_lpt = nullptr;    // all documents, longest first (queue of the persistent schedules)
@@ -217,6 +213,19 @@ struct Segment {
 
 constexpr u32 kMaxParts = 8;
 u64 g_part_min_tokens = 4ull << 20;  // smallest pipeline part (gpulda_set_part_tokens)
+
+// n_wk += counts of the segment's (word, topic) pairs: z to word order, then the per-word tasks.
+int launch_count(const Segment* sg, u32* n, u32 K, hipStream_t st) {
+  if (sg->n_tasks == 0) return 0;
+  const u64 tiles = (sg->n_tokens + kCountTile - 1) / kCountTile;
+  k_z_word_order<<<(u32)tiles, 1024, 0, st>>>(sg->z, sg->crank, sg->cdst, sg->zs, sg->n_tokens);
+  HCK(hipGetLastError());
+  const u32 wpb = 4;
+  u64 lds = (u64)wpb * K * sizeof(u32);
+  k_count<<<(sg->n_tasks + wpb - 1) / wpb, 64 * wpb, lds, st>>>(sg->tasks, sg->n_tasks, sg->zs, n, K, wpb);
+  HCK(hipGetLastError());
+  return 0;
+}
 
 // z-sweep of buckets of documents (order[first ..]): one launch per bucket, the longest maps first.
 int launch_buckets(SweepParams P, const std::vector<Segment::Bucket>& buckets, const u32* order, hipStream_t st) {
@@ -250,7 +259,9 @@ void free_segment(Segment* s) {
   dfree(s->w);
   dfree(s->z);
   dfree(s->doc_off);
-  dfree(s->perm);
+  dfree(s->crank);
+  dfree(s->cdst);
+  dfree(s->zs);
   dfree(s->tasks);
   dfree(s->doc_order);
   dfree(s->doc_lpt);
@@ -336,16 +347,24 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
     }
   }
   if (n_tokens >= (1ull << 32)) return fail("a slice must hold fewer than 2^32 tokens");
-  std::vector<u32> perm(n_tokens);
+  // word-sorted order (words ascending, positions ascending within a word) -> tile ranks and
+  // destinations of the word-order pass: walking that order, the tokens of each tile come up in
+  // (word, position) order, so a per-tile counter gives their ranks
+  std::vector<u16> crank(n_tokens);
+  std::vector<u32> cdst(n_tokens);
   {
+    std::vector<u32> perm(n_tokens);
     std::vector<u64> cur(start.begin(), start.end() - 1);
     for (u64 i = 0; i < n_tokens; ++i) perm[cur[w[i]]++] = (u32)i;
+    std::vector<u32> tile_fill((n_tokens + kCountTile - 1) / kCountTile, 0);
+    for (u64 p = 0; p < n_tokens; ++p) {
+      const u32 i = perm[p];
+      const u64 t = i / kCountTile;
+      const u32 r = tile_fill[t]++;
+      crank[i] = (u16)r;
+      cdst[t * kCountTile + r] = (u32)p;
+    }
   }
-  // count tasks in the order of their first token: the tasks running together then gather z from
-  // the same stretch of the corpus, so a z line fetched for one word serves the other words of
-  // that stretch from cache (in word order the tasks of one frequent word span the whole corpus)
-  std::sort(tasks.begin(), tasks.end(),
-            [&](const CountTask& a, const CountTask& b) { return perm[a.p0] < perm[b.p0]; });
   // sweep buckets: by register-resident line count, largest first; longest documents first
   std::vector<u32> order(n_docs);
   std::vector<int> lmax_of(n_docs);
@@ -405,11 +424,14 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
   CK(dalloc(&s->w, n_tokens));
   CK(dalloc(&s->z, n_tokens));
   CK(dalloc(&s->doc_off, (u64)n_docs + 1));
-  CK(dalloc(&s->perm, n_tokens));
+  CK(dalloc(&s->crank, n_tokens));
+  CK(dalloc(&s->cdst, n_tokens));
+  CK(dalloc(&s->zs, n_tokens));
   CK(dalloc(&s->tasks, tasks.size()));
   if (n_tokens) {
     HCK(hipMemcpy(s->w, w, n_tokens * sizeof(u32), hipMemcpyHostToDevice));
-    HCK(hipMemcpy(s->perm, perm.data(), n_tokens * sizeof(u32), hipMemcpyHostToDevice));
+    HCK(hipMemcpy(s->crank, crank.data(), n_tokens * sizeof(u16), hipMemcpyHostToDevice));
+    HCK(hipMemcpy(s->cdst, cdst.data(), n_tokens * sizeof(u32), hipMemcpyHostToDevice));
   }
   HCK(hipMemcpy(s->doc_off, off.data(), off.size() * sizeof(u64), hipMemcpyHostToDevice));
   if (!tasks.empty()) HCK(hipMemcpy(s->tasks, tasks.data(), tasks.size() * sizeof(CountTask), hipMemcpyHostToDevice));
@@ -716,7 +738,9 @@ bool segment_pinned(const Segment* s) {
 }
 
 u64 segment_bytes(const Segment* s, u64 n_tasks) {
-  return s->n_tokens * (sizeof(u32) + sizeof(u16) + sizeof(u32)) + ((u64)s->n_docs + 1) * sizeof(u64) +
+  // w, z, crank, cdst, zs per token
+  return s->n_tokens * (sizeof(u32) + sizeof(u16) + sizeof(u16) + sizeof(u32) + sizeof(u16)) +
+         ((u64)s->n_docs + 1) * sizeof(u64) +
          (u64)s->n_docs * 3 * sizeof(u32) + n_tasks * sizeof(CountTask);
 }
 
@@ -978,7 +1002,7 @@ int do_sample_z_async(Buffer* B) {
                          bs->copy_out));
     HCK(hipEventRecord(bs->ev_out[p], bs->copy_out));
   }
-  CK(launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, S.n, S.K, st));
+  CK(launch_count(sg, S.n, S.K, st));
   HCK(hipEventRecord(bs->ev_done, bs->copy_out));
   HCK(hipStreamWaitEvent(st, bs->ev_done, 0));  // the buffer's stream covers everything
   bs->pending = sg;
@@ -1272,7 +1296,7 @@ extern "C" int gpulda_corpus_counts_from_z(void) {
   if (!S.resident) return fail("no resident corpus");
   Segment* sg = S.resident;
   CK(fill_u32(S.n, (u64)S.V * S.K, 0, S.stream));
-  CK(launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, S.n, S.K, S.stream));
+  CK(launch_count(sg, S.n, S.K, S.stream));
   HCK(hipStreamSynchronize(S.stream));
   return 0;
 }
@@ -1303,7 +1327,7 @@ int resident_iteration(Segment* sg, hipStream_t st, hipEvent_t* ev) {
   if (ev) HCK(hipEventRecord(ev[2], st));
   CK(launch_sweep_schedule(sweep_params(sg, 0), sg, st));
   if (ev) HCK(hipEventRecord(ev[3], st));
-  CK(launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, S.n, S.K, st));
+  CK(launch_count(sg, S.n, S.K, st));
   if (ev) HCK(hipEventRecord(ev[4], st));
   S.doc_counter = sg->n_docs;
   return 0;
@@ -2214,7 +2238,7 @@ extern "C" int gpulda_stage_count(const uint32_t* w, const uint32_t* z, uint64_t
     k_z_narrow<<<grid_for(n_tokens, 256), 256>>>(dz32, sg->z, n_tokens, K, bad);
     if (hipGetLastError() != hipSuccess) r2 = fail("launch failed");
   }
-  if (!r2) r2 = launch_count(sg->tasks, sg->n_tasks, sg->perm, sg->z, dn, K, 0);
+  if (!r2) r2 = launch_count(sg, dn, K, 0);
   if (!r2 && hipDeviceSynchronize() != hipSuccess) r2 = fail("count failed");
   if (!r2 && hipMemcpy(n_out, dn, VK * 4, hipMemcpyDeviceToHost) != hipSuccess) r2 = fail("copy failed");
   free_segment(sg);

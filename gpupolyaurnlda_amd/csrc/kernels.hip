@@ -1102,11 +1102,29 @@ __global__ __launch_bounds__(64) void k_serve_worker(SweepParams P, FoldParams F
 }
 
 // ---------------------------------------------------------------------------------------------
-// n_wk reduction over the word-sorted token permutation.  One wavefront per task (a run of at
-// most kCountTask tokens of one word): LDS histogram over K, then nonzero bins added to n_wk.
+// n_wk reduction in two passes, both reading and writing HBM in order.
+//   k_z_word_order  z (document order) -> zs (word-sorted order: words ascending, each word's
+//                   tokens in corpus order).  One workgroup per tile of kCountTile consecutive
+//                   tokens: each topic goes to LDS at its token's rank inside the tile (the
+//                   tile's tokens ordered by word, then position), and the tile's word runs are
+//                   written out to their places in zs.  rank and dst are fixed by the words, so
+//                   they are built once per corpus.
+//   k_count         one wavefront per task (a run of at most kCountTask tokens of one word, a
+//                   contiguous range of zs): LDS histogram over K, nonzero bins added to n_wk.
 // ---------------------------------------------------------------------------------------------
-__global__ void k_count(const CountTask* __restrict__ tasks, u32 n_tasks, const u32* __restrict__ perm,
-                        const u16* __restrict__ z, u32* __restrict__ n, u32 K, u32 waves_per_block) {
+__global__ __launch_bounds__(1024) void k_z_word_order(const u16* __restrict__ z, const u16* __restrict__ rank,
+                                                       const u32* __restrict__ dst, u16* __restrict__ zs,
+                                                       u64 n_tokens) {
+  __shared__ u16 tile[kCountTile];
+  const u64 t0 = (u64)blockIdx.x * kCountTile;
+  const u32 n = (u32)(n_tokens - t0 < kCountTile ? n_tokens - t0 : kCountTile);
+  for (u32 l = threadIdx.x; l < n; l += blockDim.x) tile[rank[t0 + l]] = z[t0 + l];
+  __syncthreads();
+  for (u32 j = threadIdx.x; j < n; j += blockDim.x) zs[dst[t0 + j]] = tile[j];
+}
+
+__global__ void k_count(const CountTask* __restrict__ tasks, u32 n_tasks, const u16* __restrict__ zs,
+                        u32* __restrict__ n, u32 K, u32 waves_per_block) {
   extern __shared__ u32 count_smem[];
   const u32 lane = lane_id();
   const u32 wave = threadIdx.x >> 6;
@@ -1116,10 +1134,7 @@ __global__ void k_count(const CountTask* __restrict__ tasks, u32 n_tasks, const 
   for (u32 k = lane; k < K; k += 64) H[k] = 0;
   wave_sync();
   const CountTask tk = tasks[task];
-  for (u32 p = lane; p < tk.len; p += 64) {
-    u32 pos = perm[tk.p0 + p];
-    atomicAdd(&H[z[pos]], 1u);
-  }
+  for (u32 p = lane; p < tk.len; p += 64) atomicAdd(&H[zs[tk.p0 + p]], 1u);
   wave_sync();
   u32* row = n + (u64)tk.word * K;
   for (u32 k = lane; k < K; k += 64) {
