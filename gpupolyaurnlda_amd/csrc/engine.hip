@@ -108,6 +108,8 @@ int max_lds_per_block() {
 // ---------------------------------------------------------------------------------------------
 // Launchers shared by the ABI, the resident corpus and the stage entry points.
 // ---------------------------------------------------------------------------------------------
+int lds_per_cu();
+
 int launch_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
                       AliasSlot* out, hipStream_t st) {
   if (n_rows == 0) return 0;
@@ -116,9 +118,19 @@ int launch_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* di
   u64 lim = (u64)max_lds_per_block();
   if (per > lim) return fail("row length %u needs %llu B of LDS (limit %llu)", m, (unsigned long long)per,
                              (unsigned long long)lim);
-  u32 wpb = (u32)std::min<u64>(4, std::max<u64>(1, lim / per));
+  // wavefronts per block: the count (up to 4) that keeps the most rows resident per CU
+  u32 wpb = 1;
+  u64 best = 0;
+  for (u32 c = 1; c <= 4 && c * per <= lim; ++c) {
+    const u64 resident = ((u64)lds_per_cu() / (c * per)) * c;
+    if (resident >= best) {
+      best = resident;
+      wpb = c;
+    }
+  }
   u32 grid = (n_rows + wpb - 1) / wpb;
-  k_alias_rows<<<grid, 64 * wpb, per * wpb, st>>>(rows, n_rows, m, divisor, sigma, alpha, out, wpb);
+  if (m <= 64 * 16) k_alias_rows<16><<<grid, 64 * wpb, per * wpb, st>>>(rows, n_rows, m, divisor, sigma, alpha, out, wpb);
+  else k_alias_rows<0><<<grid, 64 * wpb, per * wpb, st>>>(rows, n_rows, m, divisor, sigma, alpha, out, wpb);
   HCK(hipGetLastError());
   return 0;
 }

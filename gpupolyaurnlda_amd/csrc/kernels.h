@@ -92,6 +92,7 @@ __global__ void k_phi_draw(const u32* n, f32* phi, unsigned long long* topic_sum
 __global__ void k_bump_u32(u32* p);
 __global__ void k_init_counts(const u32* C, u32* n, u32 V, u32 K, f32 beta, u32 k0, u32 k1, const AliasSlot* tables,
                               u32 rows_per_block);
+template <int PER>
 __global__ void k_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
                              AliasSlot* out, u32 rows_per_block_waves);
 template <int LMAX>
@@ -125,6 +126,6 @@ __global__ void k_stage_alias_draw(const AliasSlot* row, u32 m, const u32* r_slo
                                    u32* out);
 
 // LDS bytes per alias row of length m (one wavefront).
-inline u64 alias_lds_per_wave(u32 m) { return (u64)((m + 3) & ~3u) * 10; }
+LDA_HD u64 alias_lds_per_wave(u32 m) { return (u64)((m + 3) & ~3u) * 14 + 68 * 4; }
 
 }  // namespace lda

@@ -98,17 +98,23 @@ __global__ void k_init_counts(const u32* __restrict__ C, u32* __restrict__ n, u3
 
 // ---------------------------------------------------------------------------------------------
 // Alias rows.  One wavefront per row of length m:
-//   A. (optional) x_k /= (f32)divisor_k; canonical row sum T; sigma = alpha * T.
+//   A. (optional) x_k /= (f32)divisor_k; canonical row sum T (lane l adds x_l, x_{64+l}, ... in
+//      order, then the canonical wave scan); sigma = alpha * T.  The row is staged in LDS.
 //   B. q_k = floor(x_k / T * m * 2^32) (exact double ops), slot capacity Q = sum_k q_k and item
 //      weight a_k = m q_k, so that total deficit equals total surplus exactly.  a < Q is light
 //      (deficit Q - a), otherwise heavy (surplus a - Q).  Exclusive deficit prefixes D over the
 //      lights and inclusive surplus prefixes E over the heavies, in index order, are exact u64
-//      sums; they share one LDS array of m entries (the nh heavies' E from the front, the nl
-//      lights' D from the back, nh + nl = m), beside the heavies' indices.
+//      sums (lane l takes slots [l*per, (l+1)*per) in order; one wave scan of the lane totals
+//      places them); they share one LDS array of m entries (the nh heavies' E from the front,
+//      the nl lights' D from the back, nh + nl = m), as do the slot indices.
 //   C. light i: alias = first heavy with E > D_i, prob = a_i / Q.
 //      heavy j: its turn ends at D_end = D of the first light with D >= E_j, or the total deficit
 //      when there is none; if D_end > E_j it keeps prob = (Q + E_j - D_end) / Q and its
 //      alias is the next heavy, otherwise (or with no next heavy) prob = 1, alias = j.
+//      Both are ranks between the ascending sequences D' = D + 1 (lights) and E (heavies): in
+//      their merge with ties taken from D' first, light i is preceded by exactly the heavies with
+//      E < D_i + 1 and heavy j by exactly the lights with D + 1 <= E_j.  Each lane produces `per`
+//      consecutive outputs of that merge (one diagonal binary search, then a sequential merge).
 // This is Vose's pairing expressed through prefix sums (the sweep of Huebschle-Schneider &
 // Sanders, "Parallel Weighted Random Sampling", 2019), so every slot is computed independently.
 // ---------------------------------------------------------------------------------------------
@@ -118,6 +124,27 @@ __device__ __forceinline__ u64 readlane_u64(u64 v, u32 l) {
   return ((u64)hi << 32) | lo;
 }
 
+// a / b for a row's fixed divisor b, from y = 1 / b (correctly rounded, once per row): q = a y is
+// within an ulp of a / b, the residual a - b q is exact in one FMA, and one FMA correction then
+// gives the correctly rounded quotient (Markstein, IBM J. Res. Dev. 34(1), 1990, Theorem 1), the
+// same bits as the division.  Operands here are non-negative and far from overflow and underflow.
+__device__ __forceinline__ f64 div_by(f64 a, f64 b, f64 y) {
+  const f64 q = a * y;
+  const f64 r = __builtin_fma(-q, b, a);
+  return __builtin_fma(r, y, q);
+}
+
+// alias_fixed_weight with the row total's reciprocal: floor(w / T * n * 2^32), which is below
+// 2^52 (w <= T, n <= 2^20), converted to u64 by adding 2^52 (exact for integers below 2^52, whose
+// bits then sit in the mantissa).
+__device__ __forceinline__ u64 alias_fixed_weight_r(f32 w, f32 T, f64 yT, u32 n) {
+  const f64 q = div_by((f64)w, (f64)T, yT) * ((f64)n * 4294967296.0);
+  return (u64)__double_as_longlong(__builtin_trunc(q) + 0x1p52) - 0x4330000000000000ull;
+}
+
+// PER > 0: rows of at most 64 * PER slots, each lane's fixed-point weights held in registers;
+// PER == 0: any row length, weights recomputed in each pass.
+template <int PER>
 __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const unsigned long long* __restrict__ divisor,
                              f32* __restrict__ sigma, f32 alpha, AliasSlot* __restrict__ out, u32 rows_per_block_waves) {
   extern __shared__ unsigned char alias_smem[];
@@ -125,26 +152,72 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
   const u32 wave = threadIdx.x >> 6;
   const u32 row = blockIdx.x * rows_per_block_waves + wave;
   if (row >= n_rows) return;
-  // per wavefront: m4 heavy indices (u16: rows are at most kMaxTopics long) and m4 u64 prefixes
+  // per wavefront (alias_lds_per_wave): m4 u64 prefixes, m4 + 68 f32 row values (a light's
+  // probability once its prefix is placed), m4 u16 slot indices.  Lane l works on slots
+  // [l * per, (l + 1) * per); row values sit at xp(k) = k + pad * (k / per) with pad making the
+  // lanes' stride per + pad odd, so the lanes' reads of their t-th slot fall in distinct banks.
   const u32 m4 = (m + 3) & ~3u;
-  u16* hidx = reinterpret_cast<u16*>(alias_smem + (u64)wave * m4 * 10);
-  u64* hE = reinterpret_cast<u64*>(hidx + m4);  // heavy j's E at hE[j]
-  u64* lDr = hE + (m - 1);                           // light b's D at lDr[-b]
+  u64* pre = reinterpret_cast<u64*>(alias_smem + (u64)wave * alias_lds_per_wave(m));
+  f32* xs = reinterpret_cast<f32*>(pre + m4);
+  u16* idx = reinterpret_cast<u16*>(xs + m4 + 68);
+  const u32 per = (m + 63) / 64;  // at most PER when PER > 0
+  const u32 pad = (per & 1u) ^ 1u;
+  // k / per as a multiply-high by ceil(2^32 / per): exact for k < 2^16 and per < 2^16 (the
+  // reciprocal's excess adds less than k / 2^32 < 2^-16 < 1 / per to a quotient whose fraction
+  // is at most 1 - 1 / per); per == 1 has pad 0, so its quotient is not used
+  const u32 per_magic = per > 1 ? (u32)((0x100000000ull + per - 1) / per) : 0u;
+  auto xp = [&](u32 k) { return k + pad * __umulhi(k, per_magic); };
 
   f32* x = rows + (u64)row * m;
   AliasSlot* o = out + (u64)row * m;
 
   f32 acc = 0.0f;
-  for (u32 c = 0; c < m; c += 64) {
-    u32 k = c + lane;
-    if (k < m) {
-      f32 p = x[k];
-      if (divisor) {
-        f32 d = (f32)(f64)divisor[k];
-        p = d > 0.0f ? p / d : 0.0f;
-        x[k] = p;
+  if constexpr (PER > 0) {
+    // every load of the lane's PER strided slots in flight before any arithmetic
+    f32 xv[PER];
+#pragma unroll
+    for (int c = 0; c < PER; ++c) {
+      const u32 k = (u32)c * 64 + lane;
+      xv[c] = k < m ? x[k] : 0.0f;
+    }
+    if (divisor) {
+      u64 dr[PER];  // raw loads first: converting each as it arrives would wait on it
+#pragma unroll
+      for (int c = 0; c < PER; ++c) {
+        const u32 k = (u32)c * 64 + lane;
+        dr[c] = k < m ? divisor[k] : 0ull;
       }
-      acc = acc + p;
+#pragma unroll
+      for (int c = 0; c < PER; ++c) {
+        const u32 k = (u32)c * 64 + lane;
+        if (k < m) {
+          const f32 d = (f32)(f64)dr[c];
+          xv[c] = d > 0.0f ? xv[c] / d : 0.0f;
+          x[k] = xv[c];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < PER; ++c) {
+      const u32 k = (u32)c * 64 + lane;
+      if (k < m) {
+        xs[xp(k)] = xv[c];
+        acc = acc + xv[c];
+      }
+    }
+  } else {
+    for (u32 c = 0; c < m; c += 64) {
+      u32 k = c + lane;
+      if (k < m) {
+        f32 p = x[k];
+        if (divisor) {
+          f32 d = (f32)(f64)divisor[k];
+          p = d > 0.0f ? p / d : 0.0f;
+          x[k] = p;
+        }
+        xs[xp(k)] = p;
+        acc = acc + p;
+      }
     }
   }
   f32 T = readlane_f32(wave_scan_f32(acc), 63);
@@ -158,93 +231,117 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
     }
     return;
   }
+  wave_sync();
+  const f64 yT = 1.0 / (f64)T;
 
   // B1: fixed-point weights and their exact total Q (the capacity of one slot)
+  const u32 k0 = lane * per < m ? lane * per : m;
+  const u32 k1 = k0 + per < m ? k0 + per : m;
+  const u32 own = k0 + pad * lane;  // xp(k0): this lane's slots k0 + t sit at own + t
+  u64 qr[PER > 0 ? PER : 1];
   u64 qsum = 0;
-  for (u32 c = 0; c < m; c += 64) {
-    u32 k = c + lane;
-    qsum += k < m ? alias_fixed_weight(x[k], T, m) : 0;
+  if constexpr (PER > 0) {
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      qr[t] = (u32)t < per && k0 + t < k1 ? alias_fixed_weight_r(xs[own + t], T, yT, m) : 0;
+      qsum += qr[t];
+    }
+  } else {
+    for (u32 t = 0; k0 + t < k1; ++t) qsum += alias_fixed_weight_r(xs[own + t], T, yT, m);
   }
   const u64 Q = readlane_u64(wave_scan_u64(qsum), 63);
-  const u64 lt = lanemask_lt();
+  const f64 yQ = 1.0 / (f64)Q;
+  auto weight = [&](u32 t) -> u64 {  // a = m q of slot k0 + t
+    if constexpr (PER > 0) return qr[t] * m;
+    else return alias_fixed_weight_r(xs[own + t], T, yT, m) * m;
+  };
 
-  // B2: item weight a = m q; lights (a < Q) and heavies, exact prefix sums, compaction
-  u32 nh = 0, nl = 0;
-  u64 sE = 0, sD = 0;
-  for (u32 c = 0; c < m; c += 64) {
-    u32 k = c + lane;
-    bool valid = k < m;
-    u64 a = valid ? alias_fixed_weight(x[k], T, m) * m : 0;
-    bool heavy = valid && a >= Q;
-    bool light = valid && a < Q;
-    u64 hb = ballot(heavy);
-    u64 lb = ballot(light);
-    u64 sur = heavy ? a - Q : 0;
-    u64 def = light ? Q - a : 0;
-    u64 incS = wave_scan_u64(sur);
-    u64 incD = wave_scan_u64(def);
-    if (heavy) {
-      u32 pos = nh + (u32)__popcll(hb & lt);
-      hidx[pos] = (u16)k;
-      hE[pos] = sE + incS;
+  // B2: lane totals of surplus, deficit and class counts, placed by wave scans
+  u64 sur = 0, def = 0;
+  u32 nh_l = 0;
+  constexpr u32 kUnroll = PER > 0 ? (u32)PER : 1u;
+  const u32 steps = PER > 0 ? kUnroll : per;
+#pragma unroll
+  for (u32 t = 0; t < steps; ++t) {
+    if (k0 + t < k1) {
+      const u64 a = weight(t);
+      if (a >= Q) {
+        sur += a - Q;
+        ++nh_l;
+      } else {
+        def += Q - a;
+      }
     }
-    if (light) {
-      u32 pos = nl + (u32)__popcll(lb & lt);
-      *(lDr - pos) = sD + incD - def;
+  }
+  const u64 incS = wave_scan_u64(sur), incD = wave_scan_u64(def);
+  const u64 incH = wave_scan_u64((u64)nh_l);
+  const u32 nh = (u32)readlane_u64(incH, 63);
+  const u32 nl = m - nh;
+  const u64 sD = readlane_u64(incD, 63);
+  // B3: E (inclusive, heavies from the front) and D (exclusive, lights from the back); a light's
+  // probability a / Q replaces its row value
+  {
+    u64 E = incS - sur, D = incD - def;
+    u32 h = (u32)(incH - nh_l), l = (k0 - h);  // lights before this lane = slots before it - heavies
+#pragma unroll
+    for (u32 t = 0; t < steps; ++t) {
+      if (k0 + t < k1) {
+        const u32 k = k0 + t;
+        const u64 a = weight(t);
+        if (a >= Q) {
+          E += a - Q;
+          pre[h] = E;
+          idx[h] = (u16)k;
+          ++h;
+        } else {
+          pre[m - 1 - l] = D;
+          idx[m - 1 - l] = (u16)k;
+          xs[own + t] = (f32)div_by((f64)a, (f64)Q, yQ);
+          D += Q - a;
+          ++l;
+        }
+      }
     }
-    sE += readlane_u64(incS, 63);
-    sD += readlane_u64(incD, 63);
-    nh += (u32)__popcll(hb);
-    nl += (u32)__popcll(lb);
   }
   wave_sync();
 
-  // C: every slot independently
-  u32 hc = 0, lc = 0;
-  for (u32 c = 0; c < m; c += 64) {
-    u32 k = c + lane;
-    bool valid = k < m;
-    u64 a = valid ? alias_fixed_weight(x[k], T, m) * m : 0;
-    bool heavy = valid && a >= Q;
-    bool light = valid && a < Q;
-    u64 hb = ballot(heavy);
-    u64 lb = ballot(light);
+  // C: merge of D' = D + 1 (lights, A) and E (heavies, B), ties to A; this lane's outputs are
+  // merge positions [k0, k1)
+  u32 lo = k0 > nh ? k0 - nh : 0, hi = k0 < nl ? k0 : nl;
+  while (lo < hi) {
+    const u32 mid = (lo + hi) >> 1;
+    if (pre[m - 1 - mid] + 1 <= pre[k0 - 1 - mid]) lo = mid + 1;
+    else hi = mid;
+  }
+  u32 i = lo, j = k0 - lo;
+  for (u32 t = k0; t < k1; ++t) {
+    const bool take_light = j >= nh || (i < nl && pre[m - 1 - i] + 1 <= pre[j]);
     AliasSlot s;
-    s.prob = 1.0f;
-    s.alias = k;
-    // one binary search per lane, lights and heavies together: a light looks for the first heavy
-    // with E > D (E >= D + 1, exact in u64), a heavy for the first light with D >= E; both are
-    // "first index whose value is >= key" over an ascending array (heavies' E forward from hE,
-    // lights' D backward from lDr)
-    const u32 pos = light ? lc + (u32)__popcll(lb & lt) : hc + (u32)__popcll(hb & lt);
-    const u64 own = light ? *(lDr - pos) : heavy ? hE[pos] : 0;  // D of a light, E of a heavy
-    const u64 key = light ? own + 1 : own;
-    const u64* base = light ? hE : lDr;
-    const long step = light ? 1 : -1;
-    u32 lo = 0, hi = light ? nh : heavy ? nl : 0;
-    while (lo < hi) {
-      const u32 mid = (lo + hi) >> 1;
-      if (base[step * (long)mid] >= key) hi = mid;
-      else lo = mid + 1;
-    }
-    if (light) {
-      if (lo == nh) lo = nh - 1;
-      s.prob = (f32)((f64)a / (f64)Q);
-      s.alias = hidx[lo];
-    }
-    if (heavy) {
-      const u64 E = own;
-      const u64 dend = lo < nl ? *(lDr - lo) : sD;
-      if (dend > E && pos + 1 < nh) {
-        s.prob = (f32)((f64)(Q + E - dend) / (f64)Q);
-        s.alias = hidx[pos + 1];
+    u32 k;
+    if (take_light) {
+      // j heavies have E <= D_i: the first with E > D_i is heavy j (the last one if none)
+      k = idx[m - 1 - i];
+      s.prob = xs[xp(k)];
+      s.alias = idx[j < nh ? j : nh - 1];
+      ++i;
+    } else {
+      // i lights have D < E_j: the first with D >= E_j is light i
+      k = idx[j];
+      const u64 E = pre[j];
+      const u64 dend = i < nl ? pre[m - 1 - i] : sD;
+      s.prob = 1.0f;
+      s.alias = k;
+      if (dend > E && j + 1 < nh) {
+        s.prob = (f32)div_by((f64)(Q + E - dend), (f64)Q, yQ);  // Q + E - dend < Q
+        s.alias = idx[j + 1];
       }
+      ++j;
     }
-    if (valid) o[k] = s;
-    hc += (u32)__popcll(hb);
-    lc += (u32)__popcll(lb);
+    o[k] = s;
   }
 }
+template __global__ void k_alias_rows<16>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u32);
+template __global__ void k_alias_rows<0>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u32);
 
 // ---------------------------------------------------------------------------------------------
 // z-sweep, one wavefront per document, the doc-topic map resident in LDS.

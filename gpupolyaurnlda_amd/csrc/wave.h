@@ -80,14 +80,23 @@ __device__ __forceinline__ f32 wave_scan_f32(f32 v) {
   return v;
 }
 
-// Inclusive u64 scan (exact, so any association is fine).
+// Inclusive u64 scan over the same DPP network (integer sums are exact, so the network's
+// association gives the plain prefix sums): each step moves both 32-bit halves with the same DPP
+// control and adds them as one 64-bit value.
+template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
+__device__ __forceinline__ u64 dpp_u64(u64 v) {
+  const u32 lo = (u32)__builtin_amdgcn_update_dpp(0, (int)(u32)v, CTRL, ROW_MASK, 0xF, BOUND_ZERO);
+  const u32 hi = (u32)__builtin_amdgcn_update_dpp(0, (int)(u32)(v >> 32), CTRL, ROW_MASK, 0xF, BOUND_ZERO);
+  return ((u64)hi << 32) | lo;
+}
+
 __device__ __forceinline__ u64 wave_scan_u64(u64 v) {
-  const u32 lane = lane_id();
-#pragma unroll
-  for (u32 off = 1; off < 64; off <<= 1) {
-    u64 t = __shfl_up(v, off, 64);
-    if (lane >= off) v += t;
-  }
+  v += dpp_u64<0x111, 0xF, true>(v);  // row_shr:1
+  v += dpp_u64<0x112, 0xF, true>(v);  // row_shr:2
+  v += dpp_u64<0x114, 0xF, true>(v);  // row_shr:4
+  v += dpp_u64<0x118, 0xF, true>(v);  // row_shr:8
+  v += dpp_u64<0x142, 0xA, false>(v);  // row_bcast:15 into rows 1, 3
+  v += dpp_u64<0x143, 0xC, false>(v);  // row_bcast:31 into rows 2, 3
   return v;
 }
 
