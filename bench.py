@@ -218,13 +218,25 @@ def abi_rate(c, spec, iters):
     d = np.ascontiguousarray(c["d_len"], dtype=np.uint32)
     k_d = np.zeros(D, dtype=np.uint32)
     C_counts = np.bincount(w, minlength=spec.V).astype(np.uint32)
+    from gpupolyaurnlda_amd._lib import check, lib
+
+    modes = {}
     with Sampler(spec.alpha, spec.beta, spec.K, C_counts, N, D, seed=spec.seed) as smp:
-        train(smp, z, w, d, k_d, 1, N, D)
-        t0 = time.perf_counter()
-        train(smp, z, w, d, k_d, iters, N, D)
-        dt = (time.perf_counter() - t0) / iters
-    out = {"tokens_per_s": round(N / dt, 1), "ms_per_iteration": round(dt * 1e3, 3), "iterations": iters,
-           "buffer": "whole corpus in one pageable host buffer (train.rs loop; z narrowed to u16 in pinned memory, both ways per iteration)"}
+        # slice cache: 1 = reuse the device copy of w/d after verifying the whole host content
+        # (the default), 2 = trusted (host buffers unchanged between calls), 0 = upload w/d on
+        # every call as the reference does
+        for mode, name in [(1, "content_verified"), (2, "trusted"), (0, "reupload_every_call")]:
+            check(lib().gpulda_set_slice_cache(mode), "set_slice_cache")
+            train(smp, z, w, d, k_d, 1, N, D)
+            t0 = time.perf_counter()
+            train(smp, z, w, d, k_d, iters, N, D)
+            dt = (time.perf_counter() - t0) / iters
+            modes[name] = {"tokens_per_s": round(N / dt, 1), "ms_per_iteration": round(dt * 1e3, 3)}
+        check(lib().gpulda_set_slice_cache(1), "set_slice_cache")
+    out = dict(modes["content_verified"])
+    out.update({"iterations": iters, "slice_cache_modes": modes,
+                "buffer": "whole corpus in one pageable host buffer (train.rs loop; z narrowed to u16 in pinned "
+                          "memory, both ways per iteration)"})
     log(f"ABI path: {out}")
     return out
 

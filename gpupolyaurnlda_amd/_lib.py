@@ -86,6 +86,7 @@ _SIGS = {
     "gpulda_get_seed": (C.c_uint64, []),
     "gpulda_last_error": (C.c_char_p, []),
     "gpulda_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), u64p, C.POINTER(C.c_int)]),
+    "gpulda_serve_set_deadline": (C.c_int, [C.c_double]),
     "gpulda_set_slice_cache": (C.c_int, [C.c_int]),
     "gpulda_set_slice_cache_budget": (C.c_int, [C.c_uint64]),
     "gpulda_slice_cache_stats": (C.c_int, [u32p, u64p, u64p]),

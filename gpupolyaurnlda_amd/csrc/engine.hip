@@ -27,6 +27,7 @@
 #include "../../include/gpulda_abi.h"
 #include "../../include/gpulda_ext.h"
 #include "kernels.h"
+#include "serve_book.h"
 
 using namespace lda;
 
@@ -1582,6 +1583,8 @@ namespace {
 constexpr double kPoolLifetime = 2.0;  // seconds a pool keeps claiming requests
 constexpr int kPools = 3;              // pool streams (with the feeder's: four hardware queues)
 
+double g_serve_deadline = 30.0;  // seconds from publication to a lost report (gpulda_serve_set_deadline)
+
 struct ServePool {
   hipStream_t st = nullptr;
   double launched = 0.0;  // host seconds
@@ -1617,7 +1620,7 @@ struct ServeState {
   std::vector<u32> free_slots;
   std::vector<std::pair<u64, u64>> free_spans;  // (offset, length), sorted by offset
   std::vector<u32> pending;   // admitted, not published
-  std::vector<u32> inflight;  // published, not reported
+  ServeBook book;             // published requests: reported, lost, quarantined (serve_book.h)
   ServePool pools[kPools];
   hipStream_t feed = nullptr;
   u64 lds = 0;
@@ -1753,6 +1756,27 @@ int launch_pool(ServePool& pl) {
 }
 
 // Keeps a claiming pool on the device while requests are outstanding.
+// Lost requests the caller has collected, released while no pool kernel runs (no worker can touch
+// them): the ring entry (if it still names the slot) and the record are cleared on the device,
+// then the slot and its span are free again.
+int serve_release_lost() {
+  const std::vector<u32> rel = SV.book.release(true);
+  if (rel.empty()) return 0;
+  HCK(hipStreamSynchronize(SV.feed));  // no publication in flight
+  const u32 zero = 0;
+  for (u32 s : rel) {
+    const u32 pos = SV.pub_at[s] % SV.n_slots;
+    u32 e = 0;
+    HCK(hipMemcpy(&e, SV.d_ring + pos, sizeof(u32), hipMemcpyDeviceToHost));
+    if (e == s + 1u) HCK(hipMemcpy(SV.d_ring + pos, &zero, sizeof(u32), hipMemcpyHostToDevice));
+    HCK(hipMemcpy(SV.d_rec + (u64)kServeRecWords * s, &zero, sizeof(u32), hipMemcpyHostToDevice));
+    __atomic_store_n(&SV.h_done[s], 0u, __ATOMIC_RELEASE);
+    serve_free_span(SV.h_off[s], serve_span_tokens(SV.h_len[s]));
+    SV.free_slots.push_back(s);
+  }
+  return 0;
+}
+
 int serve_keep_pool() {
   const double now = host_seconds();
   double newest = -1.0;
@@ -1760,7 +1784,8 @@ int serve_keep_pool() {
     if (pl.active && hipStreamQuery(pl.st) == hipSuccess) pl.active = false;
     if (pl.active) newest = std::max(newest, pl.launched);
   }
-  if (SV.inflight.empty()) return 0;
+  if (newest < 0.0 && !SV.book.quarantine.empty()) CK(serve_release_lost());
+  if (SV.book.inflight.empty()) return 0;
   const bool none = newest < 0.0;
   const bool ageing = !none && now - newest > 0.5 * kPoolLifetime;
   if (!none && !ageing) return 0;
@@ -1808,6 +1833,8 @@ extern "C" int gpulda_serve_open(uint64_t hbm_budget_bytes, uint32_t sweeps, uin
   SV.expect.assign(SV.n_slots, 0);
   SV.wsum.assign(SV.n_slots, 0);
   SV.pub_at.assign(SV.n_slots, 0);
+  SV.book.reset(SV.n_slots);
+  SV.book.deadline = g_serve_deadline;
   SV.free_spans = {{0, SV.arena}};
   memset(SV.h_done, 0, SV.n_slots * sizeof(u32));
   memset(SV.h_drec, 0, (u64)kServeRecWords * SV.n_slots * sizeof(u32));
@@ -1876,7 +1903,8 @@ extern "C" int gpulda_serve_flush(void) {
     HCK(hipGetLastError());
     SV.meta_head += n;
     SV.tail += n;
-    SV.inflight.insert(SV.inflight.end(), SV.pending.begin(), SV.pending.end());
+    const double now = host_seconds();
+    for (u32 sl : SV.pending) SV.book.published(sl, now);
     SV.pending.clear();
   }
   return serve_keep_pool();
@@ -1885,24 +1913,22 @@ extern "C" int gpulda_serve_flush(void) {
 // Slots whose fold-in finished since the last poll (at most max).
 extern "C" int gpulda_serve_poll(uint32_t* done_slots, uint32_t max, uint32_t* n_done) {
   if (!SV.open) return fail("server not open");
-  u32 n = 0;
-  for (size_t i = 0; i < SV.inflight.size() && n < max;) {
-    const u32 sl = SV.inflight[i];
-    if (__atomic_load_n(&SV.h_done[sl], __ATOMIC_ACQUIRE)) {
-      done_slots[n++] = sl;
-      SV.inflight[i] = SV.inflight.back();
-      SV.inflight.pop_back();
-    } else {
-      ++i;
-    }
-  }
-  *n_done = n;
+  // finished slots, and slots whose request got no result within the deadline (collected as
+  // failed by gpulda_serve_result)
+  *n_done = SV.book.poll([](u32 sl) { return __atomic_load_n(&SV.h_done[sl], __ATOMIC_ACQUIRE) != 0u; },
+                         host_seconds(), done_slots, max);
   return serve_keep_pool();
 }
 
 // Copies a finished request's topics (u32) and frees its slot.
 extern "C" int gpulda_serve_result(uint32_t slot, uint32_t* z, uint32_t* request_id) {
   if (!SV.open) return fail("server not open");
+  if (slot < SV.n_slots && SV.book.lost[slot]) {
+    if (request_id) *request_id = SV.h_rid[slot];
+    SV.book.collected_lost(slot);  // slot and span come back once no worker runs
+    return fail("slot %u: request %u got no result within %.1f s of its publication", slot, SV.h_rid[slot],
+                SV.book.deadline);
+  }
   if (slot >= SV.n_slots || !__atomic_load_n(&SV.h_done[slot], __ATOMIC_ACQUIRE))
     return fail("slot %u has no finished request", slot);
   const u64 off = SV.h_off[slot];
@@ -1939,10 +1965,17 @@ extern "C" int gpulda_serve_result(uint32_t slot, uint32_t* z, uint32_t* request
   return rc;
 }
 
+extern "C" int gpulda_serve_set_deadline(double seconds) {
+  if (!(seconds > 0.0)) return fail("serve deadline must be positive");
+  g_serve_deadline = seconds;
+  if (SV.open) SV.book.deadline = seconds;
+  return 0;
+}
+
 extern "C" int gpulda_serve_stats(uint32_t* pending, uint32_t* inflight, uint64_t* free_tokens) {
   if (!SV.open) return fail("server not open");
   if (pending) *pending = (u32)SV.pending.size();
-  if (inflight) *inflight = (u32)SV.inflight.size();
+  if (inflight) *inflight = (u32)SV.book.inflight.size();
   if (free_tokens) {
     u64 f = 0;
     for (auto& sp : SV.free_spans) f += sp.second;

@@ -129,6 +129,10 @@ int gpulda_serve_flush(void);
 int gpulda_serve_poll(uint32_t* done_slots, uint32_t max, uint32_t* n_done);
 int gpulda_serve_result(uint32_t slot, uint32_t* z, uint32_t* request_id);
 int gpulda_serve_stats(uint32_t* pending, uint32_t* inflight, uint64_t* free_tokens);
+/* A published request that gets no result within `seconds` (default 30) of its publication is
+ * reported by gpulda_serve_poll and collected as failed by gpulda_serve_result; its slot and arena
+ * span return once no worker kernel runs.  No reference counterpart (the reference has no server). */
+int gpulda_serve_set_deadline(double seconds);
 int gpulda_serve_close(void);
 
 /* ---- stage entry points (host arrays in and out; the library stages device copies) ------- */

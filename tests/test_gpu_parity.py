@@ -453,6 +453,58 @@ def test_server_alone_equals_mixed(orc):
     assert not bad_mixed and not bad_alone, f"mixed wrong {bad_mixed}, alone wrong {bad_alone}"
 
 
+def test_server_deadline_reports_lost_and_recovers(orc):
+    """A request with no result within the deadline is reported by poll and collected as failed
+    (here a deadline of 1 us, so the request is reported while its worker may still run); its
+    slot is held until the pool kernels have ended, then serves the next request, whose result
+    equals the stage fold-in."""
+    import time
+    from gpupolyaurnlda_amd import corpus, stages
+    from gpupolyaurnlda_amd._lib import check, lib
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+    from gpupolyaurnlda_amd.serve import FoldInServer
+    spec = corpus.MEDIUM.scaled(2000)
+    c = corpus.generate(spec)
+    with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=1, z=c["z"]) as rc:
+        rc.iterate_untimed(3)
+        phi, sig = rc.phi(), rc.sigma_a()
+        prob, al = rc.alias()
+        d_len, _, w = corpus.requests(phi, 2, median=120, sigma=1.0, lo=10, hi=800, rate=1000.0, seed=5)
+        w0, w1 = w[:int(d_len[0])], w[int(d_len[0]):]
+        with FoldInServer(hbm_budget_bytes=64 << 20, sweeps=20, waves_per_cu=1, max_slots=1) as srv:
+            try:
+                check(lib().gpulda_serve_set_deadline(1e-6), "serve_set_deadline")
+                slot = srv.submit(w0, 0)
+                assert slot == 0
+                srv.flush()
+                got, t0 = [], time.time()
+                while not got and time.time() - t0 < 10:
+                    got = srv.poll()
+                assert got == [0]
+                with pytest.raises(RuntimeError, match="no result within"):
+                    srv.result(0, len(w0))
+            finally:
+                check(lib().gpulda_serve_set_deadline(30.0), "serve_set_deadline")
+            say("lost request reported; waiting for the pool to end")
+            slot, t0 = None, time.time()
+            while slot is None and time.time() - t0 < 20:  # the pool stops claiming after 2 s
+                srv.poll()
+                slot = srv.submit(w1, 1)
+                if slot is None:
+                    time.sleep(0.05)
+            assert slot == 0, "quarantined slot was not released"
+            srv.flush()
+            got, t0 = [], time.time()
+            while not got and time.time() - t0 < 20:
+                got = srv.poll()
+            assert got == [0]
+            z1, rid = srv.result(0, len(w1))
+    assert rid == 1
+    off = np.array([0, len(w1)], dtype=np.uint64)
+    zs, _ = stages.foldin(w1, off, np.array([1], dtype=np.uint32), 20, 1, spec.K, phi, sig, prob, al)
+    assert np.array_equal(z1, zs)
+
+
 def test_medium_config_parity_and_ll_trace(orc):
     """Medium workload shape (V=50k Zipf, K=256, lognormal lengths) on a 20k-document slice:
     three iterations bit-exact against the oracle, then a 60-iteration log-likelihood trace on
