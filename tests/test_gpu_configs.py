@@ -4,7 +4,8 @@
           iterations bit-exact (z, n_wk, phi), then a 200-iteration log-likelihood trace
   large   V=100k Zipf(1.07), K=1024, lognormal(200, 1.0) <= 4096 -- a 20,000-document slice under
           the persistent schedule the benchmark runs: two iterations, z, n_wk and the full V x K
-          phi bit-exact
+          phi bit-exact; and the whole corpus (328M tokens): exact n_wk, Phi / sigma_a / alias
+          tables of the third iteration equal the oracle's, and the first 2,000 documents' z
   batched V=100k, K=1024, power-law lengths on [4, 20000] -- a 20,000-document slice under the
           persistent + block-per-document and the persistent split schedules, eagerly and as a
           replayed hipGraph, against the oracle at every iteration
@@ -84,6 +85,56 @@ def test_large_slice_bit_exact_full_phi(orc):
             assert np.array_equal(rc.sigma_a().view(np.uint32), model.sigma_a.view(np.uint32))
             say(f"large slice iteration {it}: z, n_wk, phi, sigma_a, alias bit-exact")
         assert rc.visited() == model.visited
+
+
+def test_large_full_corpus_properties(orc):
+    """The whole large corpus (D = 1M documents, about 328M tokens) under the benchmark's schedule,
+    through properties that do not depend on its size: after two iterations n_wk equals the exact
+    count of the (w, z) pairs, every topic lies below K, each Phi column sums to 1 within f32
+    rounding, the alias tables are well formed; and the next iteration's Phi, sigma_a (alpha times
+    the canonical row sums), alias tables and the z of the first 2,000 documents equal the
+    oracle's, run from the same state with the full Phi."""
+    from gpupolyaurnlda_amd import corpus
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+    spec = corpus.LARGE
+    c = corpus.generate(spec)
+    N = len(c["w"])
+    say(f"large full corpus: D={spec.D} N={N}")
+    assert spec.D == 1_000_000 and N > 300_000_000
+    with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=spec.seed, z=c["z"]) as rc:
+        rc.schedule(1, 32)
+        rc.iterate_untimed(2)
+        z = rc.z()
+        n = rc.counts()
+        assert int(z.max()) < spec.K
+        want = np.bincount(c["w"].astype(np.int64) * spec.K + z, minlength=spec.V * spec.K)
+        assert np.array_equal(n.reshape(-1), want.astype(np.uint32)), "n_wk differs from the count of (w, z)"
+        del want
+        say("n_wk equals the exact count of the (w, z) pairs")
+        it = rc.iteration
+        rc.iterate_untimed(1)
+        phi = rc.phi()
+        col = phi.astype(np.float64).sum(axis=0)
+        used = col > 0
+        assert np.all(np.abs(col[used] - 1.0) < 1e-3), "a Phi column does not sum to 1"
+        sig = rc.sigma_a()
+        prob, al = rc.alias()
+        assert np.all((prob >= 0) & (prob <= 1)) and int(al.max()) < spec.K
+        z_gpu = rc.z()
+    tables = orc.poisson_tables(spec.beta)[2]
+    pt, ts = orc.phi_draw(n, spec.beta, it + 1, spec.seed, tables)
+    phi_c, sig_c, prob_c, al_c = orc.phi_finish(pt, ts, spec.alpha)
+    assert np.array_equal(phi_c.view(np.uint32), phi.view(np.uint32)), "Phi differs from the oracle's"
+    assert np.array_equal(sig_c.view(np.uint32), sig.view(np.uint32))
+    assert np.array_equal(prob_c.view(np.uint32), prob.view(np.uint32)) and np.array_equal(al_c, al)
+    say("Phi, sigma_a and alias tables of iteration 3 equal the oracle's")
+    D_s = 2000
+    off = c["doc_off"]
+    n_s = int(off[D_s])
+    z_s, _ = orc.sweep(z[:n_s], c["w"][:n_s], off[:D_s + 1], 0, it + 1, spec.seed, spec.K, phi_c, sig_c, prob_c, al_c,
+                       threads=16)
+    assert np.array_equal(z_s, z_gpu[:n_s]), "z of the first documents differs from the oracle's"
+    say(f"z of the first {D_s} documents ({n_s} tokens) equal the oracle's")
 
 
 @pytest.mark.parametrize("sched,graph", [(2, False), (2, True), (3, False), (3, True)])
