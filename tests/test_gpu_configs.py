@@ -6,6 +6,7 @@
           the persistent schedule the benchmark runs: two iterations, z, n_wk and the full V x K
           phi bit-exact; and the whole corpus (328M tokens): exact n_wk, Phi / sigma_a / alias
           tables of the third iteration equal the oracle's, and the first 2,000 documents' z
+          (the whole batched corpus likewise, under the split schedule as a replayed hipGraph)
   batched V=100k, K=1024, power-law lengths on [4, 20000] -- a 20,000-document slice under the
           persistent + block-per-document and the persistent split schedules, eagerly and as a
           replayed hipGraph, against the oracle at every iteration
@@ -87,22 +88,25 @@ def test_large_slice_bit_exact_full_phi(orc):
         assert rc.visited() == model.visited
 
 
-def test_large_full_corpus_properties(orc):
-    """The whole large corpus (D = 1M documents, about 328M tokens) under the benchmark's schedule,
-    through properties that do not depend on its size: after two iterations n_wk equals the exact
+@pytest.mark.parametrize("name,sched,min_len,graph", [("large", 1, 0, False), ("batched", 3, 2048, True)])
+def test_full_corpus_properties(orc, name, sched, min_len, graph):
+    """A whole corpus -- large (D = 1M documents, about 328M tokens) under the benchmark's
+    persistent schedule, batched (D = 2M, about 142M tokens) under the split schedule replayed as
+    a hipGraph -- through properties that do not depend on its size: after two iterations n_wk equals the exact
     count of the (w, z) pairs, every topic lies below K, each Phi column sums to 1 within f32
     rounding, the alias tables are well formed; and the next iteration's Phi, sigma_a (alpha times
     the canonical row sums), alias tables and the z of the first 2,000 documents equal the
     oracle's, run from the same state with the full Phi."""
     from gpupolyaurnlda_amd import corpus
     from gpupolyaurnlda_amd.resident import ResidentCorpus
-    spec = corpus.LARGE
+    spec = corpus.SPECS[name]
     c = corpus.generate(spec)
     N = len(c["w"])
-    say(f"large full corpus: D={spec.D} N={N}")
-    assert spec.D == 1_000_000 and N > 300_000_000
+    say(f"{name} full corpus: D={spec.D} N={N} schedule={sched} graph={graph}")
+    assert N > 100_000_000
     with ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=spec.seed, z=c["z"]) as rc:
-        rc.schedule(1, 32)
+        rc.schedule(sched, 32, min_len)
+        rc.use_graph(graph)
         rc.iterate_untimed(2)
         z = rc.z()
         n = rc.counts()
@@ -113,6 +117,7 @@ def test_large_full_corpus_properties(orc):
         say("n_wk equals the exact count of the (w, z) pairs")
         it = rc.iteration
         rc.iterate_untimed(1)
+        rc.use_graph(False)
         phi = rc.phi()
         col = phi.astype(np.float64).sum(axis=0)
         used = col > 0
