@@ -197,10 +197,14 @@ struct Segment {
   u32* w = nullptr;
   u16* z = nullptr;
   u64* doc_off = nullptr;
-  // n_wk reduction (launch_count): per token its rank inside its kCountTile tile in (word,
-  // position) order, per tile rank the token's place in word-sorted order, and z in that order
-  u16* crank = nullptr;
-  u32* cdst = nullptr;
+  // n_wk reduction (launch_count), two tile reorders: z -> z1 (word groups) -> zs (word-sorted);
+  // per pass the rank of each input entry inside its kCountTile tile and, per tile rank, the
+  // entry's place in the output
+  u16* crank1 = nullptr;
+  u32* cdst1 = nullptr;
+  u16* z1 = nullptr;
+  u16* crank2 = nullptr;
+  u32* cdst2 = nullptr;
   u16* zs = nullptr;
   CountTask* tasks = nullptr;
   u32* doc_order = nullptr;  // documents grouped by sweep bucket (longest first inside a bucket)
@@ -231,7 +235,9 @@ u64 g_part_min_tokens = 4ull << 20;  // smallest pipeline part (gpulda_set_part_
 int launch_count(const Segment* sg, u32* n, u32 K, hipStream_t st) {
   if (sg->n_tasks == 0) return 0;
   const u64 tiles = (sg->n_tokens + kCountTile - 1) / kCountTile;
-  k_z_word_order<<<(u32)tiles, 1024, 0, st>>>(sg->z, sg->crank, sg->cdst, sg->zs, sg->n_tokens);
+  k_tile_reorder<<<(u32)tiles, 1024, 0, st>>>(sg->z, sg->crank1, sg->cdst1, sg->z1, sg->n_tokens);
+  HCK(hipGetLastError());
+  k_tile_reorder<<<(u32)tiles, 1024, 0, st>>>(sg->z1, sg->crank2, sg->cdst2, sg->zs, sg->n_tokens);
   HCK(hipGetLastError());
   const u32 wpb = 4;
   u64 lds = (u64)wpb * K * sizeof(u32);
@@ -272,8 +278,11 @@ void free_segment(Segment* s) {
   dfree(s->w);
   dfree(s->z);
   dfree(s->doc_off);
-  dfree(s->crank);
-  dfree(s->cdst);
+  dfree(s->crank1);
+  dfree(s->cdst1);
+  dfree(s->z1);
+  dfree(s->crank2);
+  dfree(s->cdst2);
   dfree(s->zs);
   dfree(s->tasks);
   dfree(s->doc_order);
@@ -360,22 +369,58 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
     }
   }
   if (n_tokens >= (1ull << 32)) return fail("a slice must hold fewer than 2^32 tokens");
-  // word-sorted order (words ascending, positions ascending within a word) -> tile ranks and
-  // destinations of the word-order pass: walking that order, the tokens of each tile come up in
-  // (word, position) order, so a per-tile counter gives their ranks
-  std::vector<u16> crank(n_tokens);
-  std::vector<u32> cdst(n_tokens);
+  // Ranks and destinations of the two tile reorders of the n_wk reduction.  Word groups are
+  // contiguous word ranges of about n_tokens / kCountGroups tokens (a word with more is a group of
+  // its own).  z1 order: groups ascending, corpus order inside a group; word-sorted order (zs):
+  // words ascending, corpus order inside a word.  Walking an output order, the entries of each
+  // input tile come up in output order, so a per-tile counter gives their ranks.
+  const u64 n_tiles = (n_tokens + kCountTile - 1) / kCountTile;
+  std::vector<u16> crank1(n_tokens), crank2(n_tokens);
+  std::vector<u32> cdst1(n_tokens), cdst2(n_tokens);
   {
-    std::vector<u32> perm(n_tokens);
-    std::vector<u64> cur(start.begin(), start.end() - 1);
-    for (u64 i = 0; i < n_tokens; ++i) perm[cur[w[i]]++] = (u32)i;
-    std::vector<u32> tile_fill((n_tokens + kCountTile - 1) / kCountTile, 0);
-    for (u64 p = 0; p < n_tokens; ++p) {
-      const u32 i = perm[p];
+    std::vector<u32> gid(V);
+    std::vector<u64> gstart(1, 0);
+    const u64 budget = std::max<u64>(1, (n_tokens + kCountGroups - 1) / kCountGroups);
+    u64 acc = 0;
+    for (u32 v = 0; v < V; ++v) {
+      const u64 cnt = start[(u64)v + 1] - start[v];
+      if (acc > 0 && acc + cnt > budget) {
+        gstart.push_back(gstart.back() + acc);
+        acc = 0;
+      }
+      gid[v] = (u32)(gstart.size() - 1);
+      acc += cnt;
+    }
+    std::vector<u32> p1(n_tokens), inv1(n_tokens);
+    {
+      std::vector<u64> cur(gstart.begin(), gstart.end());
+      for (u64 i = 0; i < n_tokens; ++i) {
+        const u32 q = (u32)cur[gid[w[i]]]++;
+        p1[i] = q;
+        inv1[q] = (u32)i;
+      }
+    }
+    std::vector<u32> fill(n_tiles, 0);
+    for (u64 q = 0; q < n_tokens; ++q) {  // level 1: z (document order) -> z1
+      const u32 i = inv1[q];
       const u64 t = i / kCountTile;
-      const u32 r = tile_fill[t]++;
-      crank[i] = (u16)r;
-      cdst[t * kCountTile + r] = (u32)p;
+      const u32 r = fill[t]++;
+      crank1[i] = (u16)r;
+      cdst1[t * kCountTile + r] = (u32)q;
+    }
+    std::vector<u32>().swap(inv1);
+    std::vector<u32> perm(n_tokens);
+    {
+      std::vector<u64> cur(start.begin(), start.end() - 1);
+      for (u64 i = 0; i < n_tokens; ++i) perm[cur[w[i]]++] = (u32)i;
+    }
+    std::fill(fill.begin(), fill.end(), 0u);
+    for (u64 p = 0; p < n_tokens; ++p) {  // level 2: z1 -> zs (word-sorted)
+      const u32 q = p1[perm[p]];
+      const u64 t = q / kCountTile;
+      const u32 r = fill[t]++;
+      crank2[q] = (u16)r;
+      cdst2[t * kCountTile + r] = (u32)p;
     }
   }
   // sweep buckets: by register-resident line count, largest first; longest documents first
@@ -437,14 +482,19 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
   CK(dalloc(&s->w, n_tokens));
   CK(dalloc(&s->z, n_tokens));
   CK(dalloc(&s->doc_off, (u64)n_docs + 1));
-  CK(dalloc(&s->crank, n_tokens));
-  CK(dalloc(&s->cdst, n_tokens));
+  CK(dalloc(&s->crank1, n_tokens));
+  CK(dalloc(&s->cdst1, n_tokens));
+  CK(dalloc(&s->z1, n_tokens));
+  CK(dalloc(&s->crank2, n_tokens));
+  CK(dalloc(&s->cdst2, n_tokens));
   CK(dalloc(&s->zs, n_tokens));
   CK(dalloc(&s->tasks, tasks.size()));
   if (n_tokens) {
     HCK(hipMemcpy(s->w, w, n_tokens * sizeof(u32), hipMemcpyHostToDevice));
-    HCK(hipMemcpy(s->crank, crank.data(), n_tokens * sizeof(u16), hipMemcpyHostToDevice));
-    HCK(hipMemcpy(s->cdst, cdst.data(), n_tokens * sizeof(u32), hipMemcpyHostToDevice));
+    HCK(hipMemcpy(s->crank1, crank1.data(), n_tokens * sizeof(u16), hipMemcpyHostToDevice));
+    HCK(hipMemcpy(s->cdst1, cdst1.data(), n_tokens * sizeof(u32), hipMemcpyHostToDevice));
+    HCK(hipMemcpy(s->crank2, crank2.data(), n_tokens * sizeof(u16), hipMemcpyHostToDevice));
+    HCK(hipMemcpy(s->cdst2, cdst2.data(), n_tokens * sizeof(u32), hipMemcpyHostToDevice));
   }
   HCK(hipMemcpy(s->doc_off, off.data(), off.size() * sizeof(u64), hipMemcpyHostToDevice));
   if (!tasks.empty()) HCK(hipMemcpy(s->tasks, tasks.data(), tasks.size() * sizeof(CountTask), hipMemcpyHostToDevice));
@@ -751,8 +801,8 @@ bool segment_pinned(const Segment* s) {
 }
 
 u64 segment_bytes(const Segment* s, u64 n_tasks) {
-  // w, z, crank, cdst, zs per token
-  return s->n_tokens * (sizeof(u32) + sizeof(u16) + sizeof(u16) + sizeof(u32) + sizeof(u16)) +
+  // w, z, and per count pass rank, destination and output (z1, zs) per token
+  return s->n_tokens * (sizeof(u32) + sizeof(u16) + 2 * (sizeof(u16) + sizeof(u32) + sizeof(u16))) +
          ((u64)s->n_docs + 1) * sizeof(u64) +
          (u64)s->n_docs * 3 * sizeof(u32) + n_tasks * sizeof(CountTask);
 }

@@ -12,8 +12,10 @@ struct CountTask {
   u32 word;  // word id of the run
 };
 constexpr u32 kCountTask = 4096;
-// Tokens per tile of the word-order pass (one workgroup: the tile's topics staged in 64 KiB of LDS).
+// Tokens per tile of the word-order passes (one workgroup: the tile's topics staged in 64 KiB of
+// LDS), and the word groups of the first pass.
 constexpr u32 kCountTile = 32768;
+constexpr u32 kCountGroups = 256;
 
 struct SweepParams {
   const u32* w;          // word ids, doc-major
@@ -112,7 +114,7 @@ inline int sweep_bucket_lmax(u32 lines) {
     if (lines <= (u32)b) return b;
   return 0;
 }
-__global__ void k_z_word_order(const u16* z, const u16* rank, const u32* dst, u16* zs, u64 n_tokens);
+__global__ void k_tile_reorder(const u16* in, const u16* rank, const u32* dst, u16* out, u64 n);
 __global__ void k_count(const CountTask* tasks, u32 n_tasks, const u16* zs, u32* n, u32 K, u32 waves_per_block);
 
 __global__ void k_stage_philox(const u32* ctr, const u32* key, u32* out, u32 n);

@@ -1199,25 +1199,50 @@ __global__ __launch_bounds__(64) void k_serve_worker(SweepParams P, FoldParams F
 }
 
 // ---------------------------------------------------------------------------------------------
-// n_wk reduction in two passes, both reading and writing HBM in order.
-//   k_z_word_order  z (document order) -> zs (word-sorted order: words ascending, each word's
-//                   tokens in corpus order).  One workgroup per tile of kCountTile consecutive
-//                   tokens: each topic goes to LDS at its token's rank inside the tile (the
-//                   tile's tokens ordered by word, then position), and the tile's word runs are
-//                   written out to their places in zs.  rank and dst are fixed by the words, so
-//                   they are built once per corpus.
+// n_wk reduction: z to word order, then per-word histograms, every pass reading and writing HBM
+// in order.
+//   k_tile_reorder  one workgroup per tile of kCountTile consecutive entries of `in`: each entry
+//                   goes to LDS at its rank inside the tile, and the tile is written out in rank
+//                   order to out[dst[...]], a run at a time.  Ranks and destinations are fixed by
+//                   the words, so they are built once per corpus.  It runs twice: z (document
+//                   order) -> z1 (word groups: contiguous word ranges of about N / kCountGroups
+//                   tokens; tiles write runs of about kCountTile / kCountGroups tokens), then
+//                   z1 -> zs (word-sorted: words ascending, each word's tokens in corpus order;
+//                   a tile of z1 holds one or two groups, so each word writes about one run per
+//                   tile of its group).  One pass straight to word order would write one run per
+//                   token of every word rarer than one token per tile.
 //   k_count         one wavefront per task (a run of at most kCountTask tokens of one word, a
 //                   contiguous range of zs): LDS histogram over K, nonzero bins added to n_wk.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_z_word_order(const u16* __restrict__ z, const u16* __restrict__ rank,
-                                                       const u32* __restrict__ dst, u16* __restrict__ zs,
-                                                       u64 n_tokens) {
+__global__ __launch_bounds__(1024) void k_tile_reorder(const u16* __restrict__ in, const u16* __restrict__ rank,
+                                                       const u32* __restrict__ dst, u16* __restrict__ out, u64 n) {
   __shared__ u16 tile[kCountTile];
-  const u64 t0 = (u64)blockIdx.x * kCountTile;
-  const u32 n = (u32)(n_tokens - t0 < kCountTile ? n_tokens - t0 : kCountTile);
-  for (u32 l = threadIdx.x; l < n; l += blockDim.x) tile[rank[t0 + l]] = z[t0 + l];
+  const u64 t0 = (u64)blockIdx.x * kCountTile;  // tiles start 64 KiB apart in `in` and `rank`
+  const u32 m = (u32)(n - t0 < kCountTile ? n - t0 : kCountTile);
+  const u32 m4 = m & ~3u;
+  // four entries per lane with 8-byte loads of values and ranks
+  for (u32 l = 4 * threadIdx.x; l < m4; l += 4 * blockDim.x) {
+    const uint2 v = *reinterpret_cast<const uint2*>(in + t0 + l);
+    const uint2 r = *reinterpret_cast<const uint2*>(rank + t0 + l);
+    tile[r.x & 0xFFFFu] = (u16)(v.x & 0xFFFFu);
+    tile[r.x >> 16] = (u16)(v.x >> 16);
+    tile[r.y & 0xFFFFu] = (u16)(v.y & 0xFFFFu);
+    tile[r.y >> 16] = (u16)(v.y >> 16);
+  }
+  for (u32 l = m4 + threadIdx.x; l < m; l += blockDim.x) tile[rank[t0 + l]] = in[t0 + l];
   __syncthreads();
-  for (u32 j = threadIdx.x; j < n; j += blockDim.x) zs[dst[t0 + j]] = tile[j];
+  // consecutive lanes write consecutive ranks (a run's entries go to consecutive places); four
+  // independent destinations per lane in flight
+  const u32 B = blockDim.x;
+  u32 j = threadIdx.x;
+  for (; j + 3 * B < m; j += 4 * B) {
+    const u32 d0 = dst[t0 + j], d1 = dst[t0 + j + B], d2 = dst[t0 + j + 2 * B], d3 = dst[t0 + j + 3 * B];
+    out[d0] = tile[j];
+    out[d1] = tile[j + B];
+    out[d2] = tile[j + 2 * B];
+    out[d3] = tile[j + 3 * B];
+  }
+  for (; j < m; j += B) out[dst[t0 + j]] = tile[j];
 }
 
 __global__ void k_count(const CountTask* __restrict__ tasks, u32 n_tasks, const u16* __restrict__ zs,
