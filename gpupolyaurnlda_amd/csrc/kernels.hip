@@ -550,16 +550,14 @@ __device__ __forceinline__ u32 column_pick(const u32* T, u32 lines, u32 ls, cons
   return last;
 }
 
-// Sparse/dense pick with the NL lines of the map held in registers (one variant per line count):
-// all phi gathers of the token are issued before any arithmetic on them.
+// Sparse/dense pick with the NL lines of the map held in registers (one variant per line count;
+// e[j] is line j as this lane sees it): all phi gathers of the token are issued before any
+// arithmetic on them.
 template <int NL>
-__device__ __forceinline__ Pick draw_topic_regs(const u32* T, const PhiRow& prow, f32 sa, f32 ux, f32 uw,
+__device__ __forceinline__ Pick draw_topic_regs(const u32 (&e)[NL], const PhiRow& prow, f32 sa, f32 ux, f32 uw,
                                                 u32 dense) {
   const u32 lane = lane_id();
-  u32 e[NL];
   f32 m[NL];
-#pragma unroll
-  for (int j = 0; j < NL; ++j) e[j] = T[j * 64 + lane];
 #pragma unroll
   for (int j = 0; j < NL; ++j) m[j] = slot_mass(e[j], prow);
   // G_g starts at line g's mass (0 + m == m, masses are never -0); with fewer than four lines the
@@ -614,6 +612,15 @@ __device__ __forceinline__ Pick draw_topic_regs(const u32* T, const PhiRow& prow
   pk.topic = map_key(pk.entry);
   pk.slot = js * 64 + ls;
   return pk;
+}
+
+template <int NL>
+__device__ __forceinline__ Pick draw_topic_regs(const u32* T, const PhiRow& prow, f32 sa, f32 ux, f32 uw,
+                                                u32 dense) {
+  u32 e[NL];
+#pragma unroll
+  for (int j = 0; j < NL; ++j) e[j] = T[j * 64 + lane_id()];
+  return draw_topic_regs<NL>(e, prow, sa, ux, uw, dense);
 }
 
 // Sparse/dense pick for any number of lines: one pass for the column sums, then for a sparse draw
@@ -714,19 +721,56 @@ __device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, u32* T, DocSta
   for (; t < nt; ++t) {
     const u32 wt = readlane_u32(ch.ww, t);
     const u32 zt = readlane_u32(ch.zz, t);
-    ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, zt));
     const f32 sa = readlane_f32(ch.sa, t);
     Pick pk;
     pk.topic = zt;
     pk.slot = kNoSlot;
-    if (sa > 0.0f) {
+    if constexpr (NL > 0) {
+      // the whole map in registers: the decremented key is found by ballots over its lines and,
+      // unless its count reaches 0, decremented in place (its line rewritten to LDS, every lane
+      // its own slot); a count reaching 0 takes the map's backward-shift delete, then a re-read
       wave_sync();
-      visited += ds.occupied;
-      const PhiRow prow = phi_row(P.phi + (u64)wt * K, K);
-      const f32 ux = readlane_f32(ch.ux, t), uw = readlane_f32(ch.uw, t);
-      const u32 dense = readlane_u32(ch.dense, t);
-      if constexpr (NL > 0) pk = draw_topic_regs<NL>(T, prow, sa, ux, uw, dense);
-      else pk = draw_topic_loop(T, uniform_u32(ds.cap / kMapLine), prow, sa, ux, uw, dense);
+      u32 e[NL];
+#pragma unroll
+      for (int j = 0; j < NL; ++j) e[j] = T[j * 64 + lane];
+      u32 js = 0, ls = 0, entry = 0;
+#pragma unroll
+      for (int j = 0; j < NL; ++j) {
+        const u64 b = ballot(!map_is_empty(e[j])) & ballot(map_key(e[j]) == zt);
+        if (b) {  // the key is in the map exactly once
+          js = (u32)j;
+          ls = first_lane(b);
+          entry = readlane_u32(e[j], ls);
+        }
+      }
+      if (map_count(entry) > 1) {
+#pragma unroll
+        for (int j = 0; j < NL; ++j)
+          if ((u32)j == js) {
+            if (lane == ls) e[j] = entry - 1;
+            T[j * 64 + lane] = e[j];
+          }
+      } else {
+        ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, zt));
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < NL; ++j) e[j] = T[j * 64 + lane];
+      }
+      if (sa > 0.0f) {
+        visited += ds.occupied;
+        const PhiRow prow = phi_row(P.phi + (u64)wt * K, K);
+        const f32 ux = readlane_f32(ch.ux, t), uw = readlane_f32(ch.uw, t);
+        pk = draw_topic_regs<NL>(e, prow, sa, ux, uw, readlane_u32(ch.dense, t));
+      }
+    } else {
+      ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, zt));
+      if (sa > 0.0f) {
+        wave_sync();
+        visited += ds.occupied;
+        const PhiRow prow = phi_row(P.phi + (u64)wt * K, K);
+        const f32 ux = readlane_f32(ch.ux, t), uw = readlane_f32(ch.uw, t);
+        pk = draw_topic_loop(T, uniform_u32(ds.cap / kMapLine), prow, sa, ux, uw, readlane_u32(ch.dense, t));
+      }
     }
     add_topic(P, T, ds, pk, c0, t, ch.znew, ch.zz);
     if (lane == t) ch.znew = pk.topic;
