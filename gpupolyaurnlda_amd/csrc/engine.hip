@@ -337,8 +337,8 @@ u64 hash_chunk(const u32* p, u64 n, u64 index) {
   return h;
 }
 
-// Builds device copies of a slice: w, doc offsets, the word-sorted permutation (counting sort)
-// and the n_wk reduction tasks.  z is allocated, not filled.
+// Builds device copies of a slice: w, doc offsets, the document queues, the ranks and places of
+// the n_wk reduction's two tile reorders and its per-word tasks.  z is allocated, not filled.
 int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_tokens, u32 V, u32 K) {
   std::vector<u64> off((u64)n_docs + 1);
   off[0] = 0;

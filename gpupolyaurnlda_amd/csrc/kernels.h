@@ -7,7 +7,7 @@ namespace lda {
 
 // One word-sorted run of tokens for the n_wk reduction.
 struct CountTask {
-  u64 p0;    // first position in the word-sorted permutation
+  u64 p0;    // first position in word-sorted order (the topics of the run are zs[p0 .. p0 + len))
   u32 len;   // number of tokens (<= kCountTask)
   u32 word;  // word id of the run
 };
