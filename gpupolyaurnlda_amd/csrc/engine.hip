@@ -251,7 +251,7 @@ int launch_buckets(SweepParams P, const std::vector<Segment::Bucket>& buckets, c
   const u64 lim = (u64)max_lds_per_block();
   for (const Segment::Bucket& bk : buckets) {
     if (bk.count == 0) continue;
-    u64 lds = ((u64)bk.max_cap + (P.K + 31) / 32) * sizeof(u32);  // map + distinct-topic bitmap
+    u64 lds = (u64)doc_counts_bytes(P.K) + (u64)bk.max_cap * 2;  // counts + key table
     if (lds > lim) return fail("doc-topic map of %u slots exceeds LDS", bk.max_cap);
     P.doc_order = order + bk.first;
     switch (bk.lmax) {
@@ -703,8 +703,8 @@ SweepParams sweep_params(Segment* sg, u32 doc_base) {
   return P;
 }
 
-// LDS words one wavefront needs for any document of the segment (map + distinct-topic bitmap).
-u64 sweep_lds_bytes(const Segment* sg) { return ((u64)map_capacity(sg->max_len, S.K) + (S.K + 31) / 32) * 4; }
+// LDS bytes one wavefront needs for any document of the segment (counts + key table).
+u64 sweep_lds_bytes(const Segment* sg) { return doc_state_bytes(sg->max_len, S.K); }
 
 // Resident sweep under the configured schedule:
 //   buckets              one launch per length bucket (k_sweep_lines<LMAX>), longest maps first;
@@ -740,7 +740,7 @@ int launch_sweep_schedule(const SweepParams& P0, Segment* sg, hipStream_t st) {
     if (n_long < sg->n_docs) {
       SweepParams Q = P;
       Q.doc_order = sg->doc_lpt + n_long;
-      const u64 lds_s = ((u64)map_capacity(sg->lens_desc[n_long], S.K) + (S.K + 31) / 32) * 4;
+      const u64 lds_s = doc_state_bytes(sg->lens_desc[n_long], S.K);
       const u32 per_cu = std::max<u32>(1, std::min<u32>(S.waves_per_cu, (u32)(lds_per_cu() / lds_s)));
       const u32 waves = std::min<u32>((u32)S.cu_count * per_cu, sg->n_docs - n_long);
       k_sweep_persistent<8><<<waves, 64, lds_s, st>>>(Q, S.heads, sg->n_docs - n_long);
@@ -784,7 +784,7 @@ int launch_part_sweep(const SweepParams& P0, const Segment* sg, u32 p, u32* head
   if (n == 0) return 0;
   SweepParams P = P0;
   P.doc_order = sg->part_lpt + pt.doc0;
-  const u64 lds = ((u64)map_capacity(pt.max_len, S.K) + (S.K + 31) / 32) * 4;
+  const u64 lds = doc_state_bytes(pt.max_len, S.K);
   if (lds > (u64)max_lds_per_block()) return fail("doc-topic map exceeds LDS");
   const u32 per_cu = std::max<u32>(1, std::min<u32>(kAbiWavesPerCu, (u32)(lds_per_cu() / lds)));
   const u32 waves = std::min<u32>((u32)S.cu_count * per_cu, n);
@@ -1853,7 +1853,7 @@ extern "C" int gpulda_serve_open(uint64_t hbm_budget_bytes, uint32_t sweeps, uin
   SV.n_slots = max_slots ? max_slots : 65536;
   SV.waves_per_cu = std::max<u32>(1, waves_per_cu);
   SV.arena = std::max<u64>(hbm_budget_bytes / (sizeof(u32) + sizeof(u16)), 1);  // words and topics per token
-  SV.lds = ((u64)map_capacity(S.K, S.K) + (S.K + 31) / 32) * 4;
+  SV.lds = doc_state_bytes(S.K, S.K);
   if (SV.lds > (u64)max_lds_per_block()) return fail("doc-topic map exceeds LDS");
   int dev = 0, rate_khz = 0;
   HCK(hipGetDevice(&dev));
@@ -2106,15 +2106,28 @@ extern "C" int gpulda_stage_scan64(const float* in, float* out, uint32_t n_group
 
 extern "C" int gpulda_stage_map(const uint32_t* ops, uint32_t n_ops, uint32_t cap, uint32_t K, uint32_t* table_out) {
   if (cap == 0 || cap % kMapLine) return fail("map capacity must be a positive multiple of 64");
-  if (K == 0 || K > kMaxTopics) return fail("K = %u out of range", K);
+  if (K == 0 || K > kMaxSweepTopics) return fail("K = %u out of range (1..%u)", K, kMaxSweepTopics);
+  if ((u64)doc_counts_bytes(K) + (u64)cap * 2 > (u64)max_lds_per_block()) return fail("map capacity exceeds LDS");
   for (u32 i = 0; i < n_ops; ++i)
     if ((ops[i] & 0x7FFFFFFFu) >= K) return fail("op %u names topic %u >= K", i, ops[i] & 0x7FFFFFFFu);
-  if ((u64)cap * 4 > (u64)max_lds_per_block()) return fail("map capacity exceeds LDS");
+  {  // the table keeps a free slot: at most cap - 1 keys at any time
+    std::vector<u32> cnt(K, 0);
+    u32 keys = 0;
+    for (u32 i = 0; i < n_ops; ++i) {
+      const u32 k = ops[i] & 0x7FFFFFFFu;
+      if (ops[i] >> 31) {
+        if (cnt[k] > 0 && --cnt[k] == 0) --keys;
+      } else {
+        if (cnt[k] == 0xFFFFu) return fail("op %u: count of topic %u past 65535", i, k);
+        if (cnt[k]++ == 0 && ++keys >= cap) return fail("op %u: %u keys leave no free slot of %u", i, keys, cap);
+      }
+    }
+  }
   Scratch sc;
   u32 *dops, *dout;
   CK(sc.upload(&dops, ops, n_ops));
   CK(sc.alloc(&dout, cap));
-  k_stage_map<<<1, 64, (u64)cap * 4>>>(dops, n_ops, cap, K, dout);
+  k_stage_map<<<1, 64, (u64)doc_counts_bytes(K) + (u64)cap * 2>>>(dops, n_ops, cap, K, dout);
   HCK(hipGetLastError());
   HCK(hipMemcpy(table_out, dout, (u64)cap * 4, hipMemcpyDeviceToHost));
   return 0;
@@ -2159,10 +2172,16 @@ extern "C" int gpulda_stage_pick(const uint32_t* table, uint32_t cap, uint32_t K
                                  float sigma_a, const float* u_decide, const float* u_target, uint32_t count,
                                  int use_regs, uint32_t* slot_out) {
   if (cap == 0 || cap % kMapLine) return fail("map capacity must be a positive multiple of 64");
-  if (K == 0 || K > kMaxTopics) return fail("K = %u out of range", K);
-  if ((u64)cap * 4 > (u64)max_lds_per_block()) return fail("map capacity exceeds LDS");
-  for (u32 s = 0; s < cap; ++s)
-    if (map_count(table[s]) != 0 && map_key(table[s]) >= K) return fail("slot %u names topic %u >= K", s, map_key(table[s]));
+  if (K == 0 || K > kMaxSweepTopics) return fail("K = %u out of range (1..%u)", K, kMaxSweepTopics);
+  if ((u64)doc_counts_bytes(K) + (u64)cap * 2 > (u64)max_lds_per_block()) return fail("map capacity exceeds LDS");
+  {  // occupied slots name distinct topics below K (the device state holds one count per topic)
+    std::vector<u8> seen(K, 0);
+    for (u32 s = 0; s < cap; ++s) {
+      if (map_count(table[s]) == 0) continue;
+      if (map_key(table[s]) >= K) return fail("slot %u names topic %u >= K", s, map_key(table[s]));
+      if (seen[map_key(table[s])]++) return fail("slot %u repeats topic %u", s, map_key(table[s]));
+    }
+  }
   if (use_regs && cap / kMapLine > 8) return fail("register variants hold at most 8 lines");
   Scratch sc;
   u32 *dt, *dout;
@@ -2173,7 +2192,8 @@ extern "C" int gpulda_stage_pick(const uint32_t* table, uint32_t cap, uint32_t K
   CK(sc.upload(&duw, u_target, count));
   CK(sc.alloc(&dout, count));
   if (count) {
-    k_stage_pick<<<count, 64, (u64)cap * 4>>>(dt, cap, K, drow, sigma_a, dux, duw, count, use_regs ? 1u : 0u, dout);
+    k_stage_pick<<<count, 64, (u64)doc_counts_bytes(K) + (u64)cap * 2>>>(dt, cap, K, drow, sigma_a, dux, duw, count,
+                                                                          use_regs ? 1u : 0u, dout);
     HCK(hipGetLastError());
     HCK(hipMemcpy(slot_out, dout, (u64)count * 4, hipMemcpyDeviceToHost));
   }
@@ -2236,7 +2256,7 @@ extern "C" int gpulda_stage_sweep(uint32_t* z, const uint32_t* w, const uint64_t
                                   uint32_t doc_base, uint32_t iter, uint64_t seed, uint32_t K, uint32_t V,
                                   const float* phi, const float* sigma_a, const float* prob, const uint32_t* alias,
                                   uint64_t* visited) {
-  if (K == 0 || K > kMaxTopics) return fail("bad K");
+  if (K == 0 || K > kMaxSweepTopics) return fail("bad K");
   const u64 N = doc_off[n_docs];
   std::vector<u32> lens(n_docs);
   for (u32 d = 0; d < n_docs; ++d) {
@@ -2344,7 +2364,7 @@ extern "C" int gpulda_stage_foldin(const uint32_t* w, const uint64_t* doc_off, u
                                    uint32_t sweeps, uint64_t seed, uint32_t K, uint32_t V, const float* phi,
                                    const float* sigma_a, const float* prob, const uint32_t* alias, uint32_t* z_out,
                                    uint64_t* visited) {
-  if (K == 0 || K > kMaxTopics) return fail("bad K");
+  if (K == 0 || K > kMaxSweepTopics) return fail("bad K");
   const u64 N = doc_off[n_req];
   std::vector<u32> lens(n_req);
   for (u32 r = 0; r < n_req; ++r) {
@@ -2400,7 +2420,7 @@ extern "C" int gpulda_stage_foldin(const uint32_t* w, const uint64_t* doc_off, u
   F.done_value = 0;
   u32 maxL = 0;
   for (u32 L : lens) maxL = std::max(maxL, L);
-  const u64 lds = ((u64)map_capacity(maxL, K) + (K + 31) / 32) * 4;
+  const u64 lds = doc_state_bytes(maxL, K);
   if (lds > (u64)max_lds_per_block()) return fail("doc-topic map exceeds LDS");
   if (n_req) {
     k_foldin<8><<<n_req, 64, lds>>>(P, F);
@@ -2465,7 +2485,7 @@ extern "C" int gpulda_foldin_noalloc(const uint32_t* w, const uint64_t* doc_off,
     return fail("foldin_noalloc: %llu tokens in %u requests need %llu bytes of working memory (limit %llu: 6 bytes "
                 "per token, 16 per request)",
                 (unsigned long long)N, n_req, (unsigned long long)need, (unsigned long long)kNoAllocScratchBytes);
-  const u64 lds = ((u64)map_capacity(maxL, S.K) + (S.K + 31) / 32) * 4;
+  const u64 lds = doc_state_bytes(maxL, S.K);
   if (lds > (u64)max_lds_per_block()) return fail("doc-topic map exceeds LDS");
   unsigned char* base = nullptr;
   CK(scratch_on_current_device(&base));

@@ -344,18 +344,24 @@ template __global__ void k_alias_rows<16>(f32*, u32, u32, const unsigned long lo
 template __global__ void k_alias_rows<0>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u32);
 
 // ---------------------------------------------------------------------------------------------
-// z-sweep, one wavefront per document, the doc-topic map resident in LDS.
+// z-sweep, one wavefront per document, the doc-topic state resident in LDS: a count per topic and
+// the Robin Hood key table of the topics present (wave.h).
 // Per token i (word w, old topic z), with r = Philox(i, doc, iter, SWEEP):
-//   remove z; if sigma_a[w] == 0 (phi_w. is all zero) keep z; else
-//   m[j][l] = count * phi[w][key] of map slot j*64 + l (line j, lane l; 0 when empty);
+//   remove z (count - 1; the topic leaves the table at 0); if sigma_a[w] == 0 (phi_w. is all zero)
+//   keep z; else
+//   m[j][l] = count * phi[w][key] of table slot j*64 + l (line j, lane l; 0 when empty);
 //   G_g = column sums of line group g = j % 4, lines left to right; A = ((G_0 + G_1) + G_2) + G_3;
 //   P = canonical scan of A; sigma_b = P[63];
 //   sparse if u(r.x) * (sigma_a + sigma_b) > sigma_a: target t = u(r.w) * sigma_b, lane l* = the
 //   first with A > 0 and P > t (else the last with A > 0), E = P[l* - 1]; Q = canonical scan of
 //   column l*'s masses in line order (64 lines per scan, carry = total of the earlier scans);
 //   the line is the first with m > 0 and E + (carry + Q_j) > t (else the last with m > 0);
-//   otherwise the alias table of w (slot from r.y, keep test on r.z); add the new topic.
+//   otherwise the alias table of w (slot from r.y, keep test on r.z); add the new topic (count + 1;
+//   it enters the table at 1).
 // Each lane sums its own column, so a token costs one wave scan however many lines the map has.
+// A token's removal is made at the end of the token before it, in the same LDS instruction as that
+// token's addition, so a token that changes no key set (most of them) finds its lines in registers
+// and never searches the table.
 // ---------------------------------------------------------------------------------------------
 constexpr u32 kNoSlot = 0xFFFFFFFFu;
 
@@ -374,110 +380,30 @@ __device__ __forceinline__ PhiRow phi_row(const f32* prow, u32 K) {
   return r;
 }
 
-__device__ __forceinline__ f32 row_at(const PhiRow& row, u32 key) {
-  return __builtin_bit_cast(f32, __builtin_amdgcn_raw_buffer_load_b32(row.rs, (int)(key * sizeof(f32)), 0, 0));
+// phi[w][topic] for a table value key2 = 2 * topic (byte offset 2 * key2).
+__device__ __forceinline__ f32 row_at2(const PhiRow& row, u32 key2) {
+  return __builtin_bit_cast(f32, __builtin_amdgcn_raw_buffer_load_b32(row.rs, (int)(key2 * 2u), 0, 0));
 }
 
-// Mass of one map entry.  An empty slot holds count 0 at key 0xFFFF (kMapEmpty), so the read and the
-// product are unconditional (no branch, all reads of a token in flight together): the read returns 0
-// without a memory access, and the mass is 0 * 0 = +0, the value the definition gives it.
-__device__ __forceinline__ f32 slot_mass(u32 e, const PhiRow& row) {
-  static_assert((kMapEmpty & 0xFFFFu) == 0u && (kMapEmpty >> 16) == 0xFFFFu, "empty slots read past the row");
-  return (f32)map_count(e) * row_at(row, map_key(e));
+// Mass of one table slot, count * phi: an empty slot (2K) reads the zero count C[K] and a phi offset
+// past the row, so its mass is 0 * 0 = +0, the value the definition gives it, with no branch (all
+// reads of a token in flight together).
+__device__ __forceinline__ f32 slot_mass(const u16* C, u32 key2, const PhiRow& row) {
+  return (f32)count_at(C, key2) * row_at2(row, key2);
 }
 
 __device__ __forceinline__ f32 combine_groups(f32 g0, f32 g1, f32 g2, f32 g3) { return ((g0 + g1) + g2) + g3; }
 
-// Number of distinct topics among z[b .. b+L): a K-bit LDS bitmap, then a popcount reduction.
-__device__ __forceinline__ u32 distinct_topics(u32* bits, u32 K, const u16* __restrict__ z, u64 b, u32 L) {
-  const u32 lane = lane_id();
-  const u32 words = (K + 31) / 32;
-  for (u32 i = lane; i < words; i += 64) bits[i] = 0;
-  wave_sync();
-  for (u32 i = lane; i < L; i += 64) {
-    u32 t = z[b + i];
-    atomicOr(&bits[t >> 5], 1u << (t & 31));
-  }
-  wave_sync();
-  u32 c = 0;
-  for (u32 i = lane; i < words; i += 64) c += (u32)__popc(bits[i]);
-  for (u32 off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
-  return c;
-}
-
-// Builds the doc-topic map of one document from its current topics (count_topics,
-// src/cu/topics.cuh:142-158).  Every distinct topic (set in the bitmap `bits`, `distinct` of them)
-// is inserted once, with count 1, so the layout -- a function of the key set and the capacity --
-// is the one token-by-token inserts give.  Then each token adds one to its topic's slot (a
-// per-lane probe and an LDS atomic), and the extra 1 of the inserts is taken back.
-__device__ __forceinline__ void build_doc_map(u32* T, u32 cap, const u32* bits, u32 K, u32 distinct,
-                                              const u16* __restrict__ z, u64 b, u32 L) {
-  const u32 lane = lane_id();
-  const u32 hm = map_home_mult(K);
-  for (u32 s = lane; s < cap; s += 64) T[s] = kMapEmpty;
-  // 64 bitmap words per step; only words with a topic set are visited
-  const u32 words = (K + 31) / 32;
-  for (u32 w0 = 0; w0 < words; w0 += 64) {
-    wave_sync();
-    const u32 mine = w0 + lane < words ? bits[w0 + lane] : 0u;
-    u64 nz = ballot(mine != 0u);
-    while (nz) {
-      const u32 l = first_lane(nz);
-      nz &= nz - 1ull;
-      u32 bw = readlane_u32(mine, l);
-      while (bw) {
-        const u32 bit = (u32)__builtin_ctz(bw);
-        bw &= bw - 1u;
-        map_inc(T, cap, hm, (w0 + l) * 32u + bit);
-      }
-    }
-  }
-  wave_sync();
-  if (distinct == 1) {
-    // one topic holds all L tokens (1 + L would not fit the 16-bit count at L = 65535)
-    for (u32 s = lane; s < cap; s += 64)
-      if (!map_is_empty(T[s])) T[s] = map_entry(map_key(T[s]), L);
-    return;
-  }
-  // with two or more topics no topic has more than L - 1 <= 65534 tokens, so 1 + count fits
-  for (u32 c0 = 0; c0 < L; c0 += 64) {
-    if (c0 + lane < L) {
-      const u32 key = z[b + c0 + lane];
-      u32 s = map_home(key, cap, hm);
-      for (u32 n = 0; n < cap; ++n) {  // the key is present, so the probe reaches it first
-        const u32 e = T[s];
-        if (!map_is_empty(e) && map_key(e) == key) {
-          atomicAdd(&T[s], 1u);
-          break;
-        }
-        s = s + 1u == cap ? 0u : s + 1u;
-      }
-    }
-  }
-  wave_sync();
-  for (u32 s = lane; s < cap; s += 64)
-    if (!map_is_empty(T[s])) T[s] -= 1u;
-}
-
-// Regrows the map to capacity ncap in the middle of a sweep: clears it and re-inserts the topic
-// of every token except token t of the current chunk c0.  Tokens before c0 hold their new topics
-// in z, lanes < t of the chunk in znew_lane, lanes > t in zz, and tokens after the chunk their
-// old topics in z -- exactly the multiset the map held, so (the layout being a function of the
-// key set) the result equals any other way of moving the entries.
-__device__ __noinline__ void regrow_doc_map(u32* T, u32 ncap, u32 K, const u16* __restrict__ z, u64 b, u32 L,
-                                            u32 c0, u32 t, u32 znew_lane, u32 zz) {
-  const u32 lane = lane_id();
-  const u32 hm = map_home_mult(K);
-  wave_sync();
-  for (u32 s = lane; s < ncap; s += 64) T[s] = kMapEmpty;
-  for (u32 c = 0; c < L; c += 64) {
-    u32 nt = L - c < 64 ? L - c : 64;
-    u32 v;
-    if (c == c0) v = lane < t ? znew_lane : zz;
-    else v = c + lane < L ? (u32)z[b + c + lane] : 0;
-    for (u32 tt = 0; tt < nt; ++tt)
-      if (!(c == c0 && tt == t)) map_inc(T, ncap, hm, readlane_u32(v, tt));
-  }
+// The LDS of one document: counts at the start, the key table after them.
+struct DocLds {
+  u16* C;
+  u16* Kt;
+};
+__device__ __forceinline__ DocLds doc_lds(u32* smem, u32 K) {
+  DocLds d;
+  d.C = (u16*)smem;
+  d.Kt = (u16*)((char*)smem + doc_counts_bytes(K));
+  return d;
 }
 
 // Per-document state of a sweep.
@@ -490,35 +416,38 @@ struct DocState {
   u32 occupied;   // distinct topics in the map
   u32 gdoc;       // global document id (Philox counter word 1)
   u32 hm;         // map_home_mult(K)
+  u32 e2;         // table value of an empty slot (2K)
 };
 
-// Capacity rule: start sized for the n0 distinct topics; grow (rebuild) when a new topic would
-// exceed the load limit.  The reference starts from the K_d hint (hashmap.cuh:98).
-__device__ __forceinline__ DocState open_span(const SweepParams& P, u64 b, u32 L, u32 gdoc, u32* T) {
+// Counts the document's topics (count_topics, src/cu/topics.cuh:142-158) and builds the key
+// table sized for its n0 distinct topics; the map grows (rebuilds) when a new topic would exceed
+// the load limit.  The reference starts from the K_d hint (hashmap.cuh:98).
+__device__ __forceinline__ DocState open_span(const SweepParams& P, u64 b, u32 L, u32 gdoc, u32* smem) {
   DocState ds;
+  const DocLds m = doc_lds(smem, P.K);
   ds.b = uniform_u64(b);
   ds.L = uniform_u32(L);
   ds.cap_max = map_capacity(ds.L, P.K);
   ds.kmax = ds.L < P.K ? ds.L : P.K;
   ds.gdoc = uniform_u32(gdoc);
   ds.hm = map_home_mult(P.K);
-  ds.occupied = uniform_u32(distinct_topics(T + ds.cap_max, P.K, P.z, ds.b, ds.L));
+  ds.e2 = 2u * P.K;
+  ds.occupied = doc_count_topics(m.C, P.K, P.z, ds.b, ds.L);
   ds.cap = map_capacity(ds.occupied, P.K);
-  build_doc_map(T, ds.cap, T + ds.cap_max, P.K, ds.occupied, P.z, ds.b, ds.L);
+  keymap_build(m.Kt, ds.cap, ds.hm, m.C, P.K);
   return ds;
 }
 
-__device__ __forceinline__ DocState open_doc(const SweepParams& P, u32 doc, u32* T) {
+__device__ __forceinline__ DocState open_doc(const SweepParams& P, u32 doc, u32* smem) {
   const u64 b = P.doc_off[doc];
-  return open_span(P, b, (u32)(P.doc_off[doc + 1] - b), P.doc_base + doc, T);
+  return open_span(P, b, (u32)(P.doc_off[doc + 1] - b), P.doc_base + doc, smem);
 }
 
-// Outcome of the draw for one token: the new topic, and for a sparse pick the map slot (and its
-// entry) that holds it, so the increment needs no probe.
-struct Pick {
-  u32 topic;
-  u32 slot;   // kNoSlot unless sparse
-  u32 entry;
+// Outcome of the draw for one token: the table value of the sparse pick (kNoSlot for a dense draw)
+// and its slot.
+struct Draw {
+  u32 key2;
+  u32 slot;
 };
 
 // Lane l* of the sparse pick and its exclusive prefix E.
@@ -532,14 +461,14 @@ __device__ __forceinline__ u32 pick_lane(f32 A, f32 P, f32 tgt, f32& E) {
 
 // In-column search over lines read transposed (lane k holds line c0 + k of column ls), one scan
 // per 64 lines.  Returns the picked line.
-__device__ __forceinline__ u32 column_pick(const u32* T, u32 lines, u32 ls, const PhiRow& prow, f32 E,
-                                           f32 tgt) {
+__device__ __forceinline__ u32 column_pick(const u16* Kt, const u16* C, u32 lines, u32 ls, const PhiRow& prow,
+                                           f32 E, f32 tgt, u32 e2) {
   const u32 lane = lane_id();
   f32 carry = 0.0f;
   u32 last = 0;
   for (u32 c0 = 0; c0 < lines; c0 += 64) {
     const u32 j = c0 + lane;
-    const f32 m = slot_mass(j < lines ? T[j * 64 + ls] : kMapEmpty, prow);
+    const f32 m = slot_mass(C, j < lines ? (u32)Kt[j * 64 + ls] : e2, prow);
     const f32 Q = wave_scan_f32(m);
     const u64 pos = ballot(m > 0.0f);
     const u64 hit = pos & ballot(E + (carry + Q) > tgt);
@@ -550,16 +479,22 @@ __device__ __forceinline__ u32 column_pick(const u32* T, u32 lines, u32 ls, cons
   return last;
 }
 
-// Sparse/dense pick with the NL lines of the map held in registers (one variant per line count;
-// e[j] is line j as this lane sees it): all phi gathers of the token are issued before any
-// arithmetic on them.
+// Sparse/dense pick with the NL lines of the table held in registers (one variant per line count;
+// e[j] is line j's value in this lane's column): all phi gathers of the token are issued before
+// any arithmetic on them.
 template <int NL>
-__device__ __forceinline__ Pick draw_topic_regs(const u32 (&e)[NL], const PhiRow& prow, f32 sa, f32 ux, f32 uw,
-                                                u32 dense) {
-  const u32 lane = lane_id();
+__device__ __forceinline__ Draw draw_topic_regs(const u32 (&e)[NL], const u16* C, const PhiRow& prow, f32 sa,
+                                                f32 ux, f32 uw) {
+  // every count read and phi gather of the token issued before any arithmetic on them
+  u32 c[NL];
+#pragma unroll
+  for (int j = 0; j < NL; ++j) c[j] = count_at(C, e[j]);
+  f32 f[NL];
+#pragma unroll
+  for (int j = 0; j < NL; ++j) f[j] = row_at2(prow, e[j]);
   f32 m[NL];
 #pragma unroll
-  for (int j = 0; j < NL; ++j) m[j] = slot_mass(e[j], prow);
+  for (int j = 0; j < NL; ++j) m[j] = (f32)c[j] * f[j];
   // G_g starts at line g's mass (0 + m == m, masses are never -0); with fewer than four lines the
   // empty groups add +0, which leaves the sum's bits unchanged, so those adds are dropped
   f32 G[4];
@@ -573,11 +508,10 @@ __device__ __forceinline__ Pick draw_topic_regs(const u32 (&e)[NL], const PhiRow
   if (NL >= 4) A = A + G[3];
   const f32 P = wave_scan_f32(A);
   const f32 sb = readlane_f32(P, 63);
-  Pick pk;
-  pk.topic = dense;
-  pk.slot = kNoSlot;
-  pk.entry = 0;
-  if (!(ux * (sa + sb) > sa)) return pk;
+  Draw d;
+  d.key2 = kNoSlot;
+  d.slot = kNoSlot;
+  if (!(ux * (sa + sb) > sa)) return d;
   const f32 tgt = uw * sb;
   f32 E;
   const u32 ls = pick_lane(A, P, tgt, E);
@@ -608,34 +542,42 @@ __device__ __forceinline__ Pick draw_topic_regs(const u32 (&e)[NL], const PhiRow
 #pragma unroll
   for (int j = 1; j < NL; ++j)
     if ((u32)j == js) es = e[j];
-  pk.entry = readlane_u32(es, ls);
-  pk.topic = map_key(pk.entry);
-  pk.slot = js * 64 + ls;
-  return pk;
+  d.key2 = readlane_u32(es, ls);
+  d.slot = js * 64 + ls;
+  return d;
 }
 
+// The register variant with its lines loaded from the table.
 template <int NL>
-__device__ __forceinline__ Pick draw_topic_regs(const u32* T, const PhiRow& prow, f32 sa, f32 ux, f32 uw,
-                                                u32 dense) {
+__device__ __forceinline__ Draw draw_topic_regs_lds(const DocLds& m, const PhiRow& prow, f32 sa, f32 ux, f32 uw) {
   u32 e[NL];
 #pragma unroll
-  for (int j = 0; j < NL; ++j) e[j] = T[j * 64 + lane_id()];
-  return draw_topic_regs<NL>(e, prow, sa, ux, uw, dense);
+  for (int j = 0; j < NL; ++j) e[j] = m.Kt[j * 64 + lane_id()];
+  return draw_topic_regs<NL>(e, m.C, prow, sa, ux, uw);
 }
 
 // Sparse/dense pick for any number of lines: one pass for the column sums, then for a sparse draw
 // the in-column search over column l* read transposed.
-__device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const PhiRow& prow, f32 sa,
-                                                f32 ux, f32 uw, u32 dense) {
+__device__ __forceinline__ Draw draw_topic_loop(const u16* Kt, const u16* C, u32 lines, const PhiRow& prow, f32 sa,
+                                                f32 ux, f32 uw, u32 e2) {
   const u32 lane = lane_id();
   f32 G0 = 0.0f, G1 = 0.0f, G2 = 0.0f, G3 = 0.0f;
   // eight lines per step, so eight gathers are in flight at once; full steps read without a
   // bounds test, the last partial step reads an empty slot past the map's end
   u32 j = 0;
   for (; j + 8 <= lines; j += 8) {
+    u32 e[8];
+#pragma unroll
+    for (u32 k = 0; k < 8; ++k) e[k] = Kt[(j + k) * 64 + lane];
+    u32 c[8];
+#pragma unroll
+    for (u32 k = 0; k < 8; ++k) c[k] = count_at(C, e[k]);
+    f32 f[8];
+#pragma unroll
+    for (u32 k = 0; k < 8; ++k) f[k] = row_at2(prow, e[k]);
     f32 m[8];
 #pragma unroll
-    for (u32 k = 0; k < 8; ++k) m[k] = slot_mass(T[(j + k) * 64 + lane], prow);
+    for (u32 k = 0; k < 8; ++k) m[k] = (f32)c[k] * f[k];
     G0 = G0 + m[0];
     G1 = G1 + m[1];
     G2 = G2 + m[2];
@@ -646,9 +588,18 @@ __device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const P
     G3 = G3 + m[7];
   }
   if (j < lines) {
+    u32 e[8];
+#pragma unroll
+    for (u32 k = 0; k < 8; ++k) e[k] = j + k < lines ? (u32)Kt[(j + k) * 64 + lane] : e2;
+    u32 c[8];
+#pragma unroll
+    for (u32 k = 0; k < 8; ++k) c[k] = count_at(C, e[k]);
+    f32 f[8];
+#pragma unroll
+    for (u32 k = 0; k < 8; ++k) f[k] = row_at2(prow, e[k]);
     f32 m[8];
 #pragma unroll
-    for (u32 k = 0; k < 8; ++k) m[k] = slot_mass(j + k < lines ? T[(j + k) * 64 + lane] : kMapEmpty, prow);
+    for (u32 k = 0; k < 8; ++k) m[k] = (f32)c[k] * f[k];
     G0 = G0 + m[0];
     G1 = G1 + m[1];
     G2 = G2 + m[2];
@@ -661,149 +612,161 @@ __device__ __forceinline__ Pick draw_topic_loop(const u32* T, u32 lines, const P
   const f32 A = combine_groups(G0, G1, G2, G3);
   const f32 P = wave_scan_f32(A);
   const f32 sb = readlane_f32(P, 63);
-  Pick pk;
-  pk.topic = dense;
-  pk.slot = kNoSlot;
-  pk.entry = 0;
-  if (!(ux * (sa + sb) > sa)) return pk;
+  Draw d;
+  d.key2 = kNoSlot;
+  d.slot = kNoSlot;
+  if (!(ux * (sa + sb) > sa)) return d;
   const f32 tgt = uw * sb;
   f32 E;
   const u32 ls = pick_lane(A, P, tgt, E);
-  pk.slot = column_pick(T, lines, ls, prow, E, tgt) * 64 + ls;
-  pk.entry = T[pk.slot];
-  pk.topic = map_key(pk.entry);
-  return pk;
+  d.slot = column_pick(Kt, C, lines, ls, prow, E, tgt, e2) * 64 + ls;
+  d.key2 = Kt[d.slot];
+  return d;
 }
 
-// Adds the drawn topic to the map: a sparse pick increments the slot it came from; otherwise the
-// topic is probed for and inserted when absent, the map growing first if the new key would
-// exceed its load limit.
-__device__ __forceinline__ void add_topic(const SweepParams& P, u32* T, DocState& ds, const Pick& pk, u32 c0, u32 t,
-                                          u32 znew_lane, u32 zz) {
-  if (pk.slot != kNoSlot) {
-    if (lane_id() == 0) T[pk.slot] = pk.entry + 1;
-    return;
+// Count changes at the end of a token: +1 for its new topic (lane 0) and, when the chunk holds a
+// next token, -1 for that token's old topic (lane 1), both in one read and one write of the counts
+// (nothing when the two topics are equal).  Reports whether the new topic entered (its count was
+// 0) and whether the next token's topic left (its count reached 0).
+struct CountStep {
+  u32 entered;
+  u32 left;
+};
+__device__ __forceinline__ CountStep count_step(u16* C, u32 new2, u32 next2, u32 has_next) {
+  const u32 lane = lane_id();
+  const u32 n = has_next ? (next2 == new2 ? 0u : 2u) : 1u;
+  u32 old = 1u;
+  if (lane < n) {
+    const u32 a = lane == 0 ? new2 : next2;
+    old = count_at(C, a);
+    count_put(C, a, old + (lane == 0 ? 1u : 0xFFFFu));  // 16-bit: + 0xFFFF is - 1
   }
-  MapProbe pz = map_probe(T, ds.cap, ds.hm, pk.topic);
-  if (!pz.found) {
+  CountStep cs;
+  cs.entered = (n != 0u && readlane_u32(old, 0) == 0u) ? 1u : 0u;
+  cs.left = (n == 2u && readlane_u32(old, 1) == 1u) ? 1u : 0u;
+  return cs;
+}
+
+// A token's key-set changes, in the order of the definition: the new topic enters the table (the
+// table growing first when it would pass its load limit, rebuilt from the counts, which already
+// hold both changes: the layout is a function of the key set), then the next token's old topic
+// leaves it.  Returns true when the capacity changed.
+__device__ __forceinline__ bool apply_key_changes(const SweepParams& P, const DocLds& m, DocState& ds,
+                                                  const CountStep& cs, u32 new2, u32 next2) {
+  if (cs.entered) {
     if (map_capacity(ds.occupied + 1, P.K) > ds.cap) {
       const u32 want = 2 * (ds.occupied + 1) < ds.kmax ? 2 * (ds.occupied + 1) : ds.kmax;
       ds.cap = map_capacity(want, P.K);
-          regrow_doc_map(T, ds.cap, P.K, P.z, ds.b, ds.L, c0, t, znew_lane, zz);
-      pz = map_probe(T, ds.cap, ds.hm, pk.topic);
+      ds.occupied = ds.occupied + 1 - cs.left;
+      wave_sync();
+      keymap_build(m.Kt, ds.cap, ds.hm, m.C, P.K);
+      return true;
     }
+    keymap_insert(m.Kt, ds.cap, ds.hm, ds.e2, new2);
     ++ds.occupied;
   }
-  map_inc_at(T, ds.cap, pk.topic, pz);
+  if (cs.left) {
+    keymap_erase(m.Kt, ds.cap, ds.hm, ds.e2, next2);
+    --ds.occupied;
+  }
+  return false;
+}
+
+// Removal of the first token of a chunk (the others are removed at the end of the token before).
+__device__ __forceinline__ void remove_first(const DocLds& m, DocState& ds, u32 z2) {
+  u32 old = 0;
+  wave_sync();
+  if (lane_id() == 0) {
+    old = count_at(m.C, z2);
+    count_put(m.C, z2, old - 1u);
+  }
+  if (readlane_u32(old, 0) == 1u) {
+    keymap_erase(m.Kt, ds.cap, ds.hm, ds.e2, z2);
+    --ds.occupied;
+  }
 }
 
 // Per-lane values of the 64 tokens of one chunk: lane l holds token c0 + l.
 struct Chunk {
-  u32 ww;     // word
-  u32 zz;     // topic before the pass
-  u32 dense;  // alias-table (dense) draw
-  f32 sa;     // sigma_a of the word
-  f32 ux;     // uniform of the sparse/dense choice
-  f32 uw;     // uniform of the sparse target
-  u32 znew;   // topic after the pass (lanes below the current token)
+  u32 ww;      // word
+  u32 zz2;     // 2 * topic before the pass
+  u32 dense2;  // 2 * alias-table (dense) draw
+  f32 sa;      // sigma_a of the word
+  f32 ux;      // uniform of the sparse/dense choice
+  f32 uw;      // uniform of the sparse target
+  u32 znew;    // topic after the pass (lanes below the current token)
 };
 
-// Tokens t .. nt-1 of a chunk while the map has NL lines (NL = 1..8: register variant; 0: line
-// loop).  Returns the next token: nt, or the token after the one whose insertion grew the map.
-// All variants evaluate the same operations in the same order: a variant for fewer lines drops
-// only lines that would hold no mass.
+// Tokens t .. nt-1 of a chunk while the table has NL lines (NL = 1..8: lines held in registers;
+// 0: read per token).  Returns the next token: nt, or the token after the one whose insertion
+// grew the map.  All variants evaluate the same operations in the same order: a variant for fewer
+// lines drops only lines that would hold no mass.
 template <int NL>
-__device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, u32* T, DocState& ds, Chunk& ch, u32 c0, u32 t,
+__device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, const DocLds& m, DocState& ds, Chunk& ch, u32 t,
                                             u32 nt, u32& visited) {
   const u32 lane = lane_id();
   const u32 K = P.K;
-  const u32 cap0 = ds.cap;
+  u32 e[NL > 0 ? NL : 1];
+  if constexpr (NL > 0) {
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < NL; ++j) e[j] = m.Kt[j * 64 + lane];
+  }
   for (; t < nt; ++t) {
     const u32 wt = readlane_u32(ch.ww, t);
-    const u32 zt = readlane_u32(ch.zz, t);
     const f32 sa = readlane_f32(ch.sa, t);
-    Pick pk;
-    pk.topic = zt;
-    pk.slot = kNoSlot;
-    if constexpr (NL > 0) {
-      // the whole map in registers: the decremented key is found by ballots over its lines and,
-      // unless its count reaches 0, decremented in place (its line rewritten to LDS, every lane
-      // its own slot); a count reaching 0 takes the map's backward-shift delete, then a re-read
+    u32 new2 = readlane_u32(ch.zz2, t);  // a word with an all-zero phi row keeps its topic
+    wave_sync();  // the previous token's count writes before this token's reads
+    if (sa > 0.0f) {
+      visited += ds.occupied;
+      const PhiRow prow = phi_row(P.phi + (u64)wt * K, K);
+      const f32 ux = readlane_f32(ch.ux, t), uw = readlane_f32(ch.uw, t);
+      Draw d;
+      if constexpr (NL > 0) d = draw_topic_regs<NL>(e, m.C, prow, sa, ux, uw);
+      else d = draw_topic_loop(m.Kt, m.C, uniform_u32(ds.cap / kMapLine), prow, sa, ux, uw, ds.e2);
+      new2 = d.key2 != kNoSlot ? d.key2 : readlane_u32(ch.dense2, t);
+    }
+    if (lane == t) ch.znew = new2 >> 1;
+    const u32 has_next = t + 1 < nt ? 1u : 0u;
+    const u32 next2 = readlane_u32(ch.zz2, t + 1 < 64 ? t + 1 : 63);
+    const CountStep cs = count_step(m.C, new2, next2, has_next);
+    if (cs.entered | cs.left) {
       wave_sync();
-      u32 e[NL];
+      if (apply_key_changes(P, m, ds, cs, new2, next2)) return t + 1;
+      if constexpr (NL > 0) {
 #pragma unroll
-      for (int j = 0; j < NL; ++j) e[j] = T[j * 64 + lane];
-      u32 js = 0, ls = 0, entry = 0;
-#pragma unroll
-      for (int j = 0; j < NL; ++j) {
-        const u64 b = ballot(!map_is_empty(e[j])) & ballot(map_key(e[j]) == zt);
-        if (b) {  // the key is in the map exactly once
-          js = (u32)j;
-          ls = first_lane(b);
-          entry = readlane_u32(e[j], ls);
-        }
-      }
-      if (map_count(entry) > 1) {
-#pragma unroll
-        for (int j = 0; j < NL; ++j)
-          if ((u32)j == js) {
-            if (lane == ls) e[j] = entry - 1;
-            T[j * 64 + lane] = e[j];
-          }
-      } else {
-        ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, zt));
-        wave_sync();
-#pragma unroll
-        for (int j = 0; j < NL; ++j) e[j] = T[j * 64 + lane];
-      }
-      if (sa > 0.0f) {
-        visited += ds.occupied;
-        const PhiRow prow = phi_row(P.phi + (u64)wt * K, K);
-        const f32 ux = readlane_f32(ch.ux, t), uw = readlane_f32(ch.uw, t);
-        pk = draw_topic_regs<NL>(e, prow, sa, ux, uw, readlane_u32(ch.dense, t));
-      }
-    } else {
-      ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, zt));
-      if (sa > 0.0f) {
-        wave_sync();
-        visited += ds.occupied;
-        const PhiRow prow = phi_row(P.phi + (u64)wt * K, K);
-        const f32 ux = readlane_f32(ch.ux, t), uw = readlane_f32(ch.uw, t);
-        pk = draw_topic_loop(T, uniform_u32(ds.cap / kMapLine), prow, sa, ux, uw, readlane_u32(ch.dense, t));
+        for (int j = 0; j < NL; ++j) e[j] = m.Kt[j * 64 + lane];
       }
     }
-    add_topic(P, T, ds, pk, c0, t, ch.znew, ch.zz);
-    if (lane == t) ch.znew = pk.topic;
-    if (ds.cap != cap0) return t + 1;
   }
   return nt;
 }
 
-// Runs tokens t .. nt-1 of a chunk with the variant for the map's current line count (register
-// variants up to 8 lines, within the kernel's LMAX, which caps its registers; larger maps use
-// the line loop).  The choice is made again only when an insertion grows the map.
+// Runs tokens t .. nt-1 of a chunk with the variant for the table's current line count (register
+// variants up to 8 lines, within the kernel's LMAX, which caps its registers; larger tables use
+// the line loop).  The choice is made again only when an insertion grows the table.
 template <int LMAX>
-__device__ __forceinline__ u32 sweep_tokens_any(const SweepParams& P, u32* T, DocState& ds, Chunk& ch, u32 c0,
+__device__ __forceinline__ u32 sweep_tokens_any(const SweepParams& P, const DocLds& m, DocState& ds, Chunk& ch,
                                                 u32 t, u32 nt, u32& visited) {
   switch (ds.cap / kMapLine) {  // a map always has at least one line
-    case 1: if (LMAX >= 1) return sweep_tokens<1>(P, T, ds, ch, c0, t, nt, visited); break;
-    case 2: if (LMAX >= 2) return sweep_tokens<2>(P, T, ds, ch, c0, t, nt, visited); break;
-    case 3: if (LMAX >= 4) return sweep_tokens<3>(P, T, ds, ch, c0, t, nt, visited); break;
-    case 4: if (LMAX >= 4) return sweep_tokens<4>(P, T, ds, ch, c0, t, nt, visited); break;
-    case 5: if (LMAX >= 8) return sweep_tokens<5>(P, T, ds, ch, c0, t, nt, visited); break;
-    case 6: if (LMAX >= 8) return sweep_tokens<6>(P, T, ds, ch, c0, t, nt, visited); break;
-    case 7: if (LMAX >= 8) return sweep_tokens<7>(P, T, ds, ch, c0, t, nt, visited); break;
-    case 8: if (LMAX >= 8) return sweep_tokens<8>(P, T, ds, ch, c0, t, nt, visited); break;
+    case 1: if (LMAX >= 1) return sweep_tokens<1>(P, m, ds, ch, t, nt, visited); break;
+    case 2: if (LMAX >= 2) return sweep_tokens<2>(P, m, ds, ch, t, nt, visited); break;
+    case 3: if (LMAX >= 4) return sweep_tokens<3>(P, m, ds, ch, t, nt, visited); break;
+    case 4: if (LMAX >= 4) return sweep_tokens<4>(P, m, ds, ch, t, nt, visited); break;
+    case 5: if (LMAX >= 8) return sweep_tokens<5>(P, m, ds, ch, t, nt, visited); break;
+    case 6: if (LMAX >= 8) return sweep_tokens<6>(P, m, ds, ch, t, nt, visited); break;
+    case 7: if (LMAX >= 8) return sweep_tokens<7>(P, m, ds, ch, t, nt, visited); break;
+    case 8: if (LMAX >= 8) return sweep_tokens<8>(P, m, ds, ch, t, nt, visited); break;
     default: break;
   }
-  return sweep_tokens<0>(P, T, ds, ch, c0, t, nt, visited);
+  return sweep_tokens<0>(P, m, ds, ch, t, nt, visited);
 }
 
 // One pass over a document's tokens with Philox counters (token, gdoc, iter, tag).  visited gains
 // the sum over sampled tokens of the map's distinct topics (at most 65535 * 65535 per pass, so
 // the pass counts in 32 bits).
 template <int LMAX>
-__device__ __forceinline__ void sweep_pass(const SweepParams& P, u32* T, DocState& ds, u32 iter, u32 tag,
+__device__ __forceinline__ void sweep_pass(const SweepParams& P, const DocLds& m, DocState& ds, u32 iter, u32 tag,
                                            u64& visited) {
   const u32 lane = lane_id();
   const u32 K = P.K;
@@ -813,7 +776,8 @@ __device__ __forceinline__ void sweep_pass(const SweepParams& P, u32* T, DocStat
     const bool valid = i < ds.L;
     Chunk ch;
     ch.ww = valid ? P.w[ds.b + i] : 0;
-    ch.zz = valid ? (u32)P.z[ds.b + i] : 0;
+    const u32 zz = valid ? (u32)P.z[ds.b + i] : 0;
+    ch.zz2 = 2u * zz;
     const u32x4 r = philox_tagged(i, ds.gdoc, iter, tag, 0, P.k0, P.k1);
     // per-token values that do not depend on the map: computed for the whole chunk
     ch.sa = valid ? P.sigma_a[ch.ww] : 0.0f;
@@ -822,45 +786,47 @@ __device__ __forceinline__ void sweep_pass(const SweepParams& P, u32* T, DocStat
     al_l.prob = 0.0f;
     al_l.alias = 0;
     if (valid) al_l = P.alias[(u64)ch.ww * K + slot_l];
-    ch.dense = u24(r.z) < al_l.prob ? slot_l : al_l.alias;
+    ch.dense2 = 2u * (u24(r.z) < al_l.prob ? slot_l : al_l.alias);
     ch.ux = u24(r.x);
     ch.uw = u24(r.w);
-    ch.znew = ch.zz;
+    ch.znew = zz;
     const u32 nt = ds.L - c0 < 64 ? ds.L - c0 : 64;
-    for (u32 t = 0; t < nt;) t = sweep_tokens_any<LMAX>(P, T, ds, ch, c0, t, nt, pass_visited);
+    remove_first(m, ds, readlane_u32(ch.zz2, 0));
+    for (u32 t = 0; t < nt;) t = sweep_tokens_any<LMAX>(P, m, ds, ch, t, nt, pass_visited);
     if (valid) P.z[ds.b + i] = (u16)ch.znew;
   }
   visited += pass_visited;
 }
 
 // Fold-in carries the map from sweep to sweep.  Once the document's topics have thinned out, the
-// map is rebuilt at the capacity its keys need (the layout is a function of the key set and the
-// capacity), so the later sweeps scan fewer lines.
-__device__ __forceinline__ void shrink_doc_map(const SweepParams& P, u32* T, DocState& ds) {
+// key table is rebuilt at the capacity its keys need (the layout is a function of the key set and
+// the capacity), so the later sweeps scan fewer lines.
+__device__ __forceinline__ void shrink_doc_map(const SweepParams& P, const DocLds& m, DocState& ds) {
   const u32 want = map_capacity(ds.occupied, P.K);
   if (want < ds.cap) {
     ds.cap = want;
-      wave_sync();
-    const u32 distinct = uniform_u32(distinct_topics(T + ds.cap_max, P.K, P.z, ds.b, ds.L));
-    build_doc_map(T, ds.cap, T + ds.cap_max, P.K, distinct, P.z, ds.b, ds.L);
+    wave_sync();
+    keymap_build(m.Kt, ds.cap, ds.hm, m.C, P.K);
   }
 }
 
 // All sweeps of a fold-in (counters (token, request, s, FOLD_SWEEP), s = 1..sweeps).
-__device__ __forceinline__ void foldin_sweeps(const SweepParams& P, u32* T, DocState& ds, u32 sweeps, u64& visited) {
+__device__ __forceinline__ void foldin_sweeps(const SweepParams& P, u32* smem, DocState& ds, u32 sweeps,
+                                              u64& visited) {
+  const DocLds m = doc_lds(smem, P.K);
   for (u32 sweep = 1; sweep <= sweeps; ++sweep) {
-    if (sweep > 1) shrink_doc_map(P, T, ds);
-    sweep_pass<8>(P, T, ds, sweep, kTagFoldSweep, visited);
+    if (sweep > 1) shrink_doc_map(P, m, ds);
+    sweep_pass<8>(P, m, ds, sweep, kTagFoldSweep, visited);
   }
 }
 
 __device__ __forceinline__ u32 sweep_iteration(const SweepParams& P) { return P.iter_ptr ? *P.iter_ptr : P.iter; }
 
 template <int LMAX>
-__device__ __forceinline__ void sweep_doc(const SweepParams& P, u32 doc, u32* T) {
-  DocState ds = open_doc(P, doc, T);
+__device__ __forceinline__ void sweep_doc(const SweepParams& P, u32 doc, u32* smem) {
+  DocState ds = open_doc(P, doc, smem);
   u64 visited = 0;
-  sweep_pass<LMAX>(P, T, ds, sweep_iteration(P), kTagSweep, visited);
+  sweep_pass<LMAX>(P, doc_lds(smem, P.K), ds, sweep_iteration(P), kTagSweep, visited);
   if (lane_id() == 0 && P.visited) atomicAdd(P.visited, (unsigned long long)visited);
 }
 
@@ -920,9 +886,9 @@ __global__ __launch_bounds__(64) void k_sweep_persistent_long(SweepParams P, u32
 
 // ---------------------------------------------------------------------------------------------
 // Block-per-document sweep for long documents: four wavefronts share one map.  Wave 0 owns the
-// map (decrement, growth, increment) and the in-column search; wave g sums line group g (lines
-// j = g mod 4), which is exactly the grouping of the canonical order, so the result is
-// bit-identical to the wavefront-per-document sweep.
+// counts and the key table (removal, addition, growth) and the in-column search; wave g sums line
+// group g (lines j = g mod 4), which is exactly the grouping of the canonical order, so the result
+// is bit-identical to the wavefront-per-document sweep.
 // ---------------------------------------------------------------------------------------------
 constexpr u32 kBlockWaves = 4;
 
@@ -934,7 +900,8 @@ __global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u
   const u32 lane = lane_id();
   const u32 wave = threadIdx.x >> 6;
   const u32 K = P.K;
-  u32* T = sweep_smem;
+  const DocLds m = doc_lds(sweep_smem, K);
+  const u32 e2 = 2u * K;
   for (;;) {
     if (threadIdx.x == 0) sh_item = head ? atomicAdd(head, 1u) : (blockIdx.x < n_items ? blockIdx.x : n_items);
     __syncthreads();
@@ -943,11 +910,7 @@ __global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u
     if (item >= n_items) return;
     const u32 doc = P.doc_order ? P.doc_order[item] : item;
     DocState ds;
-    if (wave == 0) {
-      ds = open_doc(P, doc, T);
-      if (lane == 0) sh_cap = ds.cap;
-    }
-    __syncthreads();
+    if (wave == 0) ds = open_doc(P, doc, sweep_smem);
     const u64 b = P.doc_off[doc];
     const u32 L = (u32)(P.doc_off[doc + 1] - b);
     const u32 gdoc = P.doc_base + doc;
@@ -968,22 +931,16 @@ __global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u
         al_l.alias = 0;
         if (valid) al_l = P.alias[(u64)ww * K + slot_l];
         dense_l = u24(r.z) < al_l.prob ? slot_l : al_l.alias;
+        remove_first(m, ds, 2u * readlane_u32(zz, 0));
+        if (lane == 0) sh_cap = ds.cap;
       }
+      __syncthreads();
       u32 znew_lane = zz;
       const u32 nt = L - c0 < 64 ? L - c0 : 64;
       for (u32 t = 0; t < nt; ++t) {
         const u32 wt = readlane_u32(ww, t);
-        const u32 zt = readlane_u32(zz, t);
         const f32 sa = readlane_f32(sa_l, t);
-        if (wave == 0) {
-          ds.occupied -= uniform_u32(map_dec(T, ds.cap, ds.hm, zt));
-          if (lane == 0) sh_cap = ds.cap;
-        }
-        __syncthreads();
-        Pick pk;
-        pk.topic = zt;
-        pk.slot = kNoSlot;
-        pk.entry = 0;
+        u32 new2 = 2u * readlane_u32(zz, t);
         if (sa > 0.0f) {
           const u32 lines = sh_cap / kMapLine;
           const PhiRow prow = phi_row(P.phi + (u64)wt * K, K);
@@ -995,13 +952,13 @@ __global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u
 #pragma unroll
             for (u32 k = 0; k < 4; ++k) {
               const u32 jj = j + k * kBlockWaves;
-              e[k] = jj < lines ? T[jj * 64 + lane] : kMapEmpty;
+              e[k] = jj < lines ? (u32)m.Kt[jj * 64 + lane] : e2;
             }
-            f32 m[4];
+            f32 mm[4];
 #pragma unroll
-            for (u32 k = 0; k < 4; ++k) m[k] = slot_mass(e[k], prow);
+            for (u32 k = 0; k < 4; ++k) mm[k] = slot_mass(m.C, e[k], prow);
 #pragma unroll
-            for (u32 k = 0; k < 4; ++k) G = G + m[k];
+            for (u32 k = 0; k < 4; ++k) G = G + mm[k];
           }
           grp[wave][lane] = G;
           __syncthreads();
@@ -1014,17 +971,23 @@ __global__ __launch_bounds__(256) void k_sweep_block(SweepParams P, u32* head, u
               const f32 tgt = u24(readlane_u32(r.w, t)) * sb;
               f32 E;
               const u32 ls = pick_lane(A, Ps, tgt, E);
-              pk.slot = column_pick(T, lines, ls, prow, E, tgt) * 64 + ls;
-              pk.entry = T[pk.slot];
-              pk.topic = map_key(pk.entry);
+              new2 = m.Kt[column_pick(m.Kt, m.C, lines, ls, prow, E, tgt, e2) * 64 + ls];
             } else {
-              pk.topic = readlane_u32(dense_l, t);
+              new2 = 2u * readlane_u32(dense_l, t);
             }
           }
         }
         if (wave == 0) {
-          add_topic(P, T, ds, pk, c0, t, znew_lane, zz);
-          if (lane == t) znew_lane = pk.topic;
+          new2 = uniform_u32(new2);
+          if (lane == t) znew_lane = new2 >> 1;
+          const u32 has_next = t + 1 < nt ? 1u : 0u;
+          const u32 next2 = 2u * readlane_u32(zz, t + 1 < 64 ? t + 1 : 63);
+          const CountStep cs = count_step(m.C, new2, next2, has_next);
+          if (cs.entered | cs.left) {
+            wave_sync();
+            apply_key_changes(P, m, ds, cs, new2, next2);
+          }
+          if (lane == 0) sh_cap = ds.cap;
         }
         __syncthreads();
       }
@@ -1333,50 +1296,79 @@ __global__ void k_stage_scan(const f32* in, f32* out, u32 n_groups) {
   out[g * 64 + lane_id()] = wave_scan_f32(in[g * 64 + lane_id()]);
 }
 
-// ops[i] = key | (1u << 31) for a decrement; the final table is written to out.
+// ops[i] = key | (1u << 31) for a decrement (of a present key); the final map is written to out in
+// entry form (key << 16 | count, kMapEmpty for an empty slot).  The operations are those of the
+// sweep: a count change, and an insertion or a removal when the count leaves or reaches 0.
 __global__ void k_stage_map(const u32* ops, u32 n_ops, u32 cap, u32 K, u32* out) {
   extern __shared__ u32 map_smem[];
   const u32 lane = lane_id();
+  const DocLds m = doc_lds(map_smem, K);
   const u32 hm = map_home_mult(K);
-  for (u32 s = lane; s < cap; s += 64) map_smem[s] = kMapEmpty;
+  const u32 e2 = 2u * K;
+  for (u32 i = lane; i < (K + 2) / 2; i += 64) ((u32*)m.C)[i] = 0u;
+  for (u32 s = lane; s < cap; s += 64) m.Kt[s] = (u16)e2;
   for (u32 i = 0; i < n_ops; ++i) {
-    u32 op = ops[i];
-    if (op >> 31) map_dec(map_smem, cap, hm, op & 0x7FFFFFFFu);
-    else map_inc(map_smem, cap, hm, op);
+    const u32 op = ops[i];
+    const u32 key2 = 2u * (op & 0x7FFFFFFFu);
+    const bool dec = (op >> 31) != 0u;
+    wave_sync();
+    u32 old = count_at(m.C, key2);
+    if (dec && old == 0u) continue;  // not a present key
+    wave_sync();
+    if (lane == 0) count_put(m.C, key2, dec ? old - 1u : old + 1u);
+    wave_sync();
+    if (!dec && old == 0u) keymap_insert(m.Kt, cap, hm, e2, key2);
+    if (dec && old == 1u) keymap_erase(m.Kt, cap, hm, e2, key2);
   }
   wave_sync();
-  for (u32 s = lane; s < cap; s += 64) out[s] = map_smem[s];
+  for (u32 s = lane; s < cap; s += 64) {
+    const u32 v = m.Kt[s];
+    out[s] = v == e2 ? kMapEmpty : map_entry(v >> 1, count_at(m.C, v));
+  }
 }
 
-// One wavefront per draw: the map `table` (cap slots, any content) is copied to LDS and draw i is
-// made with the uniforms ux[i], uw[i] by the sweep's own device functions -- the register
-// variants when use_regs (maps of up to eight lines), else the line-loop form.  out[i] is the
-// picked slot, or kNoSlot for a dense draw.
+// One wavefront per draw: the map `table` (cap slots in entry form, distinct keys) is loaded into
+// the sweep's LDS state and draw i is made with the uniforms ux[i], uw[i] by the sweep's own device
+// functions -- the register variants when use_regs (maps of up to eight lines), else the line-loop
+// form.  out[i] is the picked slot, or kNoSlot for a dense draw.
 __global__ __launch_bounds__(64) void k_stage_pick(const u32* table, u32 cap, u32 K, const f32* phi_row_k, f32 sa,
                                                    const f32* ux, const f32* uw, u32 count, u32 use_regs, u32* out) {
   extern __shared__ u32 pick_smem[];
   const u32 i = blockIdx.x;
   if (i >= count) return;
-  for (u32 s = lane_id(); s < cap; s += 64) pick_smem[s] = table[s];
+  const u32 lane = lane_id();
+  const DocLds m = doc_lds(pick_smem, K);
+  const u32 e2 = 2u * K;
+  for (u32 q = lane; q < (K + 2) / 2; q += 64) ((u32*)m.C)[q] = 0u;
+  wave_sync();
+  for (u32 s = lane; s < cap; s += 64) {
+    const u32 e = table[s];
+    if (map_count(e) != 0u) {
+      m.Kt[s] = (u16)(2u * map_key(e));
+      count_put(m.C, 2u * map_key(e), map_count(e));
+    } else {
+      m.Kt[s] = (u16)e2;
+    }
+  }
   wave_sync();
   const u32 lines = uniform_u32(cap / kMapLine);
   const PhiRow prow = phi_row(phi_row_k, K);
-  Pick pk;
+  Draw d;
   if (use_regs && lines <= 8) {
     switch (lines) {
-      case 1: pk = draw_topic_regs<1>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
-      case 2: pk = draw_topic_regs<2>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
-      case 3: pk = draw_topic_regs<3>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
-      case 4: pk = draw_topic_regs<4>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
-      case 5: pk = draw_topic_regs<5>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
-      case 6: pk = draw_topic_regs<6>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
-      case 7: pk = draw_topic_regs<7>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
-      default: pk = draw_topic_regs<8>(pick_smem, prow, sa, ux[i], uw[i], kNoSlot); break;
+      case 1: d = draw_topic_regs_lds<1>(m, prow, sa, ux[i], uw[i]); break;
+      case 2: d = draw_topic_regs_lds<2>(m, prow, sa, ux[i], uw[i]); break;
+      case 3: d = draw_topic_regs_lds<3>(m, prow, sa, ux[i], uw[i]); break;
+      case 4: d = draw_topic_regs_lds<4>(m, prow, sa, ux[i], uw[i]); break;
+      case 5: d = draw_topic_regs_lds<5>(m, prow, sa, ux[i], uw[i]); break;
+      case 6: d = draw_topic_regs_lds<6>(m, prow, sa, ux[i], uw[i]); break;
+      case 7: d = draw_topic_regs_lds<7>(m, prow, sa, ux[i], uw[i]); break;
+      default: d = draw_topic_regs_lds<8>(m, prow, sa, ux[i], uw[i]); break;
     }
   } else {
-    pk = draw_topic_loop(pick_smem, lines, prow, sa, ux[i], uw[i], kNoSlot);
+    d = draw_topic_loop(m.Kt, m.C, lines, prow, sa, ux[i], uw[i], e2);
   }
-  if (lane_id() == 0) out[i] = pk.slot;
+  if (lane == 0) out[i] = d.key2 == kNoSlot ? kNoSlot : d.slot;
 }
 
 __global__ void k_stage_poisson(const u32* nval, u32 count, f32 beta, const AliasSlot* tables, u32 c1, u32 c2,

@@ -104,19 +104,21 @@ LDA_HD f64 u53(u32 a, u32 b) {
 LDA_HD u32 bounded(u32 r, u32 n) { return (u32)(((u64)r * (u64)n) >> 32); }
 
 // ---------------------------------------------------------------------------------------------
-// Doc-topic map (wave-cooperative Robin Hood table).  Entry = key << 16 | count; the table is
-// history-independent: Robin Hood linear probing with ties broken by key, and deletion by
+// Doc-topic map (wave-cooperative Robin Hood table).  On the device a document's map is a 16-bit
+// count per topic plus a table of the topics present (the sweep kernels' LDS state); as one table
+// of entries key << 16 | count it is what the conformance stages exchange with the tests.  The
+// table is history-independent: Robin Hood linear probing with ties broken by key, and deletion by
 // backward shift, so its layout is a function of the current key set only.
 // ---------------------------------------------------------------------------------------------
-// An empty slot holds count 0 (a stored entry has a count of at least 1), so its mass
-// count * phi[w][key] is +0 whatever its key field, with no test.  Every empty slot -- and every
-// read past a map's last line -- is kMapEmpty: count 0 at key 0xFFFF, beyond the K <= 65535 topics
-// of any row, so the sweep's row reads (buffer loads bounded by the row) skip it without a memory
-// access.  The content is the same for every empty slot, so a table stays a function of its keys.
+// In the entry form an empty slot holds count 0 at key 0xFFFF (kMapEmpty): a stored entry has a
+// count of at least 1, so an empty slot's mass count * phi[w][key] is +0 whatever its key field.
 constexpr u32 kMapEmpty = 0xFFFF0000u;
 constexpr u32 kMapLine = 64;          // one wavefront probes 64 consecutive slots
 constexpr u32 kMaxDocLen = 65535;     // counts are 16-bit
 constexpr u32 kMaxTopics = 65536;     // keys are 16-bit
+// The sweep's key table stores 2 * topic in 16-bit slots (the byte offset of the topic's count),
+// and 2 * K for an empty slot.
+constexpr u32 kMaxSweepTopics = 32767;
 
 // Completion flag of a served request: a fingerprint of its words (sum of w_i * (2i + 1), any
 // order), length and request id, never 0 (0 means not finished).
@@ -147,6 +149,11 @@ LDA_HD u32 map_capacity(u32 L, u32 K) {
   u32 c = need + need / 32 + 1;
   return ((c + kMapLine - 1) / kMapLine) * kMapLine;
 }
+
+// LDS of one document's sweep state: K + 1 counts (u16; entry K stays 0), padded to 8 bytes, then
+// the key table (u16 per slot) for the largest map of a document of up to maxL tokens.
+LDA_HD u32 doc_counts_bytes(u32 K) { return ((K + 1) * 2 + 7) / 8 * 8; }
+LDA_HD u64 doc_state_bytes(u32 maxL, u32 K) { return (u64)doc_counts_bytes(K) + (u64)map_capacity(maxL, K) * 2; }
 
 // ---------------------------------------------------------------------------------------------
 // Alias tables.  prob is the probability of keeping the slot (scaled convention, in [0, 1]).

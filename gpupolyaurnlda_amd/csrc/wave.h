@@ -1,8 +1,10 @@
 // wave.h -- wave64 building blocks for gfx950: canonical scan network, integer scans and the
-// wave-cooperative Robin Hood doc-topic map in LDS.
+// wave-cooperative doc-topic state in LDS.
 //
-// The doc-topic map replaces the half-warp HashMap of src/cu/hashmap.cuh:13-334 (insert2 /
-// try_insert2 / get2 at :213-333).  One wavefront probes a whole 64-slot line with one LDS read
+// The doc-topic state replaces the half-warp HashMap of src/cu/hashmap.cuh:13-334 (insert2 /
+// try_insert2 / get2 at :213-333).  It is split in two: a 16-bit count per topic, which a token
+// changes in O(1) without a search, and a Robin Hood key table of the topics present, which the
+// sparse sum walks 64 slots per step.  One wavefront probes a whole 64-slot line with one LDS read
 // and a ballot, insertion shifts the tail of the run right by one slot (Robin Hood order is
 // "sorted by home slot, then key"), and deletion shifts it back (the reference never deletes
 // zero-count keys, so its table only grows and needs device-malloc resizes).  Because the order
@@ -100,153 +102,190 @@ __device__ __forceinline__ u64 wave_scan_u64(u64 v) {
   return v;
 }
 
+// Inclusive i32 max scan over the same DPP network (max is associative, so any network gives the
+// plain prefix maxima): lanes outside a step's source keep their value (INT_MIN moved in).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_or_min(int v) {
+  return __builtin_amdgcn_update_dpp((int)0x80000000, v, CTRL, ROW_MASK, 0xF, false);
+}
+
+__device__ __forceinline__ int wave_scan_max_i32(int v) {
+  v = max(v, dpp_or_min<0x111, 0xF>(v));  // row_shr:1
+  v = max(v, dpp_or_min<0x112, 0xF>(v));  // row_shr:2
+  v = max(v, dpp_or_min<0x114, 0xF>(v));  // row_shr:4
+  v = max(v, dpp_or_min<0x118, 0xF>(v));  // row_shr:8
+  v = max(v, dpp_or_min<0x142, 0xA>(v));  // row_bcast:15 into rows 1, 3
+  v = max(v, dpp_or_min<0x143, 0xC>(v));  // row_bcast:31 into rows 2, 3
+  return v;
+}
+
+// Number of set lanes of `mask` below this lane.
+__device__ __forceinline__ u32 lanes_below(u64 mask) {
+  return __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
+}
+
 // ---------------------------------------------------------------------------------------------
-// Doc-topic map operations.  T points at `cap` LDS slots (cap a multiple of 64, never full); hm is
-// map_home_mult(K) for keys below K.  A slot is empty when its count is 0 (it then holds
-// kMapEmpty).  All 64 lanes of the wave call these with the same (uniform) arguments.
+// Doc-topic state of one document in LDS: a count per topic and a Robin Hood key table.
+//
+//   C  (u16 per topic, K + 1 entries): n_dk; entry K is always 0.
+//   Kt (u16 per slot, cap slots, cap a multiple of 64): 2 * topic for an occupied slot, 2 * K for an
+//      empty one.  A slot's value is the byte offset of its count in C, and half the byte offset of
+//      its phi value in the word's row; an empty slot thus reads the zero count C[K] and a phi
+//      offset past the row (a buffer read that returns 0 without touching memory), so its mass is
+//      0 * 0 = +0 with no test.
+//
+// The key table holds exactly the topics with a nonzero count, in the Robin Hood layout of linear
+// probing with ties broken by key (deletion by backward shift).  That layout is a function of the
+// key set and the capacity only.  With order-preserving homes (home non-decreasing in the key) it
+// is the ordered layout: keys ascending, key i at slot (i + max(w, max_{j<=i}(h_j - j))) mod cap,
+// h_j the home of key j and w the number of slots that the run passing the table's end occupies at
+// its start.  Counts change by one per token through C alone, so a sweep touches the key table
+// only when a topic enters or leaves the document.
+//
+// All 64 lanes call these functions with the same (uniform) arguments.
 // ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ u32 count_at(const u16* C, u32 key2) {
+  return *(const u16*)((const char*)C + key2);
+}
+__device__ __forceinline__ void count_put(u16* C, u32 key2, u32 v) { *(u16*)((char*)C + key2) = (u16)v; }
+
 // x mod cap for x < 2 * cap: x - cap wraps past 2^32 when x < cap, so the minimum is the slot.
 __device__ __forceinline__ u32 wrap_slot(u32 x, u32 cap) { return min(x, x - cap); }
 
-struct MapProbe {
-  u32 slot;       // slot of the key, or the slot where it belongs
-  u32 found;      // 1 if present
-  u32 entry;      // entry at `slot` when found
-  u32 lane;       // lane of `slot` in the line the probe stopped in
-  u32 line_slot;  // per lane: the slot this lane read in that line
-  u32 line_e;     // per lane: the entry it read there
-};
+__device__ __forceinline__ u32 key_home(u32 key2, u32 cap, u32 hm) { return map_home(key2 >> 1, cap, hm); }
 
-// Rewrites a line the wave has just read, with one lane's entry replaced: every lane stores its
-// own slot, so no lane needs to be masked off (no exec-mask change on the scalar unit).
-__device__ __forceinline__ void map_store_line(u32* T, u32 line_slot, u32 line_e, u32 at_lane, u32 entry) {
-  T[line_slot] = lane_id() == at_lane ? entry : line_e;
-}
-
-__device__ __forceinline__ MapProbe map_probe(const u32* T, u32 cap, u32 hm, u32 key) {
+// Zeroes the K + 1 counts and counts the topics z[b .. b + L); returns the number of distinct
+// topics (the lane whose atomic add finds a zero count is the first of its topic).  Counts are
+// 16-bit halves of 32-bit words; a document has at most 65535 tokens, so no half carries.
+__device__ __forceinline__ u32 doc_count_topics(u16* C, u32 K, const u16* __restrict__ z, u64 b, u32 L) {
   const u32 lane = lane_id();
-  const u32 h = map_home(key, cap, hm);
+  u32* Cw = (u32*)C;
+  const u32 words = (K + 2) / 2;
+  for (u32 i = lane; i < words; i += 64) Cw[i] = 0u;
   wave_sync();
-  MapProbe p;
-  p.slot = 0;
-  p.found = 0;
-  p.entry = kMapEmpty;
-  p.lane = 0;
-  p.line_slot = 0;
-  p.line_e = kMapEmpty;
-  for (u32 base = 0; base < cap; base += kMapLine) {
-    u32 d = base + lane;
-    const u32 s = wrap_slot(h + d, cap);
-    u32 e = T[s];
-    u32 ek = map_key(e);
-    bool empty = map_is_empty(e);
-    // (computed for empty slots too, where the stop mask below already holds)
-    u32 de = map_disp(s, map_home(ek, cap, hm), cap);
-    // match = !empty && ek == key; stop = empty || de < d || (de == d && ek > key), combined as
-    // lane masks of single comparisons
-    const u64 b_empty = ballot(empty);
-    u64 bm = ballot(ek == key) & ~b_empty;
-    u64 bs = b_empty | ballot(de < d) | (ballot(de == d) & ballot(ek > key));
-    u64 ev = bm | bs;
-    if (ev) {
-      u32 fl = first_lane(ev);
-      u32 sl = h + base + fl;
-      p.slot = sl >= cap ? sl - cap : sl;
-      p.found = (u32)((bm >> fl) & 1ull);
-      p.entry = readlane_u32(e, fl);
-      p.lane = fl;
-      p.line_slot = s;
-      p.line_e = e;
-      return p;
-    }
+  u32 fresh = 0;
+  for (u32 i = lane; i < L; i += 64) {
+    const u32 t = z[b + i];
+    const u32 sh = (t & 1u) << 4;
+    const u32 old = atomicAdd(&Cw[t >> 1], 1u << sh);
+    fresh += ((old >> sh) & 0xFFFFu) == 0u ? 1u : 0u;
   }
-  return p;
+  for (u32 off = 32; off > 0; off >>= 1) fresh += __shfl_xor(fresh, off, 64);
+  wave_sync();
+  return uniform_u32(fresh);
 }
 
-// Slot and entry of a key that is in the map.  Only the match is tested: a present key lies before
-// the first empty slot of its probe sequence, so no stop test is needed.
-__device__ __forceinline__ MapProbe map_find(const u32* T, u32 cap, u32 hm, u32 key) {
+// Key table at capacity cap of the topics with a nonzero count (fewer than cap), in the ordered
+// layout.  Two passes over the counts: the first finds their number n and max_j(h_j - j), hence
+// the last key's unwrapped slot n - 1 + max_j(h_j - j) and w; the second places every key.
+__device__ __forceinline__ void keymap_build(u16* Kt, u32 cap, u32 hm, const u16* C, u32 K) {
   const u32 lane = lane_id();
-  const u32 h = map_home(key, cap, hm);
-  wave_sync();
-  MapProbe p;
-  p.slot = 0;
-  p.found = 0;
-  p.entry = kMapEmpty;
-  p.lane = 0;
-  p.line_slot = 0;
-  p.line_e = kMapEmpty;
-  // the first line holds the key unless it was displaced past it (the common case is peeled)
+  const u32 e2 = 2u * K;
+  for (u32 s = lane; s < cap; s += 64) Kt[s] = (u16)e2;
+  int run = (int)0x80000000;
   u32 base = 0;
-  u32 e = T[wrap_slot(h + lane, cap)];
-  u64 bm = ballot(!map_is_empty(e)) & ballot(map_key(e) == key);
-  while (!bm) {
-    base += kMapLine;
-    if (base >= cap) return p;
-    e = T[wrap_slot(h + base + lane, cap)];
-    bm = ballot(!map_is_empty(e)) & ballot(map_key(e) == key);
+  for (u32 k0 = 0; k0 < K; k0 += 64) {
+    const u32 k = k0 + lane;
+    const bool present = k < K && C[k] != 0;
+    const u64 bal = ballot(present);
+    if (!bal) continue;
+    const u32 i = base + lanes_below(bal);
+    const int v = present ? (int)map_home(k, cap, hm) - (int)i : (int)0x80000000;
+    run = max(run, (int)readlane_u32((u32)wave_scan_max_i32(v), 63));
+    base += (u32)__popcll(bal);
   }
-  const u32 fl = first_lane(bm);
-  p.slot = wrap_slot(h + base + fl, cap);
-  p.found = 1;
-  p.entry = readlane_u32(e, fl);
-  p.lane = fl;
-  p.line_slot = wrap_slot(h + base + lane, cap);
-  p.line_e = e;
-  return p;
-}
-
-// Count of `key` +1 given its probe p; inserts it (Robin Hood shift) when absent.
-__device__ __forceinline__ u32 map_inc_at(u32* T, u32 cap, u32 key, MapProbe p) {
-  const u32 lane = lane_id();
-  if (p.found) {
-    if (lane == 0) T[p.slot] = p.entry + 1;
-    return 0;
-  }
-  u32 carry = map_entry(key, 1);
-  for (u32 base = 0; base < cap; base += kMapLine) {
+  if (base == 0) {
     wave_sync();
-    const u32 s = wrap_slot(p.slot + base + lane, cap);
-    u32 e = T[s];
-    u64 be = ballot(map_is_empty(e));
-    u32 fe = be ? first_lane(be) : 64;
-    u32 prev = __shfl_up(e, 1, 64);
-    if (lane == 0) prev = carry;
-    if (lane <= fe) T[s] = prev;
-    if (be) break;
-    carry = readlane_u32(e, 63);
+    return;
   }
-  return 1;
+  const int last = (int)base - 1 + run;
+  int carry = last + 1 - (int)cap > 0 ? last + 1 - (int)cap : 0;
+  wave_sync();
+  base = 0;
+  for (u32 k0 = 0; k0 < K; k0 += 64) {
+    const u32 k = k0 + lane;
+    const bool present = k < K && C[k] != 0;
+    const u64 bal = ballot(present);
+    if (!bal) continue;
+    const u32 i = base + lanes_below(bal);
+    const int v = present ? (int)map_home(k, cap, hm) - (int)i : (int)0x80000000;
+    const int sc = wave_scan_max_i32(v);
+    const int M = max(carry, sc);
+    const u32 slot = wrap_slot(i + (u32)M, cap);  // < cap while fewer than cap keys (the callers' rule)
+    if (present && slot < cap) Kt[slot] = (u16)(2u * k);
+    carry = max(carry, (int)readlane_u32((u32)sc, 63));
+    base += (u32)__popcll(bal);
+  }
+  wave_sync();
 }
 
-// Count of `key` +1; inserts it (Robin Hood shift) when absent.  Returns 1 if a key was added.
-__device__ __forceinline__ u32 map_inc(u32* T, u32 cap, u32 hm, u32 key) {
-  return map_inc_at(T, cap, key, map_probe(T, cap, hm, key));
-}
-
-// Count of `key` -1 (the key must be present); deletes it by backward shift at zero.
-// Returns 1 if the key was removed.
-__device__ __forceinline__ u32 map_dec(u32* T, u32 cap, u32 hm, u32 key) {
+// Slot of a key that is in the table (kNoKeySlot if it is not: never the case for the callers).
+constexpr u32 kNoKeySlot = 0xFFFFFFFFu;
+__device__ __forceinline__ u32 keymap_find(const u16* Kt, u32 cap, u32 hm, u32 key2) {
   const u32 lane = lane_id();
-  MapProbe p = map_find(T, cap, hm, key);
-  if (!p.found) return 0;  // not reached: every decremented topic is in the map
-  if (map_count(p.entry) > 1) {
-    map_store_line(T, p.line_slot, p.line_e, p.lane, p.entry - 1);
-    return 0;
+  const u32 h = key_home(key2, cap, hm);
+  wave_sync();
+  for (u32 base = 0; base < cap; base += kMapLine) {
+    const u64 bm = ballot(Kt[wrap_slot(h + base + lane, cap)] == key2);
+    if (bm) return wrap_slot(h + base + first_lane(bm), cap);
   }
-  u32 start = p.slot + 1;
-  if (start >= cap) start -= cap;
+  return kNoKeySlot;
+}
+
+// Removes a key that is in the table: the entries after it move back one slot, up to the first
+// empty slot or entry at its home.
+__device__ __forceinline__ void keymap_erase(u16* Kt, u32 cap, u32 hm, u32 e2, u32 key2) {
+  const u32 lane = lane_id();
+  const u32 at = keymap_find(Kt, cap, hm, key2);
+  if (at == kNoKeySlot) return;
+  const u32 start = wrap_slot(at + 1u, cap);
   for (u32 base = 0; base < cap; base += kMapLine) {
     wave_sync();
     const u32 s = wrap_slot(start + base + lane, cap);
-    u32 e = T[s];
-    u64 bs = ballot(map_is_empty(e)) | ballot(map_disp(s, map_home(map_key(e), cap, hm), cap) == 0);
-    u32 fs = bs ? first_lane(bs) : 64;
-    u32 dst = s == 0 ? cap - 1 : s - 1;
-    if (lane <= fs) T[dst] = lane < fs ? e : kMapEmpty;
+    const u32 v = Kt[s];
+    const u64 bs = ballot(v == e2) | ballot(key_home(v, cap, hm) == s);
+    const u32 fs = bs ? first_lane(bs) : 64u;
+    const u32 dst = s == 0 ? cap - 1u : s - 1u;
+    if (lane <= fs) Kt[dst] = (u16)(lane < fs ? v : e2);
     if (bs) break;
   }
-  return 1;
+  wave_sync();
+}
+
+// Adds a key that is not in the table (the table has a free slot): it goes to the first slot of
+// its probe sequence (distance d from its home) that is empty or holds an entry displaced less
+// than d, or displaced d with a larger key; the entries from there to the first empty slot move
+// up one slot.
+__device__ __forceinline__ void keymap_insert(u16* Kt, u32 cap, u32 hm, u32 e2, u32 key2) {
+  const u32 lane = lane_id();
+  const u32 h = key_home(key2, cap, hm);
+  wave_sync();
+  u32 at = h;
+  for (u32 base = 0; base < cap; base += kMapLine) {
+    const u32 d = base + lane;
+    const u32 s = wrap_slot(h + d, cap);
+    const u32 v = Kt[s];
+    const u32 de = map_disp(s, key_home(v, cap, hm), cap);  // (for an empty slot the stop holds anyway)
+    const u64 bs = ballot(v == e2) | ballot(de < d) | (ballot(de == d) & ballot(v > key2));
+    if (bs) {
+      at = wrap_slot(h + base + first_lane(bs), cap);
+      break;
+    }
+  }
+  u32 carry = key2;
+  for (u32 base = 0; base < cap; base += kMapLine) {
+    wave_sync();
+    const u32 s = wrap_slot(at + base + lane, cap);
+    const u32 v = Kt[s];
+    const u64 be = ballot(v == e2);
+    const u32 fe = be ? first_lane(be) : 64u;
+    u32 prev = (u32)__shfl_up((int)v, 1, 64);
+    if (lane == 0) prev = carry;
+    if (lane <= fe) Kt[s] = (u16)prev;
+    if (be) break;
+    carry = readlane_u32(v, 63);
+  }
+  wave_sync();
 }
 
 }  // namespace lda
