@@ -115,6 +115,16 @@ int launch_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* di
                       AliasSlot* out, hipStream_t st) {
   if (n_rows == 0) return 0;
   if (m > kMaxTopics) return fail("alias rows of %u entries (limit %u: slot indices are 16-bit in LDS)", m, kMaxTopics);
+  // rows of up to 4096 slots: one workgroup of four wavefronts per row
+  if (m <= kAliasBlockThreads * 16 && alias_block_lds(m) <= (u64)max_lds_per_block()) {
+    const u64 lds = alias_block_lds(m);
+    if (m <= kAliasBlockThreads * 4)
+      k_alias_block<4><<<n_rows, kAliasBlockThreads, lds, st>>>(rows, n_rows, m, divisor, sigma, alpha, out);
+    else
+      k_alias_block<16><<<n_rows, kAliasBlockThreads, lds, st>>>(rows, n_rows, m, divisor, sigma, alpha, out);
+    HCK(hipGetLastError());
+    return 0;
+  }
   u64 per = alias_lds_per_wave(m);
   u64 lim = (u64)max_lds_per_block();
   if (per > lim) return fail("row length %u needs %llu B of LDS (limit %llu)", m, (unsigned long long)per,
@@ -184,6 +194,40 @@ int build_poisson_tables(f32 beta, AliasSlot* tables, hipStream_t st) {
 // ---------------------------------------------------------------------------------------------
 // Corpus segments.
 // ---------------------------------------------------------------------------------------------
+// Output runs of one reorder pass as the host builds them: walking the output in order, tile t's
+// entries come up with ranks 0, 1, ...; a rank starts a run unless its place follows the previous
+// rank's place.
+struct HostRuns {
+  std::vector<u32> bits;                 // kCountTile / 32 words per tile
+  std::vector<std::vector<uint2>> per;   // per tile: {first rank, place} of each run
+  std::vector<u32> next;                 // per tile: the place that would continue its last run
+  explicit HostRuns(u64 n_tiles) : bits(n_tiles * (kCountTile / 32), 0u), per(n_tiles), next(n_tiles, 0u) {}
+  void add(u64 t, u32 r, u32 place) {
+    if (r == 0 || place != next[t]) {
+      bits[t * (kCountTile / 32) + r / 32] |= 1u << (r % 32);
+      per[t].push_back(make_uint2(r, place));
+    }
+    next[t] = place + 1;
+  }
+};
+
+int upload_runs(TileRuns* R, const std::vector<u16>& rank, const HostRuns& h) {
+  const u64 n = rank.size(), n_tiles = h.per.size();
+  std::vector<u32> off(n_tiles + 1, 0u);
+  for (u64 t = 0; t < n_tiles; ++t) off[t + 1] = off[t] + (u32)h.per[t].size();
+  std::vector<uint2> runs(off[n_tiles]);
+  for (u64 t = 0; t < n_tiles; ++t) std::copy(h.per[t].begin(), h.per[t].end(), runs.begin() + off[t]);
+  CK(dalloc(&R->rank, n));
+  CK(dalloc(&R->bits, h.bits.size()));
+  CK(dalloc(&R->run_off, n_tiles + 1));
+  CK(dalloc(&R->runs, runs.size()));
+  if (n) HCK(hipMemcpy(R->rank, rank.data(), n * sizeof(u16), hipMemcpyHostToDevice));
+  if (!h.bits.empty()) HCK(hipMemcpy(R->bits, h.bits.data(), h.bits.size() * sizeof(u32), hipMemcpyHostToDevice));
+  HCK(hipMemcpy(R->run_off, off.data(), off.size() * sizeof(u32), hipMemcpyHostToDevice));
+  if (!runs.empty()) HCK(hipMemcpy(R->runs, runs.data(), runs.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  return 0;
+}
+
 struct Segment {
   const u32* key_w = nullptr;
   const u32* key_d = nullptr;
@@ -198,13 +242,12 @@ struct Segment {
   u16* z = nullptr;
   u64* doc_off = nullptr;
   // n_wk reduction (launch_count), two tile reorders: z -> z1 (word groups) -> zs (word-sorted);
-  // per pass the rank of each input entry inside its kCountTile tile and, per tile rank, the
-  // entry's place in the output
-  u16* crank1 = nullptr;
-  u32* cdst1 = nullptr;
+  // per pass the rank of each input entry inside its kCountTile tile, and the tile's output runs
+  // (consecutive ranks going to consecutive places): a bitmap of the ranks that start a run
+  // (kCountTile / 32 words per tile) and per run its first rank and place in the output (runs
+  // of tile t at run_off[t] .. run_off[t + 1])
+  TileRuns runs1, runs2;
   u16* z1 = nullptr;
-  u16* crank2 = nullptr;
-  u32* cdst2 = nullptr;
   u16* zs = nullptr;
   CountTask* tasks = nullptr;
   u32* doc_order = nullptr;  // documents grouped by sweep bucket (longest first inside a bucket)
@@ -235,9 +278,9 @@ u64 g_part_min_tokens = 4ull << 20;  // smallest pipeline part (gpulda_set_part_
 int launch_count(const Segment* sg, u32* n, u32 K, hipStream_t st) {
   if (sg->n_tasks == 0) return 0;
   const u64 tiles = (sg->n_tokens + kCountTile - 1) / kCountTile;
-  k_tile_reorder<<<(u32)tiles, 1024, 0, st>>>(sg->z, sg->crank1, sg->cdst1, sg->z1, sg->n_tokens);
+  k_tile_reorder<<<(u32)tiles, 1024, 0, st>>>(sg->z, sg->runs1, sg->z1, sg->n_tokens);
   HCK(hipGetLastError());
-  k_tile_reorder<<<(u32)tiles, 1024, 0, st>>>(sg->z1, sg->crank2, sg->cdst2, sg->zs, sg->n_tokens);
+  k_tile_reorder<<<(u32)tiles, 1024, 0, st>>>(sg->z1, sg->runs2, sg->zs, sg->n_tokens);
   HCK(hipGetLastError());
   const u32 wpb = 4;
   u64 lds = (u64)wpb * K * sizeof(u32);
@@ -278,11 +321,13 @@ void free_segment(Segment* s) {
   dfree(s->w);
   dfree(s->z);
   dfree(s->doc_off);
-  dfree(s->crank1);
-  dfree(s->cdst1);
+  for (TileRuns* r : {&s->runs1, &s->runs2}) {
+    dfree(r->rank);
+    dfree(r->bits);
+    dfree(r->run_off);
+    dfree(r->runs);
+  }
   dfree(s->z1);
-  dfree(s->crank2);
-  dfree(s->cdst2);
   dfree(s->zs);
   dfree(s->tasks);
   dfree(s->doc_order);
@@ -376,7 +421,7 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
   // input tile come up in output order, so a per-tile counter gives their ranks.
   const u64 n_tiles = (n_tokens + kCountTile - 1) / kCountTile;
   std::vector<u16> crank1(n_tokens), crank2(n_tokens);
-  std::vector<u32> cdst1(n_tokens), cdst2(n_tokens);
+  HostRuns hr1(n_tiles), hr2(n_tiles);
   {
     std::vector<u32> gid(V);
     std::vector<u64> gstart(1, 0);
@@ -406,7 +451,7 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
       const u64 t = i / kCountTile;
       const u32 r = fill[t]++;
       crank1[i] = (u16)r;
-      cdst1[t * kCountTile + r] = (u32)q;
+      hr1.add(t, r, (u32)q);
     }
     std::vector<u32>().swap(inv1);
     std::vector<u32> perm(n_tokens);
@@ -420,7 +465,7 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
       const u64 t = q / kCountTile;
       const u32 r = fill[t]++;
       crank2[q] = (u16)r;
-      cdst2[t * kCountTile + r] = (u32)p;
+      hr2.add(t, r, (u32)p);
     }
   }
   // sweep buckets: by register-resident line count, largest first; longest documents first
@@ -482,19 +527,13 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
   CK(dalloc(&s->w, n_tokens));
   CK(dalloc(&s->z, n_tokens));
   CK(dalloc(&s->doc_off, (u64)n_docs + 1));
-  CK(dalloc(&s->crank1, n_tokens));
-  CK(dalloc(&s->cdst1, n_tokens));
+  CK(upload_runs(&s->runs1, crank1, hr1));
+  CK(upload_runs(&s->runs2, crank2, hr2));
   CK(dalloc(&s->z1, n_tokens));
-  CK(dalloc(&s->crank2, n_tokens));
-  CK(dalloc(&s->cdst2, n_tokens));
   CK(dalloc(&s->zs, n_tokens));
   CK(dalloc(&s->tasks, tasks.size()));
   if (n_tokens) {
     HCK(hipMemcpy(s->w, w, n_tokens * sizeof(u32), hipMemcpyHostToDevice));
-    HCK(hipMemcpy(s->crank1, crank1.data(), n_tokens * sizeof(u16), hipMemcpyHostToDevice));
-    HCK(hipMemcpy(s->cdst1, cdst1.data(), n_tokens * sizeof(u32), hipMemcpyHostToDevice));
-    HCK(hipMemcpy(s->crank2, crank2.data(), n_tokens * sizeof(u16), hipMemcpyHostToDevice));
-    HCK(hipMemcpy(s->cdst2, cdst2.data(), n_tokens * sizeof(u32), hipMemcpyHostToDevice));
   }
   HCK(hipMemcpy(s->doc_off, off.data(), off.size() * sizeof(u64), hipMemcpyHostToDevice));
   if (!tasks.empty()) HCK(hipMemcpy(s->tasks, tasks.data(), tasks.size() * sizeof(CountTask), hipMemcpyHostToDevice));

@@ -114,7 +114,14 @@ inline int sweep_bucket_lmax(u32 lines) {
     if (lines <= (u32)b) return b;
   return 0;
 }
-__global__ void k_tile_reorder(const u16* in, const u16* rank, const u32* dst, u16* out, u64 n);
+// Output runs of the kCountTile-entry tiles of one reorder pass (built once per corpus slice).
+struct TileRuns {
+  u16* rank = nullptr;     // per input entry: its rank inside its tile (output order)
+  u32* bits = nullptr;     // per tile, kCountTile / 32 words: bit r set when rank r starts a run
+  u32* run_off = nullptr;  // per tile (n_tiles + 1 entries): index of its first run
+  uint2* runs = nullptr;   // per run: {first rank, place in the output of that rank}
+};
+__global__ void k_tile_reorder(const u16* in, TileRuns R, u16* out, u64 n);
 __global__ void k_count(const CountTask* tasks, u32 n_tasks, const u16* zs, u32* n, u32 K, u32 waves_per_block);
 
 __global__ void k_stage_philox(const u32* ctr, const u32* key, u32* out, u32 n);
@@ -129,5 +136,12 @@ __global__ void k_stage_alias_draw(const AliasSlot* row, u32 m, const u32* r_slo
 
 // LDS bytes per alias row of length m (one wavefront).
 LDA_HD u64 alias_lds_per_wave(u32 m) { return (u64)((m + 3) & ~3u) * 14 + 68 * 4; }
+// Block-per-row alias kernel (kAliasBlockThreads threads per row, rows of up to 256 * PER slots):
+// dynamic LDS per row (u64 prefixes, padded f32 row values, u16 slot indices).
+constexpr u32 kAliasBlockThreads = 256;
+LDA_HD u64 alias_block_lds(u32 m) { return (u64)((m + 3) & ~3u) * 14 + (kAliasBlockThreads + 4) * 4; }
+template <int PER>
+__global__ void k_alias_block(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
+                              AliasSlot* out);
 
 }  // namespace lda

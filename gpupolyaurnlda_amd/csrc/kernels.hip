@@ -343,6 +343,210 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
 template __global__ void k_alias_rows<16>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u32);
 template __global__ void k_alias_rows<0>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u32);
 
+// Alias rows, one workgroup of kAliasBlockThreads (four wavefronts) per row: the same steps A-C
+// with thread t working on slots [t * per, (t + 1) * per), per = ceil(m / 256) <= PER.  The row
+// sum stays the canonical one (wavefront 0, lane l adding x_l, x_{64+l}, ... in order, then the
+// canonical scan), and the integer sums and prefixes are exact, so every slot gets the bits of the
+// one-wavefront kernel; each thread's share of the dependent steps (prefix placement, merge) is a
+// quarter as long.
+template <int PER>
+__global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32 n_rows, u32 m,
+                                                    const unsigned long long* __restrict__ divisor,
+                                                    f32* __restrict__ sigma, f32 alpha, AliasSlot* __restrict__ out) {
+  extern __shared__ unsigned char alias_smem[];
+  __shared__ unsigned long long red[3][kAliasBlockThreads / 64];
+  __shared__ f32 shT;
+  const u32 row = blockIdx.x;
+  if (row >= n_rows) return;
+  const u32 t = threadIdx.x;
+  const u32 lane = lane_id();
+  const u32 wave = t >> 6;
+  const u32 m4 = (m + 3) & ~3u;
+  u64* pre = reinterpret_cast<u64*>(alias_smem);
+  f32* xs = reinterpret_cast<f32*>(pre + m4);
+  u16* idx = reinterpret_cast<u16*>(xs + m4 + kAliasBlockThreads + 4);
+  f32* lp = xs;  // after B1 the row values are dead: lights' probabilities, indexed like pre
+  const u32 per = (m + kAliasBlockThreads - 1) / kAliasBlockThreads;  // <= PER
+  const u32 pad = (per & 1u) ^ 1u;  // thread stride per + pad odd: distinct banks
+  const u32 per_magic = per > 1 ? (u32)((0x100000000ull + per - 1) / per) : 0u;
+  auto xp = [&](u32 k) { return k + pad * __umulhi(k, per_magic); };
+  f32* x = rows + (u64)row * m;
+  AliasSlot* o = out + (u64)row * m;
+
+  // A: loads (coalesced), normalisation, the row staged in LDS; the canonical sum by wavefront 0
+  {
+    f32 xv[PER];
+#pragma unroll
+    for (int c = 0; c < PER; ++c) {
+      const u32 k = (u32)c * kAliasBlockThreads + t;
+      xv[c] = k < m ? x[k] : 0.0f;
+    }
+    if (divisor) {
+      u64 dr[PER];
+#pragma unroll
+      for (int c = 0; c < PER; ++c) {
+        const u32 k = (u32)c * kAliasBlockThreads + t;
+        dr[c] = k < m ? divisor[k] : 0ull;
+      }
+#pragma unroll
+      for (int c = 0; c < PER; ++c) {
+        const u32 k = (u32)c * kAliasBlockThreads + t;
+        if (k < m) {
+          const f32 d = (f32)(f64)dr[c];
+          xv[c] = d > 0.0f ? xv[c] / d : 0.0f;
+          x[k] = xv[c];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < PER; ++c) {
+      const u32 k = (u32)c * kAliasBlockThreads + t;
+      if (k < m) xs[xp(k)] = xv[c];
+    }
+  }
+  __syncthreads();
+  if (wave == 0) {
+    f32 acc = 0.0f;
+    for (u32 k = lane; k < m; k += 64) acc = acc + xs[xp(k)];
+    const f32 T0 = readlane_f32(wave_scan_f32(acc), 63);
+    if (lane == 0) {
+      shT = T0;
+      if (sigma) sigma[row] = alpha * T0;
+    }
+  }
+  __syncthreads();
+  const f32 T = shT;
+  if (!(T > 0.0f)) {
+    for (u32 k = t; k < m; k += kAliasBlockThreads) {
+      AliasSlot sl;
+      sl.prob = 1.0f;
+      sl.alias = k;
+      o[k] = sl;
+    }
+    return;
+  }
+  const f64 yT = 1.0 / (f64)T;
+
+  // B1: fixed-point weights and their exact total Q
+  const u32 k0 = t * per < m ? t * per : m;
+  const u32 k1 = k0 + per < m ? k0 + per : m;
+  const u32 own = k0 + pad * t;  // xp(k0) for k0 < m
+  u64 qr[PER];
+  u64 qsum = 0;
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    qr[s] = (u32)s < per && k0 + s < k1 ? alias_fixed_weight_r(xs[own + s], T, yT, m) : 0;
+    qsum += qr[s];
+  }
+  // B2: thread totals of surplus, deficit and heavies, with Q, placed by block scans
+  const u64 wq = wave_scan_u64(qsum);
+  if (lane == 63) red[0][wave] = wq;
+  __syncthreads();
+  u64 Q = 0;
+#pragma unroll
+  for (u32 w = 0; w < kAliasBlockThreads / 64; ++w) Q += red[0][w];
+  const f64 yQ = 1.0 / (f64)Q;
+  u64 sur = 0, def = 0;
+  u32 nh_t = 0;
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    if ((u32)s < per && k0 + s < k1) {
+      const u64 a = qr[s] * m;
+      if (a >= Q) {
+        sur += a - Q;
+        ++nh_t;
+      } else {
+        def += Q - a;
+      }
+    }
+  }
+  const u64 wS = wave_scan_u64(sur), wD = wave_scan_u64(def), wH = wave_scan_u64((u64)nh_t);
+  __syncthreads();  // everyone has read red[0]
+  if (lane == 63) {
+    red[0][wave] = wS;
+    red[1][wave] = wD;
+    red[2][wave] = wH;
+  }
+  __syncthreads();
+  u64 oS = 0, oD = 0, oH = 0, sD = 0, nh64 = 0;
+#pragma unroll
+  for (u32 w = 0; w < kAliasBlockThreads / 64; ++w) {
+    if (w < wave) {
+      oS += red[0][w];
+      oD += red[1][w];
+      oH += red[2][w];
+    }
+    sD += red[1][w];
+    nh64 += red[2][w];
+  }
+  const u32 nh = (u32)nh64;
+  const u32 nl = m - nh;
+  // B3: E (inclusive, heavies from the front) and D (exclusive, lights from the back); a light's
+  // probability a / Q replaces its row value
+  {
+    u64 E = oS + wS - sur, D = oD + wD - def;
+    u32 h = (u32)(oH + wH - nh_t), l = k0 - h;
+#pragma unroll
+    for (int s = 0; s < PER; ++s) {
+      if ((u32)s < per && k0 + s < k1) {
+        const u32 k = k0 + s;
+        const u64 a = qr[s] * m;
+        if (a >= Q) {
+          E += a - Q;
+          pre[h] = E;
+          idx[h] = (u16)k;
+          ++h;
+        } else {
+          pre[m - 1 - l] = D;
+          idx[m - 1 - l] = (u16)k;
+          lp[m - 1 - l] = (f32)div_by((f64)a, (f64)Q, yQ);
+          D += Q - a;
+          ++l;
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // C: merge of D' = D + 1 (lights) and E (heavies), ties to the lights; this thread's outputs
+  // are merge positions [k0, k1)
+  u32 lo = k0 > nh ? k0 - nh : 0, hi = k0 < nl ? k0 : nl;
+  while (lo < hi) {
+    const u32 mid = (lo + hi) >> 1;
+    if (pre[m - 1 - mid] + 1 <= pre[k0 - 1 - mid]) lo = mid + 1;
+    else hi = mid;
+  }
+  // the two heads of the merge stay in registers: Dh = D of light i (sD past the last light),
+  // Eh = E of heavy j (~0 past the last heavy, so a light is taken)
+  u32 i = lo, j = k0 - lo;
+  u64 Dh = i < nl ? pre[m - 1 - i] : sD;
+  u64 Eh = j < nh ? pre[j] : ~0ull;
+  for (u32 q = k0; q < k1; ++q) {
+    AliasSlot sl;
+    u32 k;
+    if (i < nl && Dh + 1 <= Eh) {
+      k = idx[m - 1 - i];
+      sl.prob = lp[m - 1 - i];
+      sl.alias = idx[j < nh ? j : nh - 1];
+      ++i;
+      Dh = i < nl ? pre[m - 1 - i] : sD;
+    } else {
+      k = idx[j];
+      sl.prob = 1.0f;
+      sl.alias = k;
+      if (Dh > Eh && j + 1 < nh) {  // Dh: D of the first light with D >= E_j, or the total deficit
+        sl.prob = (f32)div_by((f64)(Q + Eh - Dh), (f64)Q, yQ);
+        sl.alias = idx[j + 1];
+      }
+      ++j;
+      Eh = j < nh ? pre[j] : ~0ull;
+    }
+    o[k] = sl;
+  }
+}
+template __global__ void k_alias_block<4>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*);
+template __global__ void k_alias_block<16>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*);
+
 // ---------------------------------------------------------------------------------------------
 // z-sweep, one wavefront per document, the doc-topic state resident in LDS: a count per topic and
 // the Robin Hood key table of the topics present (wave.h).
@@ -1221,12 +1425,17 @@ __global__ __launch_bounds__(64) void k_serve_worker(SweepParams P, FoldParams F
 //   k_count         one wavefront per task (a run of at most kCountTask tokens of one word, a
 //                   contiguous range of zs): LDS histogram over K, nonzero bins added to n_wk.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_tile_reorder(const u16* __restrict__ in, const u16* __restrict__ rank,
-                                                       const u32* __restrict__ dst, u16* __restrict__ out, u64 n) {
+__global__ __launch_bounds__(1024) void k_tile_reorder(const u16* __restrict__ in, TileRuns R, u16* __restrict__ out,
+                                                       u64 n) {
+  constexpr u32 kWords = kCountTile / 32;
   __shared__ u16 tile[kCountTile];
+  __shared__ u32 wbits[kWords];   // run starts of this tile
+  __shared__ u32 wpref[kWords];   // runs starting before each word
+  __shared__ u32 wtot[1024 / 64];
   const u64 t0 = (u64)blockIdx.x * kCountTile;  // tiles start 64 KiB apart in `in` and `rank`
   const u32 m = (u32)(n - t0 < kCountTile ? n - t0 : kCountTile);
   const u32 m4 = m & ~3u;
+  const u16* rank = R.rank;
   // four entries per lane with 8-byte loads of values and ranks
   for (u32 l = 4 * threadIdx.x; l < m4; l += 4 * blockDim.x) {
     const uint2 v = *reinterpret_cast<const uint2*>(in + t0 + l);
@@ -1237,19 +1446,42 @@ __global__ __launch_bounds__(1024) void k_tile_reorder(const u16* __restrict__ i
     tile[r.y >> 16] = (u16)(v.y >> 16);
   }
   for (u32 l = m4 + threadIdx.x; l < m; l += blockDim.x) tile[rank[t0 + l]] = in[t0 + l];
+  // exclusive prefix counts of run starts per bitmap word (one word per thread: a block scan)
+  static_assert(kWords == 1024, "one bitmap word per thread");
+  const u32 wb = R.bits[(u64)blockIdx.x * kWords + threadIdx.x];
+  wbits[threadIdx.x] = wb;
+  u32 c = (u32)__popc(wb);
+  const u32 lane = lane_id();
+  const u32 wave = threadIdx.x >> 6;
+  for (u32 off = 1; off < 64; off <<= 1) {
+    const u32 y = (u32)__shfl_up((int)c, off, 64);
+    if (lane >= off) c += y;
+  }
+  if (lane == 63) wtot[wave] = c;
+  __syncthreads();
+  u32 before = 0;
+  for (u32 q = 0; q < wave; ++q) before += wtot[q];
+  wpref[threadIdx.x] = before + c - (u32)__popc(wb);
   __syncthreads();
   // consecutive lanes write consecutive ranks (a run's entries go to consecutive places); four
-  // independent destinations per lane in flight
+  // independent places per lane in flight
+  const uint2* runs = R.runs + R.run_off[blockIdx.x];
+  auto place = [&](u32 j) -> u32 {
+    const u32 wj = j >> 5;
+    const u32 below = wbits[wj] & (0xFFFFFFFFu >> (31u - (j & 31u)));  // starts at ranks <= j in the word
+    const uint2 run = runs[wpref[wj] + (u32)__popc(below) - 1u];          // rank 0 starts a run
+    return run.y + (j - run.x);
+  };
   const u32 B = blockDim.x;
   u32 j = threadIdx.x;
   for (; j + 3 * B < m; j += 4 * B) {
-    const u32 d0 = dst[t0 + j], d1 = dst[t0 + j + B], d2 = dst[t0 + j + 2 * B], d3 = dst[t0 + j + 3 * B];
+    const u32 d0 = place(j), d1 = place(j + B), d2 = place(j + 2 * B), d3 = place(j + 3 * B);
     out[d0] = tile[j];
     out[d1] = tile[j + B];
     out[d2] = tile[j + 2 * B];
     out[d3] = tile[j + 3 * B];
   }
-  for (; j < m; j += B) out[dst[t0 + j]] = tile[j];
+  for (; j < m; j += B) out[place(j)] = tile[j];
 }
 
 __global__ void k_count(const CountTask* __restrict__ tasks, u32 n_tasks, const u16* __restrict__ zs,
