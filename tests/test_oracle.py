@@ -227,6 +227,33 @@ def _check_map(T, cap, K, counts: dict, orc):
     assert seen == {k: c for k, c in counts.items() if c > 0}
 
 
+def test_map_closed_form_layout(orc):
+    """The ordered layout the sweep kernels build in two scans over the per-topic counts: keys
+    ascending, key i at slot (i + max(w, max_{j<=i}(h_j - j))) mod cap, h the home slot and w the
+    number of slots the run passing the table's end occupies at its start.  It must equal the
+    oracle's table built by sequential Robin Hood inserts, for key sets of every load (caps that a
+    regrow or a shrink uses included)."""
+    rng = np.random.default_rng(11)
+    for trial in range(1500):
+        K = int(rng.choice([7, 20, 64, 300, 1024, 4096]))
+        n = int(rng.integers(1, K + 1))
+        cap = orc.map_capacity(n, K)
+        if rng.random() < 0.3:
+            cap = orc.map_capacity(int(rng.integers(n, K + 1)), K)
+        keys = np.sort(rng.choice(K, n, replace=False)).astype(np.int64)
+        hm = orc.map_home_mult(K)
+        h = ((keys * hm) * cap) >> 32
+        i = np.arange(n)
+        M = np.maximum.accumulate(h - i)
+        w = max(0, int(n - 1 + M[-1]) + 1 - cap)
+        pos = (i + np.maximum(w, M)) % cap
+        assert len(set(pos.tolist())) == n
+        want = np.full(cap, 0xFFFF0000, np.uint32)
+        want[pos] = (keys.astype(np.uint32) << 16) | 1
+        got = orc.map_apply(rng.permutation(keys).astype(np.uint32), cap, K)
+        assert np.array_equal(got, want), (K, n, cap)
+
+
 @pytest.mark.parametrize("K,L", [(20, 50), (1024, 600), (300, 64), (4096, 5000)])
 def test_map_random_ops(orc, K, L):
     rng = np.random.default_rng(K + L)
