@@ -26,6 +26,7 @@
 
 #include "../../include/gpulda_abi.h"
 #include "../../include/gpulda_ext.h"
+#include "tile_runs.h"
 #include "kernels.h"
 #include "serve_book.h"
 
@@ -194,37 +195,20 @@ int build_poisson_tables(f32 beta, AliasSlot* tables, hipStream_t st) {
 // ---------------------------------------------------------------------------------------------
 // Corpus segments.
 // ---------------------------------------------------------------------------------------------
-// Output runs of one reorder pass as the host builds them: walking the output in order, tile t's
-// entries come up with ranks 0, 1, ...; a rank starts a run unless its place follows the previous
-// rank's place.
-struct HostRuns {
-  std::vector<u32> bits;                 // kCountTile / 32 words per tile
-  std::vector<std::vector<uint2>> per;   // per tile: {first rank, place} of each run
-  std::vector<u32> next;                 // per tile: the place that would continue its last run
-  explicit HostRuns(u64 n_tiles) : bits(n_tiles * (kCountTile / 32), 0u), per(n_tiles), next(n_tiles, 0u) {}
-  void add(u64 t, u32 r, u32 place) {
-    if (r == 0 || place != next[t]) {
-      bits[t * (kCountTile / 32) + r / 32] |= 1u << (r % 32);
-      per[t].push_back(make_uint2(r, place));
-    }
-    next[t] = place + 1;
-  }
-};
-
 int upload_runs(TileRuns* R, const std::vector<u16>& rank, const HostRuns& h) {
-  const u64 n = rank.size(), n_tiles = h.per.size();
-  std::vector<u32> off(n_tiles + 1, 0u);
-  for (u64 t = 0; t < n_tiles; ++t) off[t + 1] = off[t] + (u32)h.per[t].size();
-  std::vector<uint2> runs(off[n_tiles]);
-  for (u64 t = 0; t < n_tiles; ++t) std::copy(h.per[t].begin(), h.per[t].end(), runs.begin() + off[t]);
+  static_assert(sizeof(RunPair) == sizeof(uint2) && kTileEntries == kCountTile, "run table layout");
+  const u64 n = rank.size();
+  std::vector<u32> off;
+  std::vector<RunPair> runs;
+  h.flatten(off, runs);
   CK(dalloc(&R->rank, n));
   CK(dalloc(&R->bits, h.bits.size()));
-  CK(dalloc(&R->run_off, n_tiles + 1));
+  CK(dalloc(&R->run_off, off.size()));
   CK(dalloc(&R->runs, runs.size()));
   if (n) HCK(hipMemcpy(R->rank, rank.data(), n * sizeof(u16), hipMemcpyHostToDevice));
   if (!h.bits.empty()) HCK(hipMemcpy(R->bits, h.bits.data(), h.bits.size() * sizeof(u32), hipMemcpyHostToDevice));
   HCK(hipMemcpy(R->run_off, off.data(), off.size() * sizeof(u32), hipMemcpyHostToDevice));
-  if (!runs.empty()) HCK(hipMemcpy(R->runs, runs.data(), runs.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  if (!runs.empty()) HCK(hipMemcpy(R->runs, runs.data(), runs.size() * sizeof(RunPair), hipMemcpyHostToDevice));
   return 0;
 }
 
