@@ -138,8 +138,9 @@ int gpulda_serve_close(void);
 /* ---- stage entry points (host arrays in and out; the library stages device copies) ------- */
 int gpulda_stage_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n);
 int gpulda_stage_scan64(const float* in, float* out, uint32_t n_groups);
-/* ops[i] = topic (increment) or topic | 1u<<31 (decrement), topics below K (the home slots
- * depend on K); returns the final table. */
+/* ops[i] = topic (increment) or topic | 1u<<31 (decrement of a present topic), topics below
+ * K <= 32767 (the home slots depend on K); the table keeps a free slot (at most cap - 1 topics at
+ * any time, checked).  Returns the final table as entries key << 16 | count (0xFFFF0000 empty). */
 int gpulda_stage_map(const uint32_t* ops, uint32_t n_ops, uint32_t cap, uint32_t K, uint32_t* table_out);
 /* Row normalisation (by divisor[k] when divisor != NULL), row sums and alias tables. */
 int gpulda_stage_alias(const float* rows, uint32_t n_rows, uint32_t m, const uint64_t* divisor, float alpha,
@@ -149,7 +150,8 @@ int gpulda_stage_alias(const float* rows, uint32_t n_rows, uint32_t m, const uin
 int gpulda_stage_alias_draw(const float* prob, const uint32_t* alias, uint32_t m, const uint32_t* r_slot,
                             const uint32_t* r_keep, uint32_t count, uint32_t* out);
 /* Sparse/dense decision and sparse pick of the z-sweep on a given doc-topic map (cap slots of
- * entries key << 16 | count) for a word whose phi row is phi_row (K values) and sigma_a > 0:
+ * entries key << 16 | count, count 0 for an empty slot, occupied slots naming distinct topics
+ * below K <= 32767, checked) for a word whose phi row is phi_row (K values) and sigma_a > 0:
  * draw i is sparse when u_decide[i] * (sigma_a + sigma_b) > sigma_a and then picks the slot whose
  * interval holds u_target[i] * sigma_b (compute_product_cumsum + draw_wary_search,
  * src/cu/topics.cuh:163-192, 88-137).  slot_out[i] = that slot, or 0xFFFFFFFF for a dense draw.
