@@ -81,8 +81,10 @@ def test_map_layout(S, orc, K, L):
     assert np.array_equal(S.map_apply(ops, cap, K), orc.map_apply(ops, cap, K))
 
 
-@pytest.mark.parametrize("m", [1, 5, 20, 64, 200, 1024, 3000])
+@pytest.mark.parametrize("m", [1, 5, 20, 64, 200, 256, 257, 1024, 1025, 3000, 4096, 4097, 6000])
 def test_alias_rows(S, orc, m):
+    """Rows of up to 4,096 slots take the four-wavefront kernel (one to sixteen slots per thread),
+    longer rows the one-wavefront kernel; both against the oracle's split, bit for bit."""
     rng = np.random.default_rng(m)
     rows = np.stack([rng.random(m), rng.random(m) * (rng.random(m) < 0.1), np.ones(m), np.zeros(m),
                      np.exp(rng.normal(0, 6, m))]).astype(np.float32)
