@@ -401,10 +401,10 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
   // Ranks and destinations of the two tile reorders of the n_wk reduction.  Word groups are
   // contiguous word ranges of about n_tokens / kCountGroups tokens (a word with more is a group of
   // its own).  z1 order: groups ascending, corpus order inside a group; word-sorted order (zs):
-  // words ascending, corpus order inside a word.  Walking an output order, the entries of each
-  // input tile come up in output order, so a per-tile counter gives their ranks.
+  // words ascending, corpus order inside a word.  Built tile by tile on host threads
+  // (build_reorder_runs, tile_runs.h).
   const u64 n_tiles = (n_tokens + kCountTile - 1) / kCountTile;
-  std::vector<u16> crank1(n_tokens), crank2(n_tokens);
+  std::vector<u16> crank1, crank2;
   HostRuns hr1(n_tiles), hr2(n_tiles);
   {
     std::vector<u32> gid(V);
@@ -420,37 +420,8 @@ int build_segment(Segment* s, const u32* w, const u32* d, u32 n_docs, u64 n_toke
       gid[v] = (u32)(gstart.size() - 1);
       acc += cnt;
     }
-    std::vector<u32> p1(n_tokens), inv1(n_tokens);
-    {
-      std::vector<u64> cur(gstart.begin(), gstart.end());
-      for (u64 i = 0; i < n_tokens; ++i) {
-        const u32 q = (u32)cur[gid[w[i]]]++;
-        p1[i] = q;
-        inv1[q] = (u32)i;
-      }
-    }
-    std::vector<u32> fill(n_tiles, 0);
-    for (u64 q = 0; q < n_tokens; ++q) {  // level 1: z (document order) -> z1
-      const u32 i = inv1[q];
-      const u64 t = i / kCountTile;
-      const u32 r = fill[t]++;
-      crank1[i] = (u16)r;
-      hr1.add(t, r, (u32)q);
-    }
-    std::vector<u32>().swap(inv1);
-    std::vector<u32> perm(n_tokens);
-    {
-      std::vector<u64> cur(start.begin(), start.end() - 1);
-      for (u64 i = 0; i < n_tokens; ++i) perm[cur[w[i]]++] = (u32)i;
-    }
-    std::fill(fill.begin(), fill.end(), 0u);
-    for (u64 p = 0; p < n_tokens; ++p) {  // level 2: z1 -> zs (word-sorted)
-      const u32 q = p1[perm[p]];
-      const u64 t = q / kCountTile;
-      const u32 r = fill[t]++;
-      crank2[q] = (u16)r;
-      hr2.add(t, r, (u32)p);
-    }
+    const unsigned threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    build_reorder_runs(w, n_tokens, V, gid.data(), gstart, start, crank1, hr1, crank2, hr2, threads);
   }
   // sweep buckets: by register-resident line count, largest first; longest documents first
   std::vector<u32> order(n_docs);
