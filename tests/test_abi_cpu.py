@@ -154,3 +154,37 @@ def test_slice_hash_sees_every_word():
     t0 = time.perf_counter()
     h(big, np.array([len(big)], np.uint32))
     print(f"slice hash: {len(big) * 4 / (time.perf_counter() - t0) / 1e9:.1f} GB/s")
+
+
+def test_stage_map_and_pick_reject_tables_the_device_state_cannot_hold():
+    """The doc-topic state on the device is one count per topic plus a table of the topics present
+    (at most cap - 1 of them, 2 * topic per 16-bit slot): the stage entry points refuse inputs outside
+    that before any device work, with a message."""
+    from gpupolyaurnlda_amd._lib import lib
+    L = lib()
+
+    def err():
+        return L.gpulda_last_error().decode()
+
+    def stage_map(ops, cap, K):
+        ops = np.ascontiguousarray(ops, dtype=np.uint32)
+        out = np.zeros(cap, np.uint32)
+        return L.gpulda_stage_map(ops.ctypes.data_as(C.POINTER(C.c_uint32)), len(ops), cap, K,
+                                  out.ctypes.data_as(C.POINTER(C.c_uint32)))
+
+    assert stage_map(np.arange(64), 64, 1024) != 0 and "no free slot" in err()
+    assert stage_map(np.zeros(65536), 64, 1024) != 0 and "past 65535" in err()
+    assert stage_map([1, 2], 64, 32768) != 0 and "out of range" in err()
+    assert stage_map([1, 5000], 64, 1024) != 0 and ">= K" in err()
+
+    table = np.zeros(64, np.uint32)
+    table[3] = (7 << 16) | 2
+    table[9] = (7 << 16) | 1
+    row = np.ones(1024, np.float32)
+    u = np.full(1, 0.5, np.float32)
+    out = np.zeros(1, np.uint32)
+    rc = L.gpulda_stage_pick(table.ctypes.data_as(C.POINTER(C.c_uint32)), 64, 1024,
+                             row.ctypes.data_as(C.POINTER(C.c_float)), C.c_float(1.0),
+                             u.ctypes.data_as(C.POINTER(C.c_float)), u.ctypes.data_as(C.POINTER(C.c_float)), 1, 1,
+                             out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    assert rc != 0 and "repeats topic 7" in err()
