@@ -894,7 +894,8 @@ __device__ __forceinline__ void remove_first(const DocLds& m, DocState& ds, u32 
 struct Chunk {
   u32 ww;      // word
   u32 zz2;     // 2 * topic before the pass
-  u32 dense2;  // 2 * alias-table (dense) draw
+  u32 slot;    // alias slot of a dense draw (from r.y)
+  f32 uz;      // uniform of the dense draw's keep test (r.z)
   f32 sa;      // sigma_a of the word
   f32 ux;      // uniform of the sparse/dense choice
   f32 uw;      // uniform of the sparse target
@@ -928,7 +929,13 @@ __device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, const DocLds& 
       Draw d;
       if constexpr (NL > 0) d = draw_topic_regs<NL>(e, m.C, prow, sa, ux, uw);
       else d = draw_topic_loop(m.Kt, m.C, uniform_u32(ds.cap / kMapLine), prow, sa, ux, uw, ds.e2);
-      new2 = d.key2 != kNoSlot ? d.key2 : readlane_u32(ch.dense2, t);
+      if (d.key2 != kNoSlot) {
+        new2 = d.key2;
+      } else {  // dense: the word's alias slot is read only now (about one token in eight)
+        const u32 sl = readlane_u32(ch.slot, t);
+        const AliasSlot a = P.alias[(u64)wt * K + sl];
+        new2 = 2u * (readlane_f32(ch.uz, t) < a.prob ? sl : a.alias);
+      }
     }
     if (lane == t) ch.znew = new2 >> 1;
     const u32 has_next = t + 1 < nt ? 1u : 0u;
@@ -985,12 +992,8 @@ __device__ __forceinline__ void sweep_pass(const SweepParams& P, const DocLds& m
     const u32x4 r = philox_tagged(i, ds.gdoc, iter, tag, 0, P.k0, P.k1);
     // per-token values that do not depend on the map: computed for the whole chunk
     ch.sa = valid ? P.sigma_a[ch.ww] : 0.0f;
-    const u32 slot_l = bounded(r.y, K);
-    AliasSlot al_l;
-    al_l.prob = 0.0f;
-    al_l.alias = 0;
-    if (valid) al_l = P.alias[(u64)ch.ww * K + slot_l];
-    ch.dense2 = 2u * (u24(r.z) < al_l.prob ? slot_l : al_l.alias);
+    ch.slot = bounded(r.y, K);
+    ch.uz = u24(r.z);
     ch.ux = u24(r.x);
     ch.uw = u24(r.w);
     ch.znew = zz;
