@@ -266,9 +266,14 @@ int launch_count(const Segment* sg, u32* n, u32 K, hipStream_t st) {
   HCK(hipGetLastError());
   k_tile_reorder<<<(u32)tiles, 1024, 0, st>>>(sg->z1, sg->runs2, sg->zs, sg->n_tokens);
   HCK(hipGetLastError());
-  const u32 wpb = 4;
-  u64 lds = (u64)wpb * K * sizeof(u32);
-  k_count<<<(sg->n_tasks + wpb - 1) / wpb, 64 * wpb, lds, st>>>(sg->tasks, sg->n_tasks, sg->zs, n, K, wpb);
+  // per wavefront K bins, rounded to 4; up to four wavefronts a block as far as the LDS allows
+  const u64 hist = (u64)((K + 3u) & ~3u) * sizeof(u32);
+  const u32 wpb = (u32)std::min<u64>(4, (u64)max_lds_per_block() / hist);
+  if (wpb == 0) {
+    k_count_direct<<<(sg->n_tasks + 3) / 4, 256, 0, st>>>(sg->tasks, sg->n_tasks, sg->zs, n, K);
+  } else {
+    k_count<<<(sg->n_tasks + wpb - 1) / wpb, 64 * wpb, wpb * hist, st>>>(sg->tasks, sg->n_tasks, sg->zs, n, K, wpb);
+  }
   HCK(hipGetLastError());
   return 0;
 }

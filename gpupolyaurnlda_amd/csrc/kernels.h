@@ -123,6 +123,7 @@ struct TileRuns {
 };
 __global__ void k_tile_reorder(const u16* in, TileRuns R, u16* out, u64 n);
 __global__ void k_count(const CountTask* tasks, u32 n_tasks, const u16* zs, u32* n, u32 K, u32 waves_per_block);
+__global__ void k_count_direct(const CountTask* tasks, u32 n_tasks, const u16* zs, u32* n, u32 K);
 
 __global__ void k_stage_philox(const u32* ctr, const u32* key, u32* out, u32 n);
 __global__ void k_stage_scan(const f32* in, f32* out, u32 n_groups);

@@ -1494,17 +1494,32 @@ __global__ void k_count(const CountTask* __restrict__ tasks, u32 n_tasks, const 
   const u32 wave = threadIdx.x >> 6;
   const u32 task = blockIdx.x * waves_per_block + wave;
   if (task >= n_tasks) return;
-  u32* H = count_smem + (u64)wave * K;
-  for (u32 k = lane; k < K; k += 64) H[k] = 0;
+  // one histogram of K bins per wavefront, K rounded up to a multiple of 4 so that every
+  // histogram starts 16-byte aligned; bins are cleared four at a time
+  const u32 Kp = (K + 3u) & ~3u;
+  u32* H = count_smem + (u64)wave * Kp;
+  for (u32 k = 4 * lane; k < Kp; k += 256) *reinterpret_cast<uint4*>(&H[k]) = make_uint4(0u, 0u, 0u, 0u);
   wave_sync();
   const CountTask tk = tasks[task];
   for (u32 p = lane; p < tk.len; p += 64) atomicAdd(&H[zs[tk.p0 + p]], 1u);
   wave_sync();
+  // bins read one per lane, consecutive, so each atomic instruction covers two cache lines
   u32* row = n + (u64)tk.word * K;
   for (u32 k = lane; k < K; k += 64) {
-    u32 c = H[k];
+    const u32 c = H[k];
     if (c) atomicAdd(&row[k], c);
   }
+}
+
+// Topic counts too many for a per-wavefront LDS histogram (K above 40,960): each token adds its
+// one to n_wk directly.
+__global__ void k_count_direct(const CountTask* __restrict__ tasks, u32 n_tasks, const u16* __restrict__ zs,
+                               u32* __restrict__ n, u32 K) {
+  const u32 task = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (task >= n_tasks) return;
+  const CountTask tk = tasks[task];
+  u32* row = n + (u64)tk.word * K;
+  for (u32 p = lane_id(); p < tk.len; p += 64) atomicAdd(&row[zs[tk.p0 + p]], 1u);
 }
 
 // ---------------------------------------------------------------------------------------------

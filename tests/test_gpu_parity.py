@@ -192,9 +192,12 @@ def test_sweep_exact_conditional_gpu(S, orc):
     assert stats.chisquare(obs[keep], ex[keep]).pvalue > 1e-4
 
 
-def test_count(S, orc):
+@pytest.mark.parametrize("V,K", [(2000, 300), (500, 1023), (40, 20000), (40, 50000)])
+def test_count(S, orc, V, K):
+    """n_wk through the word-order reduction. K = 1023 leaves a partial group of four bins; K =
+    20000 fits two histograms a block in LDS; K = 50000 fits none and adds counts directly."""
     rng = np.random.default_rng(8)
-    V, K, N = 2000, 300, 300000
+    N = 300000
     w = (rng.zipf(1.3, N) % V).astype(np.uint32)
     z = rng.integers(0, K, N).astype(np.uint32)
     assert np.array_equal(S.count(w, z, V, K), orc.count(w, z, V, K))
