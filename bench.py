@@ -47,6 +47,8 @@ def main():
                     help="wavefronts per CU of the persistent schedules when --schedule is not auto")
     ap.add_argument("--split-len", type=int, default=256,
                     help="--schedule split: documents longer than this get the large-LDS launch")
+    ap.add_argument("--no-count-rows", action="store_true",
+                    help="sweep reads the f32 phi rows only (not the u16 count rows where they fit)")
     ap.add_argument("--no-serve", action="store_true", help="skip the fold-in serving measurement")
     ap.add_argument("--serve-requests", type=int, default=4000)
     ap.add_argument("--serve-serial", type=int, default=100, help="requests in the serial baseline sample")
@@ -79,6 +81,8 @@ def main():
     t0 = time.time()
     rc = ResidentCorpus(c["w"], c["d_len"], spec.K, spec.V, spec.alpha, spec.beta, seed=spec.seed, z=c["z"])
     log(f"resident in HBM in {time.time() - t0:.1f}s")
+    if args.no_count_rows:
+        rc.count_rows(False)
 
     if args.warmup:
         rc.iterate(args.warmup)
@@ -101,6 +105,10 @@ def main():
     elapsed = time.perf_counter() - t0
     visited = rc.visited()
     log(f"timed {args.steps} iterations in {elapsed:.3f}s; stage ms {stage_ms.tolist()}")
+    count_rows = {"on": not args.no_count_rows}
+    if not args.no_count_rows:  # the last Phi step's u16 count rows against its f32 rows
+        count_rows["mismatches"], count_rows["words_fit"] = rc.check_count_rows()
+        log(f"count rows: {count_rows}")
 
     tokens_per_s = N * args.steps / elapsed
     s_bar = visited / (N * args.steps)
@@ -161,6 +169,7 @@ def main():
                                 "sweep": round(float(stage_ms[2]), 3), "count": round(float(stage_ms[3]), 3)},
                    "parallelism": "1 GPU",
                    "schedule": tune if tune is not None else args.schedule,
+                   "count_rows": count_rows,
                    "abi_path_pcie_inclusive": abi},
         "roofline": roofline,
         "cpu_baseline": cpu,

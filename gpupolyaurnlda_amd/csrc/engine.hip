@@ -113,19 +113,22 @@ int max_lds_per_block() {
 int lds_per_cu();
 
 int launch_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
-                      AliasSlot* out, hipStream_t st) {
+                      AliasSlot* out, hipStream_t st, u16* counts_out = nullptr, u8* fit_out = nullptr) {
   if (n_rows == 0) return 0;
   if (m > kMaxTopics) return fail("alias rows of %u entries (limit %u: slot indices are 16-bit in LDS)", m, kMaxTopics);
   // rows of up to 4096 slots: one workgroup of four wavefronts per row
   if (m <= kAliasBlockThreads * 16 && alias_block_lds(m) <= (u64)max_lds_per_block()) {
     const u64 lds = alias_block_lds(m);
     if (m <= kAliasBlockThreads * 4)
-      k_alias_block<4><<<n_rows, kAliasBlockThreads, lds, st>>>(rows, n_rows, m, divisor, sigma, alpha, out);
+      k_alias_block<4><<<n_rows, kAliasBlockThreads, lds, st>>>(rows, n_rows, m, divisor, sigma, alpha, out,
+                                                                counts_out, fit_out);
     else
-      k_alias_block<16><<<n_rows, kAliasBlockThreads, lds, st>>>(rows, n_rows, m, divisor, sigma, alpha, out);
+      k_alias_block<16><<<n_rows, kAliasBlockThreads, lds, st>>>(rows, n_rows, m, divisor, sigma, alpha, out,
+                                                                 counts_out, fit_out);
     HCK(hipGetLastError());
     return 0;
   }
+  if (fit_out) return fail("count rows need the block alias kernel (rows of at most 4096 slots)");
   u64 per = alias_lds_per_wave(m);
   u64 lim = (u64)max_lds_per_block();
   if (per > lim) return fail("row length %u needs %llu B of LDS (limit %llu)", m, (unsigned long long)per,
@@ -633,6 +636,12 @@ struct State {
   f32* phi = nullptr;
   AliasSlot* alias = nullptr;
   f32* sigma_a = nullptr;
+  // u16 rows of the phi~ counts beside the f32 rows, written by the Phi step (count_rows on):
+  // the persistent sweep reads them for the words whose counts fit (half the bytes per row)
+  bool count_rows = true;
+  u16* phi_counts = nullptr;
+  u8* counts_fit = nullptr;  // V, padded to 4 bytes
+  float2* topic_recip = nullptr;
   u32* n = nullptr;
   u32* C = nullptr;
   unsigned long long* topic_sum = nullptr;
@@ -677,7 +686,14 @@ int phi_step(hipStream_t st, hipEvent_t mid = nullptr) {
   HCK(hipGetLastError());
   CK(launch_phi_draw(S.n, S.phi, S.topic_sum, S.V, S.K, S.beta, 0, S.iter_dev, S.seed, S.pois, st));
   if (mid) HCK(hipEventRecord(mid, st));
-  CK(launch_alias_rows(S.phi, S.V, S.K, S.topic_sum, S.sigma_a, S.alpha, S.alias, st));
+  if (S.count_rows) {
+    k_topic_recip<<<(S.K + 1 + 255) / 256, 256, 0, st>>>(S.topic_sum, S.topic_recip, S.K);
+    HCK(hipGetLastError());
+    CK(launch_alias_rows(S.phi, S.V, S.K, S.topic_sum, S.sigma_a, S.alpha, S.alias, st, S.phi_counts,
+                         S.counts_fit));
+  } else {
+    CK(launch_alias_rows(S.phi, S.V, S.K, S.topic_sum, S.sigma_a, S.alpha, S.alias, st));
+  }
   CK(fill_u32(S.n, (u64)S.V * S.K, 0, st));
   return 0;
 }
@@ -699,11 +715,40 @@ SweepParams sweep_params(Segment* sg, u32 doc_base) {
   P.sigma_a = S.sigma_a;
   P.alias = S.alias;
   P.visited = S.visited;
+  if (S.count_rows) {
+    P.phi_counts = S.phi_counts;
+    P.counts_fit = S.counts_fit;
+    P.topic_recip = S.topic_recip;
+  }
   return P;
 }
 
 // LDS bytes one wavefront needs for any document of the segment (counts + key table).
 u64 sweep_lds_bytes(const Segment* sg) { return doc_state_bytes(sg->max_len, S.K); }
+
+// Persistent sweep of documents whose maps stay within the register variants (k_sweep_persistent<8>),
+// at up to waves_per_cu wavefronts per CU as LDS (lds bytes per wavefront) allows.  With count rows
+// it runs k_sweep_persistent_counts in blocks of up to kCountsBlockWaves wavefronts sharing the
+// topic reciprocals, when that keeps at least as many wavefronts per CU.
+int launch_persistent_short(const SweepParams& P, u32* head, u32 n, u64 lds, u32 waves_per_cu, hipStream_t st) {
+  if (n == 0) return 0;
+  const u32 per_cu = std::max<u32>(1, std::min<u32>(waves_per_cu, (u32)(lds_per_cu() / lds)));
+  if (P.phi_counts) {
+    const u32 wb = std::min<u32>(kCountsBlockWaves, per_cu);
+    const u64 blk = topic_recip_lds_bytes(P.K) + (u64)wb * lds;
+    const u32 blocks_per_cu = std::min<u32>((u32)(lds_per_cu() / blk), (per_cu + wb - 1) / wb);
+    if (blk <= (u64)max_lds_per_block() && blocks_per_cu * wb >= per_cu) {
+      const u32 blocks = std::min<u32>((u32)S.cu_count * blocks_per_cu, (n + wb - 1) / wb);
+      k_sweep_persistent_counts<<<blocks, 64 * wb, blk, st>>>(P, head, n, (u32)lds);
+      HCK(hipGetLastError());
+      return 0;
+    }
+  }
+  const u32 waves = std::min<u32>((u32)S.cu_count * per_cu, n);
+  k_sweep_persistent<8><<<waves, 64, lds, st>>>(P, head, n);
+  HCK(hipGetLastError());
+  return 0;
+}
 
 // Resident sweep under the configured schedule:
 //   buckets              one launch per length bucket (k_sweep_lines<LMAX>), longest maps first;
@@ -740,10 +785,7 @@ int launch_sweep_schedule(const SweepParams& P0, Segment* sg, hipStream_t st) {
       SweepParams Q = P;
       Q.doc_order = sg->doc_lpt + n_long;
       const u64 lds_s = doc_state_bytes(sg->lens_desc[n_long], S.K);
-      const u32 per_cu = std::max<u32>(1, std::min<u32>(S.waves_per_cu, (u32)(lds_per_cu() / lds_s)));
-      const u32 waves = std::min<u32>((u32)S.cu_count * per_cu, sg->n_docs - n_long);
-      k_sweep_persistent<8><<<waves, 64, lds_s, st>>>(Q, S.heads, sg->n_docs - n_long);
-      HCK(hipGetLastError());
+      CK(launch_persistent_short(Q, S.heads, sg->n_docs - n_long, lds_s, S.waves_per_cu, st));
     }
     if (n_long > 0) HCK(hipStreamWaitEvent(st, S.join, 0));
     return 0;
@@ -763,10 +805,7 @@ int launch_sweep_schedule(const SweepParams& P0, Segment* sg, hipStream_t st) {
   if (n_long < sg->n_docs) {
     SweepParams Q = P;
     Q.doc_order = sg->doc_lpt + n_long;
-    u32 per_cu = std::max<u32>(1, std::min<u32>(S.waves_per_cu, (u32)(lds_per_cu() / lds)));
-    u32 waves = std::min<u32>((u32)S.cu_count * per_cu, sg->n_docs - n_long);
-    k_sweep_persistent<8><<<waves, 64, lds, st>>>(Q, S.heads, sg->n_docs - n_long);  // lines <= 8 in registers
-    HCK(hipGetLastError());
+    CK(launch_persistent_short(Q, S.heads, sg->n_docs - n_long, lds, S.waves_per_cu, st));
   }
   if (n_long > 0) HCK(hipStreamWaitEvent(st, S.join, 0));
   return 0;
@@ -785,11 +824,7 @@ int launch_part_sweep(const SweepParams& P0, const Segment* sg, u32 p, u32* head
   P.doc_order = sg->part_lpt + pt.doc0;
   const u64 lds = doc_state_bytes(pt.max_len, S.K);
   if (lds > (u64)max_lds_per_block()) return fail("doc-topic map exceeds LDS");
-  const u32 per_cu = std::max<u32>(1, std::min<u32>(kAbiWavesPerCu, (u32)(lds_per_cu() / lds)));
-  const u32 waves = std::min<u32>((u32)S.cu_count * per_cu, n);
-  k_sweep_persistent<8><<<waves, 64, lds, st>>>(P, head, n);
-  HCK(hipGetLastError());
-  return 0;
+  return launch_persistent_short(P, head, n, lds, kAbiWavesPerCu, st);
 }
 
 // A segment some buffer still has in flight (between its sample_z_async and sync_buffer).
@@ -900,6 +935,9 @@ void release_all() {
   dfree(S.phi);
   dfree(S.alias);
   dfree(S.sigma_a);
+  dfree(S.phi_counts);
+  dfree(S.counts_fit);
+  dfree(S.topic_recip);
   dfree(S.n);
   dfree(S.C);
   dfree(S.topic_sum);
@@ -962,6 +1000,9 @@ int do_initialize(Args* args, Buffer* buffers, u32 n_buffers) {
   CK(dalloc(&S.phi, VK));
   CK(dalloc(&S.alias, VK));
   CK(dalloc(&S.sigma_a, S.V));
+  CK(dalloc(&S.phi_counts, VK));
+  CK(dalloc(&S.counts_fit, ((u64)S.V + 3) / 4 * 4));
+  CK(dalloc(&S.topic_recip, (u64)S.K + 1));
   CK(dalloc(&S.n, VK));
   CK(dalloc(&S.C, S.V));
   CK(dalloc(&S.topic_sum, S.K));
@@ -983,6 +1024,7 @@ int do_initialize(Args* args, Buffer* buffers, u32 n_buffers) {
   // phi / alias hold the uniform table until the first sample_phi
   CK(fill_u32((u32*)S.phi, VK, 0, S.stream));
   CK(fill_u32((u32*)S.sigma_a, S.V, 0, S.stream));
+  CK(fill_u32((u32*)S.counts_fit, ((u64)S.V + 3) / 4, 0, S.stream));  // f32 rows until a Phi step
   for (u32 i = 0; i < n_buffers; ++i) {
     Buffer* B = &buffers[i];
     BufferState* bs = new BufferState();
@@ -1212,6 +1254,41 @@ extern "C" int gpulda_device_info(char* name, int name_len, int* cu_count, uint6
 extern "C" int gpulda_set_part_tokens(uint64_t min_tokens) {
   if (min_tokens == 0) return fail("part size must be positive");
   g_part_min_tokens = min_tokens;
+  return 0;
+}
+
+extern "C" int gpulda_set_count_rows(int on) {
+  CK(require_ready());
+  const bool want = on != 0;
+  if (want != S.count_rows) {
+    drop_graph();  // a captured iteration holds the Phi step and sweep launches of the old setting
+    // rows and flags of an earlier Phi step may be stale: f32 rows until the next Phi step
+    if (want) CK(fill_u32((u32*)S.counts_fit, ((u64)S.V + 3) / 4, 0, S.stream));
+    S.count_rows = want;
+  }
+  return 0;
+}
+
+extern "C" int gpulda_check_count_rows(uint64_t* mismatches, uint64_t* words_fit) {
+  CK(require_ready());
+  HCK(hipStreamSynchronize(S.stream));
+  unsigned long long* bad = nullptr;
+  CK(dalloc(&bad, 1));
+  int rc = fill_u64(bad, 1, 0, 0);
+  if (!rc) {
+    k_check_counts<<<1024, 256>>>(S.phi, S.phi_counts, S.counts_fit, S.topic_recip, S.V, S.K, bad);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = fail("count-row check failed");
+  }
+  unsigned long long nb = 0;
+  if (!rc && hipMemcpy(&nb, bad, sizeof(nb), hipMemcpyDeviceToHost) != hipSuccess) rc = fail("copy failed");
+  dfree(bad);
+  CK(rc);
+  std::vector<u8> fit(S.V);
+  HCK(hipMemcpy(fit.data(), S.counts_fit, S.V, hipMemcpyDeviceToHost));
+  u64 nf = 0;
+  for (u8 f : fit) nf += f ? 1 : 0;
+  if (mismatches) *mismatches = nb;
+  if (words_fit) *words_fit = nf;
   return 0;
 }
 

@@ -32,6 +32,12 @@ struct SweepParams {
   const f32* sigma_a;    // V
   const AliasSlot* alias;  // V x K word-major
   unsigned long long* visited;  // optional: += map entries visited by the sparse sums
+  // Rows of the phi~ Poisson counts as u16 (half the bytes of the f32 rows), from which
+  // phi = RN(count / S_k) is recovered exactly (phi_from_count); read only by
+  // k_sweep_persistent_counts, for the words whose counts all fit 16 bits.
+  const u16* phi_counts = nullptr;      // V x K, word-major (null: f32 rows only)
+  const u8* counts_fit = nullptr;       // V: 1 when the row's counts all fit 16 bits
+  const float2* topic_recip = nullptr;  // K + 1: (RN(1 / S_k), S_k); (0, 1) for S_k = 0 and entry K
 };
 
 // Fold-in batch: request slots of a token arena.
@@ -102,6 +108,16 @@ __global__ void k_sweep_lines(SweepParams P);
 template <int LMAX>
 __global__ void k_sweep_persistent(SweepParams P, u32* head, u32 n_items);
 __global__ void k_sweep_persistent_long(SweepParams P, u32* head, u32 n_items);
+// Persistent sweep reading the u16 count rows where they fit: blocks of kCountsBlockWaves
+// wavefronts share one LDS copy of topic_recip; wavefront w's document state lies at
+// topic_recip_lds_bytes(K) + w * wave_lds bytes.
+constexpr u32 kCountsBlockWaves = 16;
+LDA_HD u64 topic_recip_lds_bytes(u32 K) { return ((u64)(K + 1) * 8 + 15) / 16 * 16; }
+__global__ void k_sweep_persistent_counts(SweepParams P, u32* head, u32 n_items, u32 wave_lds);
+__global__ void k_topic_recip(const unsigned long long* topic_sum, float2* out, u32 K);
+// Number of cells of the rows marked as fitting whose recovered phi differs from the f32 row.
+__global__ void k_check_counts(const f32* phi, const u16* counts, const u8* fit, const float2* recip, u32 V, u32 K,
+                               unsigned long long* bad);
 // Block-per-document sweep (4 wavefronts per document); head == null: block b takes item b.
 __global__ void k_sweep_block(SweepParams P, u32* head, u32 n_items);
 // Launch buckets of the per-bucket schedule, by map lines (0 = any size).  Buckets 1-8 run the
@@ -141,8 +157,10 @@ LDA_HD u64 alias_lds_per_wave(u32 m) { return (u64)((m + 3) & ~3u) * 14 + 68 * 4
 // dynamic LDS per row (u64 prefixes, padded f32 row values, u16 slot indices).
 constexpr u32 kAliasBlockThreads = 256;
 LDA_HD u64 alias_block_lds(u32 m) { return (u64)((m + 3) & ~3u) * 14 + (kAliasBlockThreads + 4) * 4; }
+// With a divisor (rows of phi~ counts), counts_out / fit_out (optional) receive each row's counts
+// as u16 (clamped) and whether they all fit.
 template <int PER>
 __global__ void k_alias_block(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
-                              AliasSlot* out);
+                              AliasSlot* out, u16* counts_out, u8* fit_out);
 
 }  // namespace lda

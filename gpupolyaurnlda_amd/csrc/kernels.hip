@@ -352,10 +352,12 @@ template __global__ void k_alias_rows<0>(f32*, u32, u32, const unsigned long lon
 template <int PER>
 __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32 n_rows, u32 m,
                                                     const unsigned long long* __restrict__ divisor,
-                                                    f32* __restrict__ sigma, f32 alpha, AliasSlot* __restrict__ out) {
+                                                    f32* __restrict__ sigma, f32 alpha, AliasSlot* __restrict__ out,
+                                                    u16* __restrict__ counts_out, u8* __restrict__ fit_out) {
   extern __shared__ unsigned char alias_smem[];
   __shared__ unsigned long long red[3][kAliasBlockThreads / 64];
   __shared__ f32 shT;
+  __shared__ u32 wide[kAliasBlockThreads / 64];
   const u32 row = blockIdx.x;
   if (row >= n_rows) return;
   const u32 t = threadIdx.x;
@@ -388,6 +390,20 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
         const u32 k = (u32)c * kAliasBlockThreads + t;
         dr[c] = k < m ? divisor[k] : 0ull;
       }
+      if (counts_out) {  // the row's counts as u16 (clamped) and whether they all fit
+        u32 over = 0;
+#pragma unroll
+        for (int c = 0; c < PER; ++c) {
+          const u32 k = (u32)c * kAliasBlockThreads + t;
+          if (k < m) {
+            const bool fits = xv[c] <= 65535.0f;
+            counts_out[(u64)row * m + k] = fits ? (u16)(u32)xv[c] : (u16)0xFFFFu;
+            over |= fits ? 0u : 1u;
+          }
+        }
+        const u64 b = ballot(over != 0u);
+        if (lane == 0) wide[wave] = b ? 1u : 0u;
+      }
 #pragma unroll
       for (int c = 0; c < PER; ++c) {
         const u32 k = (u32)c * kAliasBlockThreads + t;
@@ -405,6 +421,12 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
     }
   }
   __syncthreads();
+  if (divisor && counts_out && t == 0) {
+    u32 any = 0;
+#pragma unroll
+    for (u32 g = 0; g < kAliasBlockThreads / 64; ++g) any |= wide[g];
+    fit_out[row] = any ? 0 : 1;
+  }
   if (wave == 0) {
     f32 acc = 0.0f;
     for (u32 k = lane; k < m; k += 64) acc = acc + xs[xp(k)];
@@ -544,8 +566,9 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
     o[k] = sl;
   }
 }
-template __global__ void k_alias_block<4>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*);
-template __global__ void k_alias_block<16>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*);
+template __global__ void k_alias_block<4>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u16*, u8*);
+template __global__ void k_alias_block<16>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u16*,
+                                           u8*);
 
 // ---------------------------------------------------------------------------------------------
 // z-sweep, one wavefront per document, the doc-topic state resident in LDS: a count per topic and
@@ -573,27 +596,83 @@ constexpr u32 kNoSlot = 0xFFFFFFFFu;
 // of an empty slot) returns 0 without touching memory, and the address path of a buffer load (a
 // 32-bit offset under a scalar base) processes a wavefront's scattered reads faster than a global
 // load does.
+// A word's row of phi as the sweep reads it: the f32 row, or (counts == true) the row of phi~
+// counts as u16 with the per-topic (RN(1 / S_k), S_k) in LDS, from which each value is recovered
+// exactly.  Either way a buffer resource bounds the row, so the empty slot's offset (past the row)
+// reads 0.
 struct PhiRow {
   __amdgpu_buffer_rsrc_t rs;
+  const float2* recip;  // LDS, K + 1 entries (counts rows only)
+  bool counts;
 };
 constexpr int kBufferWord3 = 0x00020000;  // raw 32-bit buffer access (gfx9 family)
 
 __device__ __forceinline__ PhiRow phi_row(const f32* prow, u32 K) {
   PhiRow r;
   r.rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<f32*>(prow), (short)0, (int)(K * sizeof(f32)), kBufferWord3);
+  r.recip = nullptr;
+  r.counts = false;
+  return r;
+}
+__device__ __forceinline__ PhiRow phi_count_row(const u16* crow, u32 K, const float2* recip) {
+  PhiRow r;
+  r.rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<u16*>(crow), (short)0, (int)(K * sizeof(u16)), kBufferWord3);
+  r.recip = recip;
+  r.counts = true;
   return r;
 }
 
-// phi[w][topic] for a table value key2 = 2 * topic (byte offset 2 * key2).
+// phi = RN(x / S_k) from the count x < 2^16 and rc = (y, S_k), y = RN(1 / S_k): q0 = RN(x * y) is
+// within an ulp of x / S_k, its remainder x - q0 * S_k is exact in one fma, and one correction
+// q0 + r * y rounds to RN(x / S_k) (Markstein's theorem for a correctly rounded reciprocal).  The
+// identity is scale-free, so checking every 16-bit significand of x against every 24-bit
+// significand of S_k covers all x < 2^16 and all S_k of the sampler; x = 0 gives +0.
+__device__ __forceinline__ f32 phi_from_count(u32 x, float2 rc) {
+  const f32 xf = (f32)x;
+  const f32 q0 = xf * rc.x;
+  const f32 r = __builtin_fmaf(-q0, rc.y, xf);
+  return __builtin_fmaf(r, rc.x, q0);
+}
+
+// phi[w][topic] for a table value key2 = 2 * topic: the f32 row at byte offset 2 * key2, the u16
+// count row at byte offset key2.
 __device__ __forceinline__ f32 row_at2(const PhiRow& row, u32 key2) {
   return __builtin_bit_cast(f32, __builtin_amdgcn_raw_buffer_load_b32(row.rs, (int)(key2 * 2u), 0, 0));
+}
+__device__ __forceinline__ u32 count_row_at2(const PhiRow& row, u32 key2) {
+  return (u32)__builtin_amdgcn_raw_buffer_load_b16(row.rs, (int)key2, 0, 0);
+}
+__device__ __forceinline__ f32 row_value(const PhiRow& row, u32 key2) {
+  if (row.counts) return phi_from_count(count_row_at2(row, key2), row.recip[key2 >> 1]);
+  return row_at2(row, key2);
+}
+// N values of a row, every load issued before any arithmetic on them.
+template <int N>
+__device__ __forceinline__ void row_values(const PhiRow& row, const u32 (&e)[N], f32 (&f)[N]) {
+  if (row.counts) {
+    u32 x[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) x[j] = count_row_at2(row, e[j]);
+    // the reciprocals (LDS) four at a time, which keeps the registers of eight lines in budget
+#pragma unroll
+    for (int j0 = 0; j0 < N; j0 += 4) {
+      float2 rc[4];
+#pragma unroll
+      for (int j = j0; j < j0 + 4 && j < N; ++j) rc[j - j0] = row.recip[e[j] >> 1];
+#pragma unroll
+      for (int j = j0; j < j0 + 4 && j < N; ++j) f[j] = phi_from_count(x[j], rc[j - j0]);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) f[j] = row_at2(row, e[j]);
+  }
 }
 
 // Mass of one table slot, count * phi: an empty slot (2K) reads the zero count C[K] and a phi offset
 // past the row, so its mass is 0 * 0 = +0, the value the definition gives it, with no branch (all
 // reads of a token in flight together).
 __device__ __forceinline__ f32 slot_mass(const u16* C, u32 key2, const PhiRow& row) {
-  return (f32)count_at(C, key2) * row_at2(row, key2);
+  return (f32)count_at(C, key2) * row_value(row, key2);
 }
 
 __device__ __forceinline__ f32 combine_groups(f32 g0, f32 g1, f32 g2, f32 g3) { return ((g0 + g1) + g2) + g3; }
@@ -602,11 +681,13 @@ __device__ __forceinline__ f32 combine_groups(f32 g0, f32 g1, f32 g2, f32 g3) { 
 struct DocLds {
   u16* C;
   u16* Kt;
+  const float2* recip;  // the block's copy of topic_recip (sweeps that read count rows), else null
 };
-__device__ __forceinline__ DocLds doc_lds(u32* smem, u32 K) {
+__device__ __forceinline__ DocLds doc_lds(u32* smem, u32 K, const float2* recip = nullptr) {
   DocLds d;
   d.C = (u16*)smem;
   d.Kt = (u16*)((char*)smem + doc_counts_bytes(K));
+  d.recip = recip;
   return d;
 }
 
@@ -694,8 +775,7 @@ __device__ __forceinline__ Draw draw_topic_regs(const u32 (&e)[NL], const u16* C
 #pragma unroll
   for (int j = 0; j < NL; ++j) c[j] = count_at(C, e[j]);
   f32 f[NL];
-#pragma unroll
-  for (int j = 0; j < NL; ++j) f[j] = row_at2(prow, e[j]);
+  row_values<NL>(prow, e, f);
   f32 m[NL];
 #pragma unroll
   for (int j = 0; j < NL; ++j) m[j] = (f32)c[j] * f[j];
@@ -777,8 +857,7 @@ __device__ __forceinline__ Draw draw_topic_loop(const u16* Kt, const u16* C, u32
 #pragma unroll
     for (u32 k = 0; k < 8; ++k) c[k] = count_at(C, e[k]);
     f32 f[8];
-#pragma unroll
-    for (u32 k = 0; k < 8; ++k) f[k] = row_at2(prow, e[k]);
+    row_values<8>(prow, e, f);
     f32 m[8];
 #pragma unroll
     for (u32 k = 0; k < 8; ++k) m[k] = (f32)c[k] * f[k];
@@ -799,8 +878,7 @@ __device__ __forceinline__ Draw draw_topic_loop(const u16* Kt, const u16* C, u32
 #pragma unroll
     for (u32 k = 0; k < 8; ++k) c[k] = count_at(C, e[k]);
     f32 f[8];
-#pragma unroll
-    for (u32 k = 0; k < 8; ++k) f[k] = row_at2(prow, e[k]);
+    row_values<8>(prow, e, f);
     f32 m[8];
 #pragma unroll
     for (u32 k = 0; k < 8; ++k) m[k] = (f32)c[k] * f[k];
@@ -894,7 +972,7 @@ __device__ __forceinline__ void remove_first(const DocLds& m, DocState& ds, u32 
 struct Chunk {
   u32 ww;      // word
   u32 zz2;     // 2 * topic before the pass
-  u32 slot;    // alias slot of a dense draw (from r.y)
+  u32 slot;    // alias slot of a dense draw (from r.y); kCountsFitBit: the word's count row fits
   f32 uz;      // uniform of the dense draw's keep test (r.z)
   f32 sa;      // sigma_a of the word
   f32 ux;      // uniform of the sparse/dense choice
@@ -902,11 +980,13 @@ struct Chunk {
   u32 znew;    // topic after the pass (lanes below the current token)
 };
 
+constexpr u32 kCountsFitBit = 0x80000000u;
+
 // Tokens t .. nt-1 of a chunk while the table has NL lines (NL = 1..8: lines held in registers;
 // 0: read per token).  Returns the next token: nt, or the token after the one whose insertion
 // grew the map.  All variants evaluate the same operations in the same order: a variant for fewer
 // lines drops only lines that would hold no mass.
-template <int NL>
+template <int NL, bool CR>
 __device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, const DocLds& m, DocState& ds, Chunk& ch, u32 t,
                                             u32 nt, u32& visited) {
   const u32 lane = lane_id();
@@ -924,7 +1004,9 @@ __device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, const DocLds& 
     wave_sync();  // the previous token's count writes before this token's reads
     if (sa > 0.0f) {
       visited += ds.occupied;
-      const PhiRow prow = phi_row(P.phi + (u64)wt * K, K);
+      const u32 slot_fit = readlane_u32(ch.slot, t);
+      const PhiRow prow = CR && (slot_fit & kCountsFitBit) ? phi_count_row(P.phi_counts + (u64)wt * K, K, m.recip)
+                                                          : phi_row(P.phi + (u64)wt * K, K);
       const f32 ux = readlane_f32(ch.ux, t), uw = readlane_f32(ch.uw, t);
       Draw d;
       if constexpr (NL > 0) d = draw_topic_regs<NL>(e, m.C, prow, sa, ux, uw);
@@ -932,7 +1014,7 @@ __device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, const DocLds& 
       if (d.key2 != kNoSlot) {
         new2 = d.key2;
       } else {  // dense: the word's alias slot is read only now (about one token in eight)
-        const u32 sl = readlane_u32(ch.slot, t);
+        const u32 sl = slot_fit & ~kCountsFitBit;
         const AliasSlot a = P.alias[(u64)wt * K + sl];
         new2 = 2u * (readlane_f32(ch.uz, t) < a.prob ? sl : a.alias);
       }
@@ -956,27 +1038,27 @@ __device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, const DocLds& 
 // Runs tokens t .. nt-1 of a chunk with the variant for the table's current line count (register
 // variants up to 8 lines, within the kernel's LMAX, which caps its registers; larger tables use
 // the line loop).  The choice is made again only when an insertion grows the table.
-template <int LMAX>
+template <int LMAX, bool CR>
 __device__ __forceinline__ u32 sweep_tokens_any(const SweepParams& P, const DocLds& m, DocState& ds, Chunk& ch,
                                                 u32 t, u32 nt, u32& visited) {
   switch (ds.cap / kMapLine) {  // a map always has at least one line
-    case 1: if (LMAX >= 1) return sweep_tokens<1>(P, m, ds, ch, t, nt, visited); break;
-    case 2: if (LMAX >= 2) return sweep_tokens<2>(P, m, ds, ch, t, nt, visited); break;
-    case 3: if (LMAX >= 4) return sweep_tokens<3>(P, m, ds, ch, t, nt, visited); break;
-    case 4: if (LMAX >= 4) return sweep_tokens<4>(P, m, ds, ch, t, nt, visited); break;
-    case 5: if (LMAX >= 8) return sweep_tokens<5>(P, m, ds, ch, t, nt, visited); break;
-    case 6: if (LMAX >= 8) return sweep_tokens<6>(P, m, ds, ch, t, nt, visited); break;
-    case 7: if (LMAX >= 8) return sweep_tokens<7>(P, m, ds, ch, t, nt, visited); break;
-    case 8: if (LMAX >= 8) return sweep_tokens<8>(P, m, ds, ch, t, nt, visited); break;
+    case 1: if (LMAX >= 1) return sweep_tokens<1, CR>(P, m, ds, ch, t, nt, visited); break;
+    case 2: if (LMAX >= 2) return sweep_tokens<2, CR>(P, m, ds, ch, t, nt, visited); break;
+    case 3: if (LMAX >= 4) return sweep_tokens<3, CR>(P, m, ds, ch, t, nt, visited); break;
+    case 4: if (LMAX >= 4) return sweep_tokens<4, CR>(P, m, ds, ch, t, nt, visited); break;
+    case 5: if (LMAX >= 8) return sweep_tokens<5, CR>(P, m, ds, ch, t, nt, visited); break;
+    case 6: if (LMAX >= 8) return sweep_tokens<6, CR>(P, m, ds, ch, t, nt, visited); break;
+    case 7: if (LMAX >= 8) return sweep_tokens<7, CR>(P, m, ds, ch, t, nt, visited); break;
+    case 8: if (LMAX >= 8) return sweep_tokens<8, CR>(P, m, ds, ch, t, nt, visited); break;
     default: break;
   }
-  return sweep_tokens<0>(P, m, ds, ch, t, nt, visited);
+  return sweep_tokens<0, CR>(P, m, ds, ch, t, nt, visited);
 }
 
 // One pass over a document's tokens with Philox counters (token, gdoc, iter, tag).  visited gains
 // the sum over sampled tokens of the map's distinct topics (at most 65535 * 65535 per pass, so
 // the pass counts in 32 bits).
-template <int LMAX>
+template <int LMAX, bool CR = false>
 __device__ __forceinline__ void sweep_pass(const SweepParams& P, const DocLds& m, DocState& ds, u32 iter, u32 tag,
                                            u64& visited) {
   const u32 lane = lane_id();
@@ -993,13 +1075,14 @@ __device__ __forceinline__ void sweep_pass(const SweepParams& P, const DocLds& m
     // per-token values that do not depend on the map: computed for the whole chunk
     ch.sa = valid ? P.sigma_a[ch.ww] : 0.0f;
     ch.slot = bounded(r.y, K);
+    if (CR && valid && P.counts_fit[ch.ww]) ch.slot |= kCountsFitBit;
     ch.uz = u24(r.z);
     ch.ux = u24(r.x);
     ch.uw = u24(r.w);
     ch.znew = zz;
     const u32 nt = ds.L - c0 < 64 ? ds.L - c0 : 64;
     remove_first(m, ds, readlane_u32(ch.zz2, 0));
-    for (u32 t = 0; t < nt;) t = sweep_tokens_any<LMAX>(P, m, ds, ch, t, nt, pass_visited);
+    for (u32 t = 0; t < nt;) t = sweep_tokens_any<LMAX, CR>(P, m, ds, ch, t, nt, pass_visited);
     if (valid) P.z[ds.b + i] = (u16)ch.znew;
   }
   visited += pass_visited;
@@ -1029,11 +1112,12 @@ __device__ __forceinline__ void foldin_sweeps(const SweepParams& P, u32* smem, D
 
 __device__ __forceinline__ u32 sweep_iteration(const SweepParams& P) { return P.iter_ptr ? *P.iter_ptr : P.iter; }
 
-template <int LMAX>
-__device__ __forceinline__ void sweep_doc(const SweepParams& P, u32 doc, u32* smem) {
+template <int LMAX, bool CR = false>
+__device__ __forceinline__ void sweep_doc(const SweepParams& P, u32 doc, u32* smem,
+                                          const float2* recip = nullptr) {
   DocState ds = open_doc(P, doc, smem);
   u64 visited = 0;
-  sweep_pass<LMAX>(P, doc_lds(smem, P.K), ds, sweep_iteration(P), kTagSweep, visited);
+  sweep_pass<LMAX, CR>(P, doc_lds(smem, P.K, recip), ds, sweep_iteration(P), kTagSweep, visited);
   if (lane_id() == 0 && P.visited) atomicAdd(P.visited, (unsigned long long)visited);
 }
 
@@ -1075,6 +1159,52 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
   }
 }
 template __global__ void k_sweep_persistent<8>(SweepParams, u32*, u32);
+
+// The persistent sweep reading u16 count rows for the words whose counts fit (half the bytes of a
+// row: the sweep's time follows the row lines it misses in L2).  The block's wavefronts share one
+// LDS copy of topic_recip, loaded before any of them starts; after that the wavefronts run
+// independently, as in k_sweep_persistent.
+__global__ __launch_bounds__(64 * kCountsBlockWaves) __attribute__((amdgpu_waves_per_eu(8))) void
+k_sweep_persistent_counts(SweepParams P, u32* head, u32 n_items, u32 wave_lds) {
+  extern __shared__ u32 sweep_smem[];
+  float2* recip = reinterpret_cast<float2*>(sweep_smem);
+  for (u32 k = threadIdx.x; k <= P.K; k += blockDim.x) recip[k] = P.topic_recip[k];
+  __syncthreads();
+  const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: the state's address in SGPRs
+  u32* smem = sweep_smem + (topic_recip_lds_bytes(P.K) + (u64)wave * wave_lds) / 4;
+  for (;;) {
+    u32 item = 0;
+    if (lane_id() == 0) item = atomicAdd(head, 1u);
+    item = readlane_u32(item, 0);
+    if (item >= n_items) return;
+    sweep_doc<8, true>(P, P.doc_order[item], smem, recip);
+  }
+}
+
+__global__ void k_topic_recip(const unsigned long long* __restrict__ topic_sum, float2* __restrict__ out, u32 K) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > K) return;
+  float2 v = make_float2(0.0f, 1.0f);
+  if (k < K) {
+    const f32 d = (f32)(f64)topic_sum[k];  // the divisor the f32 rows were normalised by
+    if (d > 0.0f) v = make_float2(1.0f / d, d);
+  }
+  out[k] = v;
+}
+
+__global__ void k_check_counts(const f32* __restrict__ phi, const u16* __restrict__ counts,
+                               const u8* __restrict__ fit, const float2* __restrict__ recip, u32 V, u32 K,
+                               unsigned long long* bad) {
+  const u64 n = (u64)V * K;
+  u32 mine = 0;
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
+    const u32 v = (u32)(i / K), k = (u32)(i - (u64)v * K);
+    if (!fit[v]) continue;
+    const f32 a = phi_from_count(counts[i], recip[k]);
+    mine += __builtin_bit_cast(u32, a) != __builtin_bit_cast(u32, phi[i]) ? 1u : 0u;
+  }
+  if (mine) atomicAdd(bad, (unsigned long long)mine);
+}
 
 // Persistent form of the line-loop sweep for long documents (maps past the register variants),
 // without the eight-wavefront register budget: the line loop then keeps its scalar state in

@@ -63,6 +63,19 @@ class ResidentCorpus:
                 for r in table[:int(n[0])]]
         return rows, int(chosen.value)
 
+    def count_rows(self, enabled: bool = True) -> None:
+        """u16 rows of the phi~ counts read by the persistent sweep where they fit (default on);
+        z does not depend on it."""
+        check(self._lib.gpulda_set_count_rows(1 if enabled else 0), "set_count_rows")
+
+    def check_count_rows(self):
+        """(cells whose phi recovered from the count rows differs from the f32 rows, words held as
+        count rows) for the last Phi step."""
+        bad = np.zeros(1, dtype=np.uint64)
+        fit = np.zeros(1, dtype=np.uint64)
+        check(self._lib.gpulda_check_count_rows(ptr(bad, C.c_uint64), ptr(fit, C.c_uint64)), "check_count_rows")
+        return int(bad[0]), int(fit[0])
+
     def iterate_untimed(self, n: int = 1) -> None:
         check(self._lib.gpulda_corpus_iterate(n, None), "corpus_iterate")
 

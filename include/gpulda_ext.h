@@ -62,6 +62,14 @@ uint64_t gpulda_slice_hash(const uint32_t* w, const uint32_t* d_len, uint32_t n_
  * tokens each (default 4 Mi), and overlaps the copies of z for one part with the sweep of another.
  * Applies to slices first seen after the call.  Results do not depend on it. */
 int gpulda_set_part_tokens(uint64_t min_tokens);
+/* Count rows (default on): each Phi step also writes the rows of the phi~ Poisson counts as u16
+ * for the words whose counts all fit 16 bits, and the persistent sweep reads those (half the bytes
+ * of an f32 row) and recovers phi = count / S_k exactly.  Off: the f32 rows only.  The sampled
+ * z does not depend on it.  Switching on takes effect at the next Phi step. */
+int gpulda_set_count_rows(int on);
+/* Checks the count rows of the last Phi step against the f32 rows: cells whose recovered phi
+ * differs (bitwise), and the words whose rows are held as counts. */
+int gpulda_check_count_rows(uint64_t* mismatches, uint64_t* words_fit);
 
 /* ---- model state (valid between initialize and cleanup) ---------------------------------- */
 uint32_t gpulda_iteration(void);                         /* number of sample_phi calls so far */

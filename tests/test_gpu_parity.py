@@ -405,6 +405,42 @@ def test_autotune_keeps_chain():
         assert np.array_equal(rc.z(), z_tuned)
 
 
+def test_count_rows_exact_and_unfit_rows(orc):
+    """The u16 count rows the persistent sweep reads where a word's counts fit 16 bits: every phi
+    recovered from them equals the f32 row bit for bit, and the chain equals the f32 rows' chain and
+    the oracle's.  Word 0 takes half of the 600k tokens over four topics, so its counts pass 16 bits
+    and its row stays f32 (the kernel mixes both kinds of row in one sweep)."""
+    from gpupolyaurnlda_amd.resident import ResidentCorpus
+    rng = np.random.default_rng(21)
+    V, K, D, L = 60, 4, 2000, 300
+    N = D * L
+    w = np.where(rng.random(N) < 0.5, 0, rng.integers(1, V, N)).astype(np.uint32)
+    d_len = np.full(D, L, np.uint32)
+    z = rng.integers(0, K, N).astype(np.uint32)
+    doc_off = np.concatenate([[0], np.cumsum(d_len)]).astype(np.uint64)
+    alpha, beta = 0.1, 0.01
+    runs = {}
+    for on in (True, False):
+        with ResidentCorpus(w, d_len, K, V, alpha, beta, seed=7, z=z) as rc:
+            rc.schedule(1, 32)
+            rc.count_rows(on)
+            zs = []
+            for it in range(3):
+                rc.iterate_untimed(1)
+                zs.append(rc.z())
+                if on:
+                    bad, fit = rc.check_count_rows()
+                    say(f"count rows it={it + 1}: mismatches={bad} words held as counts={fit} of {V}")
+                    assert bad == 0 and fit == V - 1
+            runs[on] = (zs, rc.counts())
+    model = orc.Model(w, doc_off, z, V, K, alpha, beta, seed=7, counts="z", threads=8)
+    for it in range(3):
+        model.iterate()
+        assert np.array_equal(runs[True][0][it], model.z), it
+        assert np.array_equal(runs[False][0][it], model.z), it
+    assert np.array_equal(runs[True][1], model.n) and np.array_equal(runs[False][1], model.n)
+
+
 @pytest.mark.parametrize("V,K,Lmax,n_req,zero_rows", [(500, 256, 800, 60, 0), (3000, 1024, 2500, 24, 40)])
 def test_foldin_bit_exact(S, orc, V, K, Lmax, n_req, zero_rows):
     """Fold-in against the oracle.  The second case has maps beyond eight lines (the line-loop
