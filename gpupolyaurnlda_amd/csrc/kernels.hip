@@ -622,18 +622,6 @@ __device__ __forceinline__ PhiRow phi_count_row(const u16* crow, u32 K, const fl
   return r;
 }
 
-// phi = RN(x / S_k) from the count x < 2^16 and rc = (y, S_k), y = RN(1 / S_k): q0 = RN(x * y) is
-// within an ulp of x / S_k, its remainder x - q0 * S_k is exact in one fma, and one correction
-// q0 + r * y rounds to RN(x / S_k) (Markstein's theorem for a correctly rounded reciprocal).  The
-// identity is scale-free, so checking every 16-bit significand of x against every 24-bit
-// significand of S_k covers all x < 2^16 and all S_k of the sampler; x = 0 gives +0.
-__device__ __forceinline__ f32 phi_from_count(u32 x, float2 rc) {
-  const f32 xf = (f32)x;
-  const f32 q0 = xf * rc.x;
-  const f32 r = __builtin_fmaf(-q0, rc.y, xf);
-  return __builtin_fmaf(r, rc.x, q0);
-}
-
 // phi[w][topic] for a table value key2 = 2 * topic: the f32 row at byte offset 2 * key2, the u16
 // count row at byte offset key2.
 __device__ __forceinline__ f32 row_at2(const PhiRow& row, u32 key2) {
@@ -643,7 +631,7 @@ __device__ __forceinline__ u32 count_row_at2(const PhiRow& row, u32 key2) {
   return (u32)__builtin_amdgcn_raw_buffer_load_b16(row.rs, (int)key2, 0, 0);
 }
 __device__ __forceinline__ f32 row_value(const PhiRow& row, u32 key2) {
-  if (row.counts) return phi_from_count(count_row_at2(row, key2), row.recip[key2 >> 1]);
+  if (row.counts) return phi_from_count(count_row_at2(row, key2), row.recip[key2 >> 1].x, row.recip[key2 >> 1].y);
   return row_at2(row, key2);
 }
 // N values of a row, every load issued before any arithmetic on them.
@@ -660,7 +648,7 @@ __device__ __forceinline__ void row_values(const PhiRow& row, const u32 (&e)[N],
 #pragma unroll
       for (int j = j0; j < j0 + 4 && j < N; ++j) rc[j - j0] = row.recip[e[j] >> 1];
 #pragma unroll
-      for (int j = j0; j < j0 + 4 && j < N; ++j) f[j] = phi_from_count(x[j], rc[j - j0]);
+      for (int j = j0; j < j0 + 4 && j < N; ++j) f[j] = phi_from_count(x[j], rc[j - j0].x, rc[j - j0].y);
     }
   } else {
 #pragma unroll
@@ -1200,7 +1188,7 @@ __global__ void k_check_counts(const f32* __restrict__ phi, const u16* __restric
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
     const u32 v = (u32)(i / K), k = (u32)(i - (u64)v * K);
     if (!fit[v]) continue;
-    const f32 a = phi_from_count(counts[i], recip[k]);
+    const f32 a = phi_from_count(counts[i], recip[k].x, recip[k].y);
     mine += __builtin_bit_cast(u32, a) != __builtin_bit_cast(u32, phi[i]) ? 1u : 0u;
   }
   if (mine) atomicAdd(bad, (unsigned long long)mine);

@@ -163,6 +163,20 @@ struct AliasSlot {
   u32 alias;
 };
 
+// phi = RN(x / d) for a phi~ count x < 2^16 and a topic divisor d = (f32)S_k > 0, from
+// y = RN(1 / d): q0 = RN(x * y) is within an ulp of x / d, its remainder x - q0 * d is exact in
+// one fma, and the correction q0 + r * y rounds to RN(x / d) (Markstein's theorem for a correctly
+// rounded reciprocal).  Every operation scales exactly with powers of two here (no underflow or
+// overflow for x < 2^16, 1 <= d < 2^64), so checking every 16-bit significand of x against every
+// 24-bit significand of d covers all inputs; x = 0 gives +0, and so does y = 0 with d = 1 (the
+// value used for S_k = 0, whose counts are all 0).
+LDA_HD f32 phi_from_count(u32 x, f32 y, f32 d) {
+  const f32 xf = (f32)x;
+  const f32 q0 = xf * y;
+  const f32 r = __builtin_fmaf(-q0, d, xf);
+  return __builtin_fmaf(r, y, q0);
+}
+
 LDA_HD u32 draw_alias_slot(const AliasSlot* row, u32 n, u32 r_slot, u32 r_keep) {
   u32 slot = bounded(r_slot, n);
   AliasSlot s = row[slot];
