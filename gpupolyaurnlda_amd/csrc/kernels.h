@@ -158,9 +158,10 @@ LDA_HD u64 alias_lds_per_wave(u32 m) { return (u64)((m + 3) & ~3u) * 14 + 68 * 4
 constexpr u32 kAliasBlockThreads = 256;
 LDA_HD u64 alias_block_lds(u32 m) { return (u64)((m + 3) & ~3u) * 14 + (kAliasBlockThreads + 4) * 4; }
 // With a divisor (rows of phi~ counts), counts_out / fit_out (optional) receive each row's counts
-// as u16 (clamped) and whether they all fit.
+// as u16 (clamped) and whether the sweep is to read them: they all fit and keep_f32 (optional)
+// does not mark the row.
 template <int PER>
 __global__ void k_alias_block(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
-                              AliasSlot* out, u16* counts_out, u8* fit_out);
+                              AliasSlot* out, u16* counts_out, u8* fit_out, const u8* keep_f32);
 
 }  // namespace lda

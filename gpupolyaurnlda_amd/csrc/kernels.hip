@@ -353,7 +353,8 @@ template <int PER>
 __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32 n_rows, u32 m,
                                                     const unsigned long long* __restrict__ divisor,
                                                     f32* __restrict__ sigma, f32 alpha, AliasSlot* __restrict__ out,
-                                                    u16* __restrict__ counts_out, u8* __restrict__ fit_out) {
+                                                    u16* __restrict__ counts_out, u8* __restrict__ fit_out,
+                                                    const u8* __restrict__ keep_f32) {
   extern __shared__ unsigned char alias_smem[];
   __shared__ unsigned long long red[3][kAliasBlockThreads / 64];
   __shared__ f32 shT;
@@ -425,7 +426,7 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
     u32 any = 0;
 #pragma unroll
     for (u32 g = 0; g < kAliasBlockThreads / 64; ++g) any |= wide[g];
-    fit_out[row] = any ? 0 : 1;
+    fit_out[row] = any || (keep_f32 && keep_f32[row]) ? 0 : 1;
   }
   if (wave == 0) {
     f32 acc = 0.0f;
@@ -566,9 +567,10 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
     o[k] = sl;
   }
 }
-template __global__ void k_alias_block<4>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u16*, u8*);
+template __global__ void k_alias_block<4>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u16*, u8*,
+                                          const u8*);
 template __global__ void k_alias_block<16>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u16*,
-                                           u8*);
+                                           u8*, const u8*);
 
 // ---------------------------------------------------------------------------------------------
 // z-sweep, one wavefront per document, the doc-topic state resident in LDS: a count per topic and

@@ -68,6 +68,10 @@ class ResidentCorpus:
         z does not depend on it."""
         check(self._lib.gpulda_set_count_rows(1 if enabled else 0), "set_count_rows")
 
+    def f32_rows(self, n_words: int) -> None:
+        """The n_words most frequent words keep f32 rows (default: 1 MiB of rows)."""
+        check(self._lib.gpulda_set_f32_rows(int(n_words)), "set_f32_rows")
+
     def check_count_rows(self):
         """(cells whose phi recovered from the count rows differs from the f32 rows, words held as
         count rows) for the last Phi step."""

@@ -67,6 +67,10 @@ int gpulda_set_part_tokens(uint64_t min_tokens);
  * of an f32 row) and recovers phi = count / S_k exactly.  Off: the f32 rows only.  The sampled
  * z does not depend on it.  Switching on takes effect at the next Phi step. */
 int gpulda_set_count_rows(int on);
+/* The n most frequent words (by the counts C given to initialize) keep f32 rows even when their
+ * counts fit: their rows stay in L2, so the count rows would add conversion work and save no
+ * misses.  Default: as many words as 1 MiB of f32 rows (256 at K = 1024).  From the next Phi step. */
+int gpulda_set_f32_rows(uint32_t n_words);
 /* Checks the count rows of the last Phi step against the f32 rows: cells whose recovered phi
  * differs (bitwise), and the words whose rows are held as counts. */
 int gpulda_check_count_rows(uint64_t* mismatches, uint64_t* words_fit);

@@ -409,7 +409,8 @@ def test_count_rows_exact_and_unfit_rows(orc):
     """The u16 count rows the persistent sweep reads where a word's counts fit 16 bits: every phi
     recovered from them equals the f32 row bit for bit, and the chain equals the f32 rows' chain and
     the oracle's.  Word 0 takes half of the 600k tokens over four topics, so its counts pass 16 bits
-    and its row stays f32 (the kernel mixes both kinds of row in one sweep)."""
+    and its row stays f32 (the kernel mixes both kinds of row in one sweep); the two next most
+    frequent words are kept on f32 rows by gpulda_set_f32_rows."""
     from gpupolyaurnlda_amd.resident import ResidentCorpus
     rng = np.random.default_rng(21)
     V, K, D, L = 60, 4, 2000, 300
@@ -424,6 +425,7 @@ def test_count_rows_exact_and_unfit_rows(orc):
         with ResidentCorpus(w, d_len, K, V, alpha, beta, seed=7, z=z) as rc:
             rc.schedule(1, 32)
             rc.count_rows(on)
+            rc.f32_rows(3)  # word 0 and the two next most frequent
             zs = []
             for it in range(3):
                 rc.iterate_untimed(1)
@@ -431,7 +433,7 @@ def test_count_rows_exact_and_unfit_rows(orc):
                 if on:
                     bad, fit = rc.check_count_rows()
                     say(f"count rows it={it + 1}: mismatches={bad} words held as counts={fit} of {V}")
-                    assert bad == 0 and fit == V - 1
+                    assert bad == 0 and fit == V - 3
             runs[on] = (zs, rc.counts())
     model = orc.Model(w, doc_off, z, V, K, alpha, beta, seed=7, counts="z", threads=8)
     for it in range(3):
