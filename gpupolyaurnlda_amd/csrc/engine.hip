@@ -113,7 +113,7 @@ int max_lds_per_block() {
 int lds_per_cu();
 
 int launch_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
-                      AliasSlot* out, hipStream_t st, u16* counts_out = nullptr, u8* fit_out = nullptr,
+                      AliasSlot* out, hipStream_t st, u8* wide = nullptr, u8* fit_out = nullptr,
                       const u8* keep_f32 = nullptr) {
   if (n_rows == 0) return 0;
   if (m > kMaxTopics) return fail("alias rows of %u entries (limit %u: slot indices are 16-bit in LDS)", m, kMaxTopics);
@@ -122,10 +122,10 @@ int launch_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* di
     const u64 lds = alias_block_lds(m);
     if (m <= kAliasBlockThreads * 4)
       k_alias_block<4><<<n_rows, kAliasBlockThreads, lds, st>>>(rows, n_rows, m, divisor, sigma, alpha, out,
-                                                                counts_out, fit_out, keep_f32);
+                                                                wide, fit_out, keep_f32);
     else
       k_alias_block<16><<<n_rows, kAliasBlockThreads, lds, st>>>(rows, n_rows, m, divisor, sigma, alpha, out,
-                                                                 counts_out, fit_out, keep_f32);
+                                                                 wide, fit_out, keep_f32);
     HCK(hipGetLastError());
     return 0;
   }
@@ -154,11 +154,12 @@ int launch_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* di
 constexpr u32 kPhiRows = 64;
 
 int launch_phi_draw(const u32* n, f32* phi, unsigned long long* tsum, u32 V, u32 K, f32 beta, u32 iter,
-                    const u32* iter_ptr, u64 seed, const AliasSlot* tables, hipStream_t st) {
+                    const u32* iter_ptr, u64 seed, const AliasSlot* tables, hipStream_t st, u16* counts = nullptr,
+                    u8* wide = nullptr) {
   CK(fill_u64(tsum, K, 0, st));
   dim3 grid((K + 255) / 256, (V + kPhiRows - 1) / kPhiRows);
   k_phi_draw<<<grid, 256, 0, st>>>(n, phi, tsum, V, K, beta, iter, iter_ptr, (u32)seed, (u32)(seed >> 32), tables,
-                                   kPhiRows);
+                                   kPhiRows, counts, wide);
   HCK(hipGetLastError());
   return 0;
 }
@@ -643,6 +644,7 @@ struct State {
   u16* phi_counts = nullptr;
   u8* counts_fit = nullptr;  // V, padded to 4 bytes
   u8* keep_f32 = nullptr;    // V, padded to 4 bytes: the most frequent words, read as f32 rows
+  u8* counts_wide = nullptr; // V, padded to 4 bytes: a count of the row passed 16 bits (per Phi step)
   std::vector<u32> host_C;   // corpus counts per word given to initialize (empty: none given)
   float2* topic_recip = nullptr;
   u32* n = nullptr;
@@ -687,12 +689,16 @@ int phi_step(hipStream_t st, hipEvent_t mid = nullptr) {
   S.doc_counter = 0;
   k_bump_u32<<<1, 64, 0, st>>>(S.iter_dev);
   HCK(hipGetLastError());
-  CK(launch_phi_draw(S.n, S.phi, S.topic_sum, S.V, S.K, S.beta, 0, S.iter_dev, S.seed, S.pois, st));
+  if (S.count_rows)
+    CK(launch_phi_draw(S.n, S.phi, S.topic_sum, S.V, S.K, S.beta, 0, S.iter_dev, S.seed, S.pois, st, S.phi_counts,
+                       S.counts_wide));
+  else
+    CK(launch_phi_draw(S.n, S.phi, S.topic_sum, S.V, S.K, S.beta, 0, S.iter_dev, S.seed, S.pois, st));
   if (mid) HCK(hipEventRecord(mid, st));
   if (S.count_rows) {
     k_topic_recip<<<(S.K + 1 + 255) / 256, 256, 0, st>>>(S.topic_sum, S.topic_recip, S.K);
     HCK(hipGetLastError());
-    CK(launch_alias_rows(S.phi, S.V, S.K, S.topic_sum, S.sigma_a, S.alpha, S.alias, st, S.phi_counts,
+    CK(launch_alias_rows(S.phi, S.V, S.K, S.topic_sum, S.sigma_a, S.alpha, S.alias, st, S.counts_wide,
                          S.counts_fit, S.keep_f32));
   } else {
     CK(launch_alias_rows(S.phi, S.V, S.K, S.topic_sum, S.sigma_a, S.alpha, S.alias, st));
@@ -941,6 +947,7 @@ void release_all() {
   dfree(S.phi_counts);
   dfree(S.counts_fit);
   dfree(S.keep_f32);
+  dfree(S.counts_wide);
   dfree(S.topic_recip);
   dfree(S.n);
   dfree(S.C);
@@ -1026,6 +1033,7 @@ int do_initialize(Args* args, Buffer* buffers, u32 n_buffers) {
   CK(dalloc(&S.phi_counts, VK));
   CK(dalloc(&S.counts_fit, ((u64)S.V + 3) / 4 * 4));
   CK(dalloc(&S.keep_f32, ((u64)S.V + 3) / 4 * 4));
+  CK(dalloc(&S.counts_wide, ((u64)S.V + 3) / 4 * 4));
   CK(dalloc(&S.topic_recip, (u64)S.K + 1));
   CK(dalloc(&S.n, VK));
   CK(dalloc(&S.C, S.V));
@@ -1051,6 +1059,7 @@ int do_initialize(Args* args, Buffer* buffers, u32 n_buffers) {
   CK(fill_u32((u32*)S.phi, VK, 0, S.stream));
   CK(fill_u32((u32*)S.sigma_a, S.V, 0, S.stream));
   CK(fill_u32((u32*)S.counts_fit, ((u64)S.V + 3) / 4, 0, S.stream));  // f32 rows until a Phi step
+  CK(fill_u32((u32*)S.counts_wide, ((u64)S.V + 3) / 4, 0, S.stream));
   for (u32 i = 0; i < n_buffers; ++i) {
     Buffer* B = &buffers[i];
     BufferState* bs = new BufferState();

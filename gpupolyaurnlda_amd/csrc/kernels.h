@@ -95,8 +95,10 @@ __global__ void k_fill_u64(unsigned long long* p, u64 n, unsigned long long v);
 __global__ void k_z_narrow(const u32* in, u16* out, u64 n, u32 K, u32* bad);
 __global__ void k_z_widen(const u16* in, u32* out, u64 n);
 __global__ void k_z_init(u16* z, const u64* doc_off, u32 n_docs, u32 doc_base, u32 K, u32 k0, u32 k1);
+// counts (optional): the draws as u16 rows (clamped), wide[v] set where a draw of row v passes 16 bits.
 __global__ void k_phi_draw(const u32* n, f32* phi, unsigned long long* topic_sum, u32 V, u32 K, f32 beta, u32 iter,
-                           const u32* iter_ptr, u32 k0, u32 k1, const AliasSlot* tables, u32 rows_per_block);
+                           const u32* iter_ptr, u32 k0, u32 k1, const AliasSlot* tables, u32 rows_per_block,
+                           u16* counts, u8* wide);
 __global__ void k_bump_u32(u32* p);
 __global__ void k_init_counts(const u32* C, u32* n, u32 V, u32 K, f32 beta, u32 k0, u32 k1, const AliasSlot* tables,
                               u32 rows_per_block);
@@ -157,11 +159,11 @@ LDA_HD u64 alias_lds_per_wave(u32 m) { return (u64)((m + 3) & ~3u) * 14 + 68 * 4
 // dynamic LDS per row (u64 prefixes, padded f32 row values, u16 slot indices).
 constexpr u32 kAliasBlockThreads = 256;
 LDA_HD u64 alias_block_lds(u32 m) { return (u64)((m + 3) & ~3u) * 14 + (kAliasBlockThreads + 4) * 4; }
-// With a divisor (rows of phi~ counts), counts_out / fit_out (optional) receive each row's counts
-// as u16 (clamped) and whether the sweep is to read them: they all fit and keep_f32 (optional)
+// fit_out (optional) receives whether the sweep is to read each row's u16 counts (written by
+// k_phi_draw): none passed 16 bits (wide, cleared here for the next step) and keep_f32 (optional)
 // does not mark the row.
 template <int PER>
 __global__ void k_alias_block(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
-                              AliasSlot* out, u16* counts_out, u8* fit_out, const u8* keep_f32);
+                              AliasSlot* out, u8* wide, u8* fit_out, const u8* keep_f32);
 
 }  // namespace lda

@@ -69,7 +69,8 @@ __global__ void k_z_init(u16* z, const u64* doc_off, u32 n_docs, u32 doc_base, u
 // ---------------------------------------------------------------------------------------------
 __global__ void k_phi_draw(const u32* __restrict__ n, f32* __restrict__ phi, unsigned long long* topic_sum,
                            u32 V, u32 K, f32 beta, u32 iter_arg, const u32* iter_ptr, u32 k0, u32 k1,
-                           const AliasSlot* __restrict__ tables, u32 rows_per_block) {
+                           const AliasSlot* __restrict__ tables, u32 rows_per_block, u16* __restrict__ counts,
+                           u8* __restrict__ wide) {
   u32 k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K) return;
   const u32 iter = iter_ptr ? *iter_ptr : iter_arg;
@@ -80,6 +81,10 @@ __global__ void k_phi_draw(const u32* __restrict__ n, f32* __restrict__ phi, uns
     u64 idx = (u64)v * K + k;
     u32 x = poisson_draw(n[idx], beta, tables, k, v, iter, kTagPhi, k0, k1);
     phi[idx] = (f32)x;
+    if (counts) {  // the u16 count row, and its word marked when a count passes 16 bits
+      counts[idx] = x <= 0xFFFFu ? (u16)x : (u16)0xFFFFu;
+      if (x > 0xFFFFu) wide[v] = 1;
+    }
     acc += x;
   }
   if (acc) atomicAdd(&topic_sum[k], (unsigned long long)acc);
@@ -353,12 +358,11 @@ template <int PER>
 __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32 n_rows, u32 m,
                                                     const unsigned long long* __restrict__ divisor,
                                                     f32* __restrict__ sigma, f32 alpha, AliasSlot* __restrict__ out,
-                                                    u16* __restrict__ counts_out, u8* __restrict__ fit_out,
+                                                    u8* __restrict__ wide, u8* __restrict__ fit_out,
                                                     const u8* __restrict__ keep_f32) {
   extern __shared__ unsigned char alias_smem[];
   __shared__ unsigned long long red[3][kAliasBlockThreads / 64];
   __shared__ f32 shT;
-  __shared__ u32 wide[kAliasBlockThreads / 64];
   const u32 row = blockIdx.x;
   if (row >= n_rows) return;
   const u32 t = threadIdx.x;
@@ -391,20 +395,6 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
         const u32 k = (u32)c * kAliasBlockThreads + t;
         dr[c] = k < m ? divisor[k] : 0ull;
       }
-      if (counts_out) {  // the row's counts as u16 (clamped) and whether they all fit
-        u32 over = 0;
-#pragma unroll
-        for (int c = 0; c < PER; ++c) {
-          const u32 k = (u32)c * kAliasBlockThreads + t;
-          if (k < m) {
-            const bool fits = xv[c] <= 65535.0f;
-            counts_out[(u64)row * m + k] = fits ? (u16)(u32)xv[c] : (u16)0xFFFFu;
-            over |= fits ? 0u : 1u;
-          }
-        }
-        const u64 b = ballot(over != 0u);
-        if (lane == 0) wide[wave] = b ? 1u : 0u;
-      }
 #pragma unroll
       for (int c = 0; c < PER; ++c) {
         const u32 k = (u32)c * kAliasBlockThreads + t;
@@ -422,11 +412,9 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
     }
   }
   __syncthreads();
-  if (divisor && counts_out && t == 0) {
-    u32 any = 0;
-#pragma unroll
-    for (u32 g = 0; g < kAliasBlockThreads / 64; ++g) any |= wide[g];
-    fit_out[row] = any || (keep_f32 && keep_f32[row]) ? 0 : 1;
+  if (fit_out && t == 0) {  // whether the sweep reads the row's counts; the mark cleared for the next step
+    fit_out[row] = wide[row] || (keep_f32 && keep_f32[row]) ? 0 : 1;
+    wide[row] = 0;
   }
   if (wave == 0) {
     f32 acc = 0.0f;
@@ -567,10 +555,10 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
     o[k] = sl;
   }
 }
-template __global__ void k_alias_block<4>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u16*, u8*,
+template __global__ void k_alias_block<4>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u8*, u8*,
                                           const u8*);
-template __global__ void k_alias_block<16>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u16*,
-                                           u8*, const u8*);
+template __global__ void k_alias_block<16>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u8*, u8*,
+                                           const u8*);
 
 // ---------------------------------------------------------------------------------------------
 // z-sweep, one wavefront per document, the doc-topic state resident in LDS: a count per topic and
@@ -632,8 +620,15 @@ __device__ __forceinline__ f32 row_at2(const PhiRow& row, u32 key2) {
 __device__ __forceinline__ u32 count_row_at2(const PhiRow& row, u32 key2) {
   return (u32)__builtin_amdgcn_raw_buffer_load_b16(row.rs, (int)key2, 0, 0);
 }
+// The reciprocal pair of a table value key2 = 2 * topic (byte offset 4 * key2: one shift-add).
+__device__ __forceinline__ float2 recip_at2(const PhiRow& row, u32 key2) {
+  return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(row.recip) + key2 * 4u);
+}
 __device__ __forceinline__ f32 row_value(const PhiRow& row, u32 key2) {
-  if (row.counts) return phi_from_count(count_row_at2(row, key2), row.recip[key2 >> 1].x, row.recip[key2 >> 1].y);
+  if (row.counts) {
+    const float2 rc = recip_at2(row, key2);
+    return phi_from_count(count_row_at2(row, key2), rc.x, rc.y);
+  }
   return row_at2(row, key2);
 }
 // N values of a row, every load issued before any arithmetic on them.
@@ -648,7 +643,7 @@ __device__ __forceinline__ void row_values(const PhiRow& row, const u32 (&e)[N],
     for (int j0 = 0; j0 < N; j0 += 4) {
       float2 rc[4];
 #pragma unroll
-      for (int j = j0; j < j0 + 4 && j < N; ++j) rc[j - j0] = row.recip[e[j] >> 1];
+      for (int j = j0; j < j0 + 4 && j < N; ++j) rc[j - j0] = recip_at2(row, e[j]);
 #pragma unroll
       for (int j = j0; j < j0 + 4 && j < N; ++j) f[j] = phi_from_count(x[j], rc[j - j0].x, rc[j - j0].y);
     }
