@@ -902,9 +902,10 @@ struct CountStep {
 __device__ __forceinline__ CountStep count_step(u16* C, u32 new2, u32 next2, u32 has_next) {
   const u32 lane = lane_id();
   const u32 n = has_next ? (next2 == new2 ? 0u : 2u) : 1u;
-  u32 old = 1u;
+  // lane 0 the new topic, lane 1 the next token's old topic (every other lane the new topic)
+  const u32 a = lane == 1 ? next2 : new2;
+  u32 old = 0u;
   if (lane < n) {
-    const u32 a = lane == 0 ? new2 : next2;
     old = count_at(C, a);
     count_put(C, a, old + (lane == 0 ? 1u : 0xFFFFu));  // 16-bit: + 0xFFFF is - 1
   }
@@ -982,10 +983,11 @@ __device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, const DocLds& 
 #pragma unroll
     for (int j = 0; j < NL; ++j) e[j] = m.Kt[j * 64 + lane];
   }
+  u32 cur2 = readlane_u32(ch.zz2, t);  // token t's topic before the pass (carried from next2 below)
   for (; t < nt; ++t) {
     const u32 wt = readlane_u32(ch.ww, t);
     const f32 sa = readlane_f32(ch.sa, t);
-    u32 new2 = readlane_u32(ch.zz2, t);  // a word with an all-zero phi row keeps its topic
+    u32 new2 = cur2;  // a word with an all-zero phi row keeps its topic
     wave_sync();  // the previous token's count writes before this token's reads
     if (sa > 0.0f) {
       visited += ds.occupied;
@@ -1007,6 +1009,7 @@ __device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, const DocLds& 
     if (lane == t) ch.znew = new2 >> 1;
     const u32 has_next = t + 1 < nt ? 1u : 0u;
     const u32 next2 = readlane_u32(ch.zz2, t + 1 < 64 ? t + 1 : 63);
+    cur2 = next2;
     const CountStep cs = count_step(m.C, new2, next2, has_next);
     if (cs.entered | cs.left) {
       wave_sync();
