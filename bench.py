@@ -123,9 +123,10 @@ def main():
         try:
             tr = json.load(open(TRAFFIC_FILE))
             if tr.get("workload") == spec.name:
-                traffic_run = {k: tr.get(k) for k in ("s_bar", "alg_bytes_per_launch", "bytes_per_launch",
-                                                      "traffic_over_alg", "command")}
-                if tr.get("s_bar") is not None and abs(float(tr["s_bar"]) - s_bar) <= 1e-3 * s_bar:
+                traffic_run = {k: tr.get(k) for k in ("kernel", "count_rows", "s_bar", "alg_bytes_per_launch",
+                                                      "bytes_per_launch", "traffic_over_alg", "command")}
+                same_kernel = bool(tr.get("count_rows", False)) == (not args.no_count_rows)
+                if same_kernel and tr.get("s_bar") is not None and abs(float(tr["s_bar"]) - s_bar) <= 1e-3 * s_bar:
                     traffic = tr.get("bytes_per_launch")
         except (OSError, ValueError):
             traffic = None

@@ -380,6 +380,12 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
   f32* x = rows + (u64)row * m;
   AliasSlot* o = out + (u64)row * m;
 
+  // whether the sweep is to read the row's u16 counts, read with the row by a thread off the canonical
+  // sum's path (wavefront 1); the row's mark is cleared for the next step
+  const bool fit_lane = fit_out && t == 64;
+  u32 fit = 0;
+  if (fit_lane) fit = wide[row] || (keep_f32 && keep_f32[row]) ? 0u : 1u;
+
   // A: loads (coalesced), normalisation, the row staged in LDS; the canonical sum by wavefront 0
   {
     f32 xv[PER];
@@ -412,8 +418,8 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
     }
   }
   __syncthreads();
-  if (fit_out && t == 0) {  // whether the sweep reads the row's counts; the mark cleared for the next step
-    fit_out[row] = wide[row] || (keep_f32 && keep_f32[row]) ? 0 : 1;
+  if (fit_lane) {
+    fit_out[row] = (u8)fit;
     wide[row] = 0;
   }
   if (wave == 0) {
