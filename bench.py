@@ -105,10 +105,11 @@ def main():
     elapsed = time.perf_counter() - t0
     visited = rc.visited()
     log(f"timed {args.steps} iterations in {elapsed:.3f}s; stage ms {stage_ms.tolist()}")
-    count_rows = {"on": not args.no_count_rows}
-    if not args.no_count_rows:  # the last Phi step's u16 count rows against its f32 rows
-        count_rows["mismatches"], count_rows["words_fit"] = rc.check_count_rows()
-        log(f"count rows: {count_rows}")
+    # count rows as the timed iterations ran them (the autotuner may switch them off), and the last
+    # Phi step's u16 count rows against its f32 rows
+    on, bad, fit = rc.check_count_rows()
+    count_rows = {"on": on, "mismatches": bad, "words_fit": fit} if on else {"on": False}
+    log(f"count rows: {count_rows}")
 
     tokens_per_s = N * args.steps / elapsed
     s_bar = visited / (N * args.steps)
@@ -125,7 +126,7 @@ def main():
             if tr.get("workload") == spec.name:
                 traffic_run = {k: tr.get(k) for k in ("kernel", "count_rows", "s_bar", "alg_bytes_per_launch",
                                                       "bytes_per_launch", "traffic_over_alg", "command")}
-                same_kernel = bool(tr.get("count_rows", False)) == (not args.no_count_rows)
+                same_kernel = bool(tr.get("count_rows", False)) == count_rows["on"]
                 if same_kernel and tr.get("s_bar") is not None and abs(float(tr["s_bar"]) - s_bar) <= 1e-3 * s_bar:
                     traffic = tr.get("bytes_per_launch")
         except (OSError, ValueError):

@@ -101,7 +101,7 @@ _SIGS = {
     "gpulda_set_part_tokens": (C.c_int, [C.c_uint64]),
     "gpulda_set_count_rows": (C.c_int, [C.c_int]),
     "gpulda_set_f32_rows": (C.c_int, [C.c_uint32]),
-    "gpulda_check_count_rows": (C.c_int, [u64p, u64p]),
+    "gpulda_check_count_rows": (C.c_int, [C.POINTER(C.c_int), u64p, u64p]),
     "gpulda_copy_topic_sums": (C.c_int, [u64p]),
     "gpulda_copy_poisson_tables": (C.c_int, [f32p, u32p]),
     "gpulda_sweep_stats": (C.c_int, [u64p]),

@@ -1309,8 +1309,14 @@ extern "C" int gpulda_set_f32_rows(uint32_t n_words) {
   return set_keep_f32(n_words);
 }
 
-extern "C" int gpulda_check_count_rows(uint64_t* mismatches, uint64_t* words_fit) {
+extern "C" int gpulda_check_count_rows(int* on, uint64_t* mismatches, uint64_t* words_fit) {
   CK(require_ready());
+  if (on) *on = S.count_rows ? 1 : 0;
+  if (!S.count_rows) {  // rows and flags of an earlier Phi step may be stale: nothing to check
+    if (mismatches) *mismatches = 0;
+    if (words_fit) *words_fit = 0;
+    return 0;
+  }
   HCK(hipStreamSynchronize(S.stream));
   unsigned long long* bad = nullptr;
   CK(dalloc(&bad, 1));
@@ -1604,14 +1610,20 @@ extern "C" int gpulda_corpus_autotune(float* table, uint32_t max_candidates, uin
     int sched;
     u32 waves;
     u32 minlen;
+    u32 rows;  // 1: count rows where the persistent sweep can read them (as the last Phi step wrote them)
   };
-  std::vector<Cand> cands = {{kSchedBuckets, 0, 0}};
-  for (u32 w : {4u, 8u, 16u, 32u}) cands.push_back({kSchedPersistent, w, 0});
-  const u32 longest = sg->lens_desc.empty() ? 0 : sg->lens_desc[0];
-  for (u32 m : {2048u, 8192u})
-    if (longest >= m) cands.push_back({kSchedPersistentBlock, 16, m});
-  for (u32 m : {512u, 1024u, 2048u})
-    if (longest > m) cands.push_back({kSchedPersistentSplit, 32, m});
+  // count rows only as the last Phi step left them: when they are on, each persistent candidate
+  // is also timed on f32 rows, and switching them off applies from the next Phi step
+  const u32 cr = S.count_rows ? 1u : 0u;
+  std::vector<Cand> cands = {{kSchedBuckets, 0, 0, 0}};
+  for (u32 r = 0; r <= cr; ++r) {
+    for (u32 w : {4u, 8u, 16u, 32u}) cands.push_back({kSchedPersistent, w, 0, r});
+    const u32 longest = sg->lens_desc.empty() ? 0 : sg->lens_desc[0];
+    for (u32 m : {2048u, 8192u})
+      if (longest >= m) cands.push_back({kSchedPersistentBlock, 16, m, r});
+    for (u32 m : {512u, 1024u, 2048u})
+      if (longest > m) cands.push_back({kSchedPersistentSplit, 32, m, r});
+  }
   u16* saved = nullptr;
   CK(dalloc(&saved, sg->n_tokens));
   unsigned long long vis0 = 0;
@@ -1624,8 +1636,14 @@ extern "C" int gpulda_corpus_autotune(float* table, uint32_t max_candidates, uin
     S.schedule = cands[i].sched;
     S.waves_per_cu = cands[i].waves ? cands[i].waves : 16;
     S.block_min_len = cands[i].minlen;
+    SweepParams P = sweep_params(sg, 0);
+    if (!cands[i].rows) {
+      P.phi_counts = nullptr;
+      P.counts_fit = nullptr;
+      P.topic_recip = nullptr;
+    }
     HCK(hipEventRecord(S.ev[0], S.stream));
-    int rc = launch_sweep_schedule(sweep_params(sg, 0), sg, S.stream);
+    int rc = launch_sweep_schedule(P, sg, S.stream);
     HCK(hipEventRecord(S.ev[1], S.stream));
     HCK(hipMemcpyAsync(sg->z, saved, sg->n_tokens * 2, hipMemcpyDeviceToDevice, S.stream));
     HCK(hipStreamSynchronize(S.stream));
@@ -1636,10 +1654,11 @@ extern "C" int gpulda_corpus_autotune(float* table, uint32_t max_candidates, uin
     float ms = 0;
     HCK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
     if (table && i < max_candidates) {
-      table[4 * i + 0] = (float)cands[i].sched;
-      table[4 * i + 1] = (float)cands[i].waves;
-      table[4 * i + 2] = (float)cands[i].minlen;
-      table[4 * i + 3] = ms;
+      table[5 * i + 0] = (float)cands[i].sched;
+      table[5 * i + 1] = (float)cands[i].waves;
+      table[5 * i + 2] = (float)cands[i].minlen;
+      table[5 * i + 3] = (float)cands[i].rows;
+      table[5 * i + 4] = ms;
     }
     if (ms < best) {
       best = ms;
@@ -1651,6 +1670,7 @@ extern "C" int gpulda_corpus_autotune(float* table, uint32_t max_candidates, uin
   S.schedule = cands[best_i].sched;
   S.waves_per_cu = cands[best_i].waves ? cands[best_i].waves : 16;
   S.block_min_len = cands[best_i].minlen;
+  if (cr && !cands[best_i].rows) S.count_rows = false;  // the Phi step stops writing them
   drop_graph();
   if (n_candidates) *n_candidates = (u32)cands.size();
   if (chosen) *chosen = best_i;

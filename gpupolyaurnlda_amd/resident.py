@@ -53,14 +53,14 @@ class ResidentCorpus:
 
     def autotune(self):
         """Times every candidate schedule on one sweep (chain unchanged) and applies the fastest.
-        Returns (table rows {schedule, waves_per_cu, block_min_len, ms}, chosen index)."""
-        table = np.zeros((16, 4), dtype=np.float32)
+        Returns (table rows {schedule, waves_per_cu, block_min_len, count_rows, ms}, chosen index)."""
+        table = np.zeros((32, 5), dtype=np.float32)
         n = np.zeros(1, dtype=np.uint32)
         chosen = C.c_int(-1)
-        check(self._lib.gpulda_corpus_autotune(ptr(table, C.c_float), 16, ptr(n, C.c_uint32), C.byref(chosen)),
+        check(self._lib.gpulda_corpus_autotune(ptr(table, C.c_float), 32, ptr(n, C.c_uint32), C.byref(chosen)),
               "corpus_autotune")
-        rows = [dict(schedule=int(r[0]), waves_per_cu=int(r[1]), block_min_len=int(r[2]), ms=float(r[3]))
-                for r in table[:int(n[0])]]
+        rows = [dict(schedule=int(r[0]), waves_per_cu=int(r[1]), block_min_len=int(r[2]), count_rows=bool(r[3]),
+                     ms=float(r[4])) for r in table[:int(n[0])]]
         return rows, int(chosen.value)
 
     def count_rows(self, enabled: bool = True) -> None:
@@ -73,12 +73,14 @@ class ResidentCorpus:
         check(self._lib.gpulda_set_f32_rows(int(n_words)), "set_f32_rows")
 
     def check_count_rows(self):
-        """(cells whose phi recovered from the count rows differs from the f32 rows, words held as
-        count rows) for the last Phi step."""
+        """(count rows on, cells whose phi recovered from the count rows differs from the f32 rows,
+        words read as count rows) for the last Phi step."""
+        on = C.c_int(0)
         bad = np.zeros(1, dtype=np.uint64)
         fit = np.zeros(1, dtype=np.uint64)
-        check(self._lib.gpulda_check_count_rows(ptr(bad, C.c_uint64), ptr(fit, C.c_uint64)), "check_count_rows")
-        return int(bad[0]), int(fit[0])
+        check(self._lib.gpulda_check_count_rows(C.byref(on), ptr(bad, C.c_uint64), ptr(fit, C.c_uint64)),
+              "check_count_rows")
+        return bool(on.value), int(bad[0]), int(fit[0])
 
     def iterate_untimed(self, n: int = 1) -> None:
         check(self._lib.gpulda_corpus_iterate(n, None), "corpus_iterate")

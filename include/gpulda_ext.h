@@ -71,9 +71,10 @@ int gpulda_set_count_rows(int on);
  * counts fit: their rows stay in L2, so the count rows would add conversion work and save no
  * misses.  Default: as many words as 1 MiB of f32 rows (256 at K = 1024).  From the next Phi step. */
 int gpulda_set_f32_rows(uint32_t n_words);
-/* Checks the count rows of the last Phi step against the f32 rows: cells whose recovered phi
- * differs (bitwise), and the words whose rows are held as counts. */
-int gpulda_check_count_rows(uint64_t* mismatches, uint64_t* words_fit);
+/* Whether count rows are on, and (when on) the count rows of the last Phi step checked against the
+ * f32 rows: cells whose recovered phi differs (bitwise), and the words whose rows the sweep reads
+ * as counts. */
+int gpulda_check_count_rows(int* on, uint64_t* mismatches, uint64_t* words_fit);
 
 /* ---- model state (valid between initialize and cleanup) ---------------------------------- */
 uint32_t gpulda_iteration(void);                         /* number of sample_phi calls so far */
@@ -115,8 +116,10 @@ int gpulda_corpus_schedule(int schedule, uint32_t waves_per_cu, uint32_t block_m
  * memory); stage_ms of gpulda_corpus_iterate then reports only the total, in stage_ms[2]. */
 int gpulda_corpus_use_graph(int enabled);
 /* Times one sweep under each candidate schedule on the resident corpus (z restored after each
- * trial) and applies the fastest.  table: 4 floats per candidate
- * {schedule, waves_per_cu, block_min_len, ms}.  Needs at least one iteration first. */
+ * trial) and applies the fastest.  With count rows on, each persistent schedule is also timed on
+ * f32 rows, and count rows are switched off when that wins.  table: 5 floats per candidate
+ * {schedule, waves_per_cu, block_min_len, count rows (0/1), ms}.  Needs at least one iteration
+ * first. */
 int gpulda_corpus_autotune(float* table, uint32_t max_candidates, uint32_t* n_candidates, int* chosen);
 /* Collapsed joint log-likelihood log p(w, z | alpha, beta) of the current z and n_wk. */
 int gpulda_corpus_loglik(double* out);
