@@ -915,9 +915,12 @@ __device__ __forceinline__ CountStep count_step(u16* C, u32 new2, u32 next2, u32
     old = count_at(C, a);
     count_put(C, a, old + (lane == 0 ? 1u : 0xFFFFu));  // 16-bit: + 0xFFFF is - 1
   }
+  // entered: n != 0 and lane 0's old count was 0; left: n == 2 and lane 1's old count was 1 (the
+  // tests as lane masks, bits 0 and 1 taken on the scalar unit)
+  const u32 z0 = (u32)ballot(old == 0u), z1 = (u32)ballot(old == 1u);
   CountStep cs;
-  cs.entered = (n != 0u && readlane_u32(old, 0) == 0u) ? 1u : 0u;
-  cs.left = (n == 2u && readlane_u32(old, 1) == 1u) ? 1u : 0u;
+  cs.entered = z0 & (n | (n >> 1)) & 1u;  // n is 0, 1 or 2: (n | n >> 1) & 1 is n != 0
+  cs.left = (z1 >> 1) & (n >> 1) & 1u;    // n >> 1 is n == 2
   return cs;
 }
 
@@ -990,7 +993,7 @@ __device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, const DocLds& 
     for (int j = 0; j < NL; ++j) e[j] = m.Kt[j * 64 + lane];
   }
   u32 cur2 = readlane_u32(ch.zz2, t);  // token t's topic before the pass (carried from next2 below)
-  for (; t < nt; ++t) {
+  do {  // entered with t < nt
     const u32 wt = readlane_u32(ch.ww, t);
     const f32 sa = readlane_f32(ch.sa, t);
     u32 new2 = cur2;  // a word with an all-zero phi row keeps its topic
@@ -1009,7 +1012,7 @@ __device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, const DocLds& 
       } else {  // dense: the word's alias slot is read only now (about one token in eight)
         const u32 sl = slot_fit & ~kCountsFitBit;
         const AliasSlot a = P.alias[(u64)wt * K + sl];
-        new2 = 2u * (readlane_f32(ch.uz, t) < a.prob ? sl : a.alias);
+        new2 = uniform_u32(2u * (readlane_f32(ch.uz, t) < a.prob ? sl : a.alias));  // (the load is uniform)
       }
     }
     if (lane == t) ch.znew = new2 >> 1;
@@ -1025,7 +1028,7 @@ __device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, const DocLds& 
         for (int j = 0; j < NL; ++j) e[j] = m.Kt[j * 64 + lane];
       }
     }
-  }
+  } while (++t < nt);
   return nt;
 }
 
