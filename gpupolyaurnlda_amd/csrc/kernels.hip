@@ -755,12 +755,26 @@ __device__ __forceinline__ u32 column_pick(const u16* Kt, const u16* C, u32 line
   return last;
 }
 
+// A token's values as the draw reads them: lane t of sa, ux and uw holds its sigma_a and its
+// sparse/dense and target uniforms (a sweep passes its chunk's per-lane values, so nothing is moved
+// to scalar registers; a caller with one token passes the same values in every lane and t = 0).
+struct TokenU {
+  f32 sa, ux, uw;
+  u32 t;
+};
+// Sparse when u_x * (sigma_a + sigma_b) > sigma_a, evaluated in lane t and read from the lane mask.
+__device__ __forceinline__ bool draw_sparse(const TokenU& u, f32 sb) {
+  return (ballot(u.ux * (u.sa + sb) > u.sa) >> u.t) & 1u;
+}
+// The sparse target u_w * sigma_b, evaluated in lane t.
+__device__ __forceinline__ f32 draw_target(const TokenU& u, f32 sb) { return readlane_f32(u.uw * sb, u.t); }
+
 // Sparse/dense pick with the NL lines of the table held in registers (one variant per line count;
 // e[j] is line j's value in this lane's column): all phi gathers of the token are issued before
 // any arithmetic on them.
 template <int NL>
-__device__ __forceinline__ Draw draw_topic_regs(const u32 (&e)[NL], const u16* C, const PhiRow& prow, f32 sa,
-                                                f32 ux, f32 uw) {
+__device__ __forceinline__ Draw draw_topic_regs(const u32 (&e)[NL], const u16* C, const PhiRow& prow,
+                                                const TokenU& u) {
   // every count read and phi gather of the token issued before any arithmetic on them
   u32 c[NL];
 #pragma unroll
@@ -786,8 +800,8 @@ __device__ __forceinline__ Draw draw_topic_regs(const u32 (&e)[NL], const u16* C
   Draw d;
   d.key2 = kNoSlot;
   d.slot = kNoSlot;
-  if (!(ux * (sa + sb) > sa)) return d;
-  const f32 tgt = uw * sb;
+  if (!draw_sparse(u, sb)) return d;
+  const f32 tgt = draw_target(u, sb);
   f32 E;
   const u32 ls = pick_lane(A, P, tgt, E);
   // every lane scans its own column in registers with the canonical network (NL <= 16 lines lie
@@ -824,17 +838,17 @@ __device__ __forceinline__ Draw draw_topic_regs(const u32 (&e)[NL], const u16* C
 
 // The register variant with its lines loaded from the table.
 template <int NL>
-__device__ __forceinline__ Draw draw_topic_regs_lds(const DocLds& m, const PhiRow& prow, f32 sa, f32 ux, f32 uw) {
+__device__ __forceinline__ Draw draw_topic_regs_lds(const DocLds& m, const PhiRow& prow, const TokenU& u) {
   u32 e[NL];
 #pragma unroll
   for (int j = 0; j < NL; ++j) e[j] = m.Kt[j * 64 + lane_id()];
-  return draw_topic_regs<NL>(e, m.C, prow, sa, ux, uw);
+  return draw_topic_regs<NL>(e, m.C, prow, u);
 }
 
 // Sparse/dense pick for any number of lines: one pass for the column sums, then for a sparse draw
 // the in-column search over column l* read transposed.
-__device__ __forceinline__ Draw draw_topic_loop(const u16* Kt, const u16* C, u32 lines, const PhiRow& prow, f32 sa,
-                                                f32 ux, f32 uw, u32 e2) {
+__device__ __forceinline__ Draw draw_topic_loop(const u16* Kt, const u16* C, u32 lines, const PhiRow& prow,
+                                                const TokenU& u, u32 e2) {
   const u32 lane = lane_id();
   f32 G0 = 0.0f, G1 = 0.0f, G2 = 0.0f, G3 = 0.0f;
   // eight lines per step, so eight gathers are in flight at once; full steps read without a
@@ -888,8 +902,8 @@ __device__ __forceinline__ Draw draw_topic_loop(const u16* Kt, const u16* C, u32
   Draw d;
   d.key2 = kNoSlot;
   d.slot = kNoSlot;
-  if (!(ux * (sa + sb) > sa)) return d;
-  const f32 tgt = uw * sb;
+  if (!draw_sparse(u, sb)) return d;
+  const f32 tgt = draw_target(u, sb);
   f32 E;
   const u32 ls = pick_lane(A, P, tgt, E);
   d.slot = column_pick(Kt, C, lines, ls, prow, E, tgt, e2) * 64 + ls;
@@ -993,24 +1007,25 @@ __device__ __forceinline__ u32 sweep_tokens(const SweepParams& P, const DocLds& 
     for (int j = 0; j < NL; ++j) e[j] = m.Kt[j * 64 + lane];
   }
   u32 cur2 = readlane_u32(ch.zz2, t);  // token t's topic before the pass (carried from next2 below)
+  // per chunk, as lane masks: words with mass, and words whose count rows the sweep reads
+  const u64 live = ballot(ch.sa > 0.0f);
+  const u64 fit = CR ? ballot((ch.slot & kCountsFitBit) != 0u) : 0ull;
   do {  // entered with t < nt
     const u32 wt = readlane_u32(ch.ww, t);
-    const f32 sa = readlane_f32(ch.sa, t);
     u32 new2 = cur2;  // a word with an all-zero phi row keeps its topic
     wave_sync();  // the previous token's count writes before this token's reads
-    if (sa > 0.0f) {
+    if ((live >> t) & 1u) {
       visited += ds.occupied;
-      const u32 slot_fit = readlane_u32(ch.slot, t);
-      const PhiRow prow = CR && (slot_fit & kCountsFitBit) ? phi_count_row(P.phi_counts + (u64)wt * K, K, m.recip)
-                                                          : phi_row(P.phi + (u64)wt * K, K);
-      const f32 ux = readlane_f32(ch.ux, t), uw = readlane_f32(ch.uw, t);
+      const PhiRow prow = CR && ((fit >> t) & 1u) ? phi_count_row(P.phi_counts + (u64)wt * K, K, m.recip)
+                                                  : phi_row(P.phi + (u64)wt * K, K);
+      const TokenU u{ch.sa, ch.ux, ch.uw, t};
       Draw d;
-      if constexpr (NL > 0) d = draw_topic_regs<NL>(e, m.C, prow, sa, ux, uw);
-      else d = draw_topic_loop(m.Kt, m.C, uniform_u32(ds.cap / kMapLine), prow, sa, ux, uw, ds.e2);
+      if constexpr (NL > 0) d = draw_topic_regs<NL>(e, m.C, prow, u);
+      else d = draw_topic_loop(m.Kt, m.C, uniform_u32(ds.cap / kMapLine), prow, u, ds.e2);
       if (d.key2 != kNoSlot) {
         new2 = d.key2;
       } else {  // dense: the word's alias slot is read only now (about one token in eight)
-        const u32 sl = slot_fit & ~kCountsFitBit;
+        const u32 sl = readlane_u32(ch.slot, t) & ~kCountsFitBit;
         const AliasSlot a = P.alias[(u64)wt * K + sl];
         new2 = uniform_u32(2u * (readlane_f32(ch.uz, t) < a.prob ? sl : a.alias));  // (the load is uniform)
       }
@@ -1730,20 +1745,21 @@ __global__ __launch_bounds__(64) void k_stage_pick(const u32* table, u32 cap, u3
   wave_sync();
   const u32 lines = uniform_u32(cap / kMapLine);
   const PhiRow prow = phi_row(phi_row_k, K);
+  const TokenU u{sa, ux[i], uw[i], 0u};  // one token: the same values in every lane
   Draw d;
   if (use_regs && lines <= 8) {
     switch (lines) {
-      case 1: d = draw_topic_regs_lds<1>(m, prow, sa, ux[i], uw[i]); break;
-      case 2: d = draw_topic_regs_lds<2>(m, prow, sa, ux[i], uw[i]); break;
-      case 3: d = draw_topic_regs_lds<3>(m, prow, sa, ux[i], uw[i]); break;
-      case 4: d = draw_topic_regs_lds<4>(m, prow, sa, ux[i], uw[i]); break;
-      case 5: d = draw_topic_regs_lds<5>(m, prow, sa, ux[i], uw[i]); break;
-      case 6: d = draw_topic_regs_lds<6>(m, prow, sa, ux[i], uw[i]); break;
-      case 7: d = draw_topic_regs_lds<7>(m, prow, sa, ux[i], uw[i]); break;
-      default: d = draw_topic_regs_lds<8>(m, prow, sa, ux[i], uw[i]); break;
+      case 1: d = draw_topic_regs_lds<1>(m, prow, u); break;
+      case 2: d = draw_topic_regs_lds<2>(m, prow, u); break;
+      case 3: d = draw_topic_regs_lds<3>(m, prow, u); break;
+      case 4: d = draw_topic_regs_lds<4>(m, prow, u); break;
+      case 5: d = draw_topic_regs_lds<5>(m, prow, u); break;
+      case 6: d = draw_topic_regs_lds<6>(m, prow, u); break;
+      case 7: d = draw_topic_regs_lds<7>(m, prow, u); break;
+      default: d = draw_topic_regs_lds<8>(m, prow, u); break;
     }
   } else {
-    d = draw_topic_loop(m.Kt, m.C, lines, prow, sa, ux[i], uw[i], e2);
+    d = draw_topic_loop(m.Kt, m.C, lines, prow, u, e2);
   }
   if (lane == 0) out[i] = d.key2 == kNoSlot ? kNoSlot : d.slot;
 }
