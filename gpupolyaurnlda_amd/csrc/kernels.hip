@@ -727,10 +727,13 @@ struct Draw {
 };
 
 // Lane l* of the sparse pick and its exclusive prefix E.
-__device__ __forceinline__ u32 pick_lane(f32 A, f32 P, f32 tgt, f32& E) {
+__device__ __forceinline__ u32 pick_lane_only(f32 A, f32 P, f32 tgt) {
   const u64 pos = ballot(A > 0.0f);
   const u64 c = pos & ballot(P > tgt);
-  const u32 ls = c ? first_lane(c) : 63u - (u32)__clzll((long long)pos);
+  return c ? first_lane(c) : 63u - (u32)__clzll((long long)pos);
+}
+__device__ __forceinline__ u32 pick_lane(f32 A, f32 P, f32 tgt, f32& E) {
+  const u32 ls = pick_lane_only(A, P, tgt);
   E = ls ? readlane_f32(P, ls - 1) : 0.0f;
   return ls;
 }
@@ -802,8 +805,9 @@ __device__ __forceinline__ Draw draw_topic_regs(const u32 (&e)[NL], const u16* C
   d.slot = kNoSlot;
   if (!draw_sparse(u, sb)) return d;
   const f32 tgt = draw_target(u, sb);
-  f32 E;
-  const u32 ls = pick_lane(A, P, tgt, E);
+  const u32 ls = pick_lane_only(A, P, tgt);
+  // lane l holds P[l - 1] (0 in lane 0), so lane ls holds E, the exclusive prefix of its column
+  const f32 E = wave_shr1_zero(P);
   // every lane scans its own column in registers with the canonical network (NL <= 16 lines lie
   // in the network's first row: offsets 1, 2, 4, 8, entries below the offset kept); lane ls's
   // result is the one used

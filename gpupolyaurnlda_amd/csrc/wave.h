@@ -64,6 +64,12 @@ __device__ __forceinline__ f32 row_shr_zero(f32 v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 + N, 0xF, 0xF, true));
 }
 
+// Lane l receives lane l - 1's value, lane 0 receives 0 (DPP wave_shr:1): an inclusive scan shifted
+// into the exclusive one.
+__device__ __forceinline__ f32 wave_shr1_zero(f32 v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+
 // Broadcast steps: rows outside row_mask read 0 (and so keep their value after the add).
 __device__ __forceinline__ f32 row_bcast15_odd(f32 v) {  // rows 1, 3 <- lane 15 of rows 0, 2
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xA, 0xF, false));
