@@ -818,25 +818,23 @@ __device__ __forceinline__ Draw draw_topic_regs(const u32 (&e)[NL], const u16* C
   for (int off = 1; off < 16; off <<= 1)
 #pragma unroll
     for (int j = NL - 1; j >= off; --j) q[j] = q[j] + q[j - off];
-  u32 hit = kNoSlot;
+  // the picked line and its table value, packed as line << 16 | key2 (key2 < 2^16), so that one
+  // readlane of lane ls gives both
+  u32 pick = kNoSlot;
 #pragma unroll
   for (int j = NL - 1; j >= 0; --j) {
-    if (m[j] > 0.0f && E + q[j] > tgt) hit = (u32)j;  // carry 0: E + (0 + q) == E + q
+    if (m[j] > 0.0f && E + q[j] > tgt) pick = ((u32)j << 16) | e[j];  // carry 0: E + (0 + q) == E + q
   }
-  u32 js = readlane_u32(hit, ls);
-  if (js == kNoSlot) {  // no line passes the target (rounding): the last line with mass
-    u32 last = 0;
+  u32 pk = readlane_u32(pick, ls);
+  if (pk == kNoSlot) {  // no line passes the target (rounding): the last line with mass
+    u32 last = e[0];
 #pragma unroll
-    for (int j = 0; j < NL; ++j)
-      if (m[j] > 0.0f) last = (u32)j;
-    js = readlane_u32(last, ls);
+    for (int j = 1; j < NL; ++j)
+      if (m[j] > 0.0f) last = ((u32)j << 16) | e[j];
+    pk = readlane_u32(last, ls);
   }
-  u32 es = e[0];
-#pragma unroll
-  for (int j = 1; j < NL; ++j)
-    if ((u32)j == js) es = e[j];
-  d.key2 = readlane_u32(es, ls);
-  d.slot = js * 64 + ls;
+  d.key2 = pk & 0xFFFFu;
+  d.slot = (pk >> 16) * 64 + ls;
   return d;
 }
 
