@@ -136,7 +136,11 @@ LDA_HD bool map_is_empty(u32 e) { return map_count(e) == 0u; }
 // run of neighbouring topics, so the phi values its 64 slots gather lie in a few cache lines
 // (a multiplicative hash spreads them over the whole phi row, one cache line per slot).
 LDA_HD u32 map_home_mult(u32 K) { return 0xFFFFFFFFu / K; }
-LDA_HD u32 map_home(u32 key, u32 cap, u32 hm) { return (u32)(((u64)(key * hm) * (u64)cap) >> 32); }
+// floor(key * hm * cap / 2^32), formed as (key * cap) * hm: the same integer product, with key * cap
+// a 24-bit multiply on the device.  Needs key * cap < 2^32, which K <= 32767 (cap <= 33,792) gives.
+LDA_HD u32 map_home(u32 key, u32 cap, u32 hm) {
+  return (u32)(((u64)((key & 0xFFFFFFu) * (cap & 0xFFFFFFu)) * (u64)hm) >> 32);
+}
 
 // Displacement of an entry with home h stored in slot s.
 LDA_HD u32 map_disp(u32 s, u32 h, u32 cap) { return s >= h ? s - h : s + cap - h; }
