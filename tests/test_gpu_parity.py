@@ -208,8 +208,12 @@ def _tiny_corpus():
     return corpus.generate(corpus.TINY)
 
 
-def test_abi_tiny_100_iterations(orc):
-    """train.rs buffer loop through the five ABI calls vs the oracle Model (counts from C)."""
+@pytest.mark.parametrize("f32_rows", [None, 0])
+def test_abi_tiny_100_iterations(orc, f32_rows):
+    """train.rs buffer loop through the five ABI calls vs the oracle Model (counts from C).  With
+    f32_rows = 0 every word's sweep reads its u16 count row (by default the tiny model's rows are
+    all kept f32: 1 MiB of them is more than its whole Φ)."""
+    from gpupolyaurnlda_amd._lib import check, lib
     from gpupolyaurnlda_amd.corpus import TINY
     from gpupolyaurnlda_amd.train import Sampler, train
     c = _tiny_corpus()
@@ -220,6 +224,9 @@ def test_abi_tiny_100_iterations(orc):
     z = z0.copy()
     checkpoints = {1, 10, 100}
     with Sampler(TINY.alpha, TINY.beta, TINY.K, c["C"], buffer_size, max_D, seed=0) as smp:
+        if f32_rows is not None:
+            check(lib().gpulda_set_f32_rows(f32_rows), "set_f32_rows")
+
         def on_it(it):
             # the oracle replays the same iteration: phi step, sweep of every buffer, counts
             model.phi_step()
@@ -243,6 +250,12 @@ def test_abi_tiny_100_iterations(orc):
                 assert np.array_equal(p.view(np.uint32), model.prob.view(np.uint32)) and np.array_equal(a, model.al)
         train(smp, z, w, d, k_d, n_mc=100, buffer_size=buffer_size, max_D=max_D, on_iteration=on_it)
         assert smp.iteration == 100
+        if f32_rows == 0:
+            on, bad, fit = C.c_int(0), np.zeros(1, np.uint64), np.zeros(1, np.uint64)
+            check(lib().gpulda_check_count_rows(C.byref(on), bad.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                fit.ctypes.data_as(C.POINTER(C.c_uint64))), "check_count_rows")
+            say(f"ABI count rows: on={on.value} mismatches={int(bad[0])} words read as counts={int(fit[0])}")
+            assert on.value == 1 and bad[0] == 0 and fit[0] > 0
 
 
 @pytest.mark.parametrize("buffer_size,max_D", [(1 << 20, 1 << 16), (4096, 64)])
