@@ -285,7 +285,7 @@ __device__ __forceinline__ void keymap_insert(u16* Kt, u32 cap, u32 hm, u32 e2, 
     const u32 v = Kt[s];
     const u64 be = ballot(v == e2);
     const u32 fe = be ? first_lane(be) : 64u;
-    u32 prev = (u32)__shfl_up((int)v, 1, 64);
+    u32 prev = (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, false);  // wave_shr:1
     if (lane == 0) prev = carry;
     if (lane <= fe) Kt[s] = (u16)prev;
     if (be) break;
