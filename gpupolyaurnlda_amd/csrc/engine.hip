@@ -114,7 +114,7 @@ int lds_per_cu();
 
 int launch_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
                       AliasSlot* out, hipStream_t st, u8* wide = nullptr, u8* fit_out = nullptr,
-                      const u8* keep_f32 = nullptr) {
+                      const u8* keep_f32 = nullptr, const float2* recip = nullptr) {
   if (n_rows == 0) return 0;
   if (m > kMaxTopics) return fail("alias rows of %u entries (limit %u: slot indices are 16-bit in LDS)", m, kMaxTopics);
   // rows of up to 4096 slots: one workgroup of four wavefronts per row
@@ -122,10 +122,10 @@ int launch_alias_rows(f32* rows, u32 n_rows, u32 m, const unsigned long long* di
     const u64 lds = alias_block_lds(m);
     if (m <= kAliasBlockThreads * 4)
       k_alias_block<4><<<n_rows, kAliasBlockThreads, lds, st>>>(rows, n_rows, m, divisor, sigma, alpha, out,
-                                                                wide, fit_out, keep_f32);
+                                                                wide, fit_out, keep_f32, recip);
     else
       k_alias_block<16><<<n_rows, kAliasBlockThreads, lds, st>>>(rows, n_rows, m, divisor, sigma, alpha, out,
-                                                                 wide, fit_out, keep_f32);
+                                                                 wide, fit_out, keep_f32, recip);
     HCK(hipGetLastError());
     return 0;
   }
@@ -695,14 +695,14 @@ int phi_step(hipStream_t st, hipEvent_t mid = nullptr) {
   else
     CK(launch_phi_draw(S.n, S.phi, S.topic_sum, S.V, S.K, S.beta, 0, S.iter_dev, S.seed, S.pois, st));
   if (mid) HCK(hipEventRecord(mid, st));
-  if (S.count_rows) {
-    k_topic_recip<<<(S.K + 1 + 255) / 256, 256, 0, st>>>(S.topic_sum, S.topic_recip, S.K);
-    HCK(hipGetLastError());
+  k_topic_recip<<<(S.K + 1 + 255) / 256, 256, 0, st>>>(S.topic_sum, S.topic_recip, S.K);
+  HCK(hipGetLastError());
+  if (S.count_rows)
     CK(launch_alias_rows(S.phi, S.V, S.K, S.topic_sum, S.sigma_a, S.alpha, S.alias, st, S.counts_wide,
-                         S.counts_fit, S.keep_f32));
-  } else {
-    CK(launch_alias_rows(S.phi, S.V, S.K, S.topic_sum, S.sigma_a, S.alpha, S.alias, st));
-  }
+                         S.counts_fit, S.keep_f32, S.topic_recip));
+  else
+    CK(launch_alias_rows(S.phi, S.V, S.K, S.topic_sum, S.sigma_a, S.alpha, S.alias, st, nullptr, nullptr, nullptr,
+                         S.topic_recip));
   CK(fill_u32(S.n, (u64)S.V * S.K, 0, st));
   return 0;
 }

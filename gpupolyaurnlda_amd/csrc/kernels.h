@@ -161,9 +161,10 @@ constexpr u32 kAliasBlockThreads = 256;
 LDA_HD u64 alias_block_lds(u32 m) { return (u64)((m + 3) & ~3u) * 14 + (kAliasBlockThreads + 4) * 4; }
 // fit_out (optional) receives whether the sweep is to read each row's u16 counts (written by
 // k_phi_draw): none passed 16 bits (wide, cleared here for the next step) and keep_f32 (optional)
-// does not mark the row.
+// does not mark the row.  recip (optional, with divisor) holds (RN(1 / d), d) per slot for
+// d = (f32)divisor (k_topic_recip); given only for rows of integer counts (the Poisson draws).
 template <int PER>
 __global__ void k_alias_block(f32* rows, u32 n_rows, u32 m, const unsigned long long* divisor, f32* sigma, f32 alpha,
-                              AliasSlot* out, u8* wide, u8* fit_out, const u8* keep_f32);
+                              AliasSlot* out, u8* wide, u8* fit_out, const u8* keep_f32, const float2* recip);
 
 }  // namespace lda

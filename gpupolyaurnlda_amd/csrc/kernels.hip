@@ -170,7 +170,7 @@ __global__ void k_alias_rows(f32* __restrict__ rows, u32 n_rows, u32 m, const un
   // k / per as a multiply-high by ceil(2^32 / per): exact for k < 2^16 and per < 2^16 (the
   // reciprocal's excess adds less than k / 2^32 < 2^-16 < 1 / per to a quotient whose fraction
   // is at most 1 - 1 / per); per == 1 has pad 0, so its quotient is not used
-  const u32 per_magic = per > 1 ? (u32)((0x100000000ull + per - 1) / per) : 0u;
+  const u32 per_magic = per > 1 ? 0xFFFFFFFFu / per + 1u : 0u;  // ceil(2^32 / per), per <= 2^16
   auto xp = [&](u32 k) { return k + pad * __umulhi(k, per_magic); };
 
   f32* x = rows + (u64)row * m;
@@ -359,7 +359,8 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
                                                     const unsigned long long* __restrict__ divisor,
                                                     f32* __restrict__ sigma, f32 alpha, AliasSlot* __restrict__ out,
                                                     u8* __restrict__ wide, u8* __restrict__ fit_out,
-                                                    const u8* __restrict__ keep_f32) {
+                                                    const u8* __restrict__ keep_f32,
+                                                    const float2* __restrict__ recip) {
   extern __shared__ unsigned char alias_smem[];
   __shared__ unsigned long long red[3][kAliasBlockThreads / 64];
   __shared__ f32 shT;
@@ -375,7 +376,7 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
   f32* lp = xs;  // after B1 the row values are dead: lights' probabilities, indexed like pre
   const u32 per = (m + kAliasBlockThreads - 1) / kAliasBlockThreads;  // <= PER
   const u32 pad = (per & 1u) ^ 1u;  // thread stride per + pad odd: distinct banks
-  const u32 per_magic = per > 1 ? (u32)((0x100000000ull + per - 1) / per) : 0u;
+  const u32 per_magic = per > 1 ? 0xFFFFFFFFu / per + 1u : 0u;  // ceil(2^32 / per), per <= 2^16
   auto xp = [&](u32 k) { return k + pad * __umulhi(k, per_magic); };
   f32* x = rows + (u64)row * m;
   AliasSlot* o = out + (u64)row * m;
@@ -383,8 +384,7 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
   // whether the sweep is to read the row's u16 counts, read with the row by a thread off the canonical
   // sum's path (wavefront 1); the row's mark is cleared for the next step
   const bool fit_lane = fit_out && t == 64;
-  u32 fit = 0;
-  if (fit_lane) fit = wide[row] || (keep_f32 && keep_f32[row]) ? 0u : 1u;
+  u32 unfit = 0;
 
   // A: loads (coalesced), normalisation, the row staged in LDS; the canonical sum by wavefront 0
   {
@@ -394,7 +394,39 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
       const u32 k = (u32)c * kAliasBlockThreads + t;
       xv[c] = k < m ? x[k] : 0.0f;
     }
-    if (divisor) {
+    // a wavefront whose values are all counts below 2^16 divides through the divisors' reciprocals
+    // (phi_from_count_f: the bits of the division, as the sweep's count rows recover them); the
+    // reciprocals are loaded with the row
+    const bool with_recip = divisor && recip;
+    float2 rc[PER];
+    if (with_recip) {
+#pragma unroll
+      for (int c = 0; c < PER; ++c) {
+        const u32 k = (u32)c * kAliasBlockThreads + t;
+        rc[c] = k < m ? recip[k] : make_float2(0.0f, 1.0f);
+      }
+    }
+    if (fit_lane) {  // the flags, loaded behind the row and its reciprocals
+      unfit = wide[row];
+      if (keep_f32) unfit |= keep_f32[row];
+    }
+    bool by_recip = false;
+    if (with_recip) {
+      bool big = false;
+#pragma unroll
+      for (int c = 0; c < PER; ++c) big = big || xv[c] > 65535.0f;
+      by_recip = ballot(big) == 0ull;
+    }
+    if (by_recip) {
+#pragma unroll
+      for (int c = 0; c < PER; ++c) {
+        const u32 k = (u32)c * kAliasBlockThreads + t;
+        if (k < m) {
+          xv[c] = phi_from_count_f(xv[c], rc[c].x, rc[c].y);
+          x[k] = xv[c];
+        }
+      }
+    } else if (divisor) {
       u64 dr[PER];
 #pragma unroll
       for (int c = 0; c < PER; ++c) {
@@ -419,7 +451,7 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
   }
   __syncthreads();
   if (fit_lane) {
-    fit_out[row] = (u8)fit;
+    fit_out[row] = unfit ? (u8)0 : (u8)1;
     wide[row] = 0;
   }
   if (wave == 0) {
@@ -562,9 +594,9 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
   }
 }
 template __global__ void k_alias_block<4>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u8*, u8*,
-                                          const u8*);
+                                          const u8*, const float2*);
 template __global__ void k_alias_block<16>(f32*, u32, u32, const unsigned long long*, f32*, f32, AliasSlot*, u8*, u8*,
-                                           const u8*);
+                                           const u8*, const float2*);
 
 // ---------------------------------------------------------------------------------------------
 // z-sweep, one wavefront per document, the doc-topic state resident in LDS: a count per topic and

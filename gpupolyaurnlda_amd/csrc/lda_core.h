@@ -174,12 +174,12 @@ struct AliasSlot {
 // overflow for x < 2^16, 1 <= d < 2^64), so checking every 16-bit significand of x against every
 // 24-bit significand of d covers all inputs; x = 0 gives +0, and so does y = 0 with d = 1 (the
 // value used for S_k = 0, whose counts are all 0).
-LDA_HD f32 phi_from_count(u32 x, f32 y, f32 d) {
-  const f32 xf = (f32)x;
+LDA_HD f32 phi_from_count_f(f32 xf, f32 y, f32 d) {  // xf: a count below 2^16, as f32
   const f32 q0 = xf * y;
   const f32 r = __builtin_fmaf(-q0, d, xf);
   return __builtin_fmaf(r, y, q0);
 }
+LDA_HD f32 phi_from_count(u32 x, f32 y, f32 d) { return phi_from_count_f((f32)x, y, d); }
 
 LDA_HD u32 draw_alias_slot(const AliasSlot* row, u32 n, u32 r_slot, u32 r_keep) {
   u32 slot = bounded(r_slot, n);
