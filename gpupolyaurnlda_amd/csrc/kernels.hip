@@ -77,15 +77,45 @@ __global__ void k_phi_draw(const u32* __restrict__ n, f32* __restrict__ phi, uns
   u32 v0 = blockIdx.y * rows_per_block;
   u32 v1 = v0 + rows_per_block < V ? v0 + rows_per_block : V;
   u64 acc = 0;
-  for (u32 v = v0; v < v1; ++v) {
-    u64 idx = (u64)v * K + k;
-    u32 x = poisson_draw(n[idx], beta, tables, k, v, iter, kTagPhi, k0, k1);
-    phi[idx] = (f32)x;
+  // kPhiBatch words at a time, so that their counts and then their table slots are loaded
+  // together rather than one dependent pair of loads per word: the draw of every cell is
+  // poisson_draw's (the table draw below n = kPoisTables, PTRS above it)
+  constexpr u32 kPhiBatch = 8;
+  auto put = [&](u32 v, u32 xb) {
+    const u64 idx = (u64)v * K + k;
+    phi[idx] = (f32)xb;
     if (counts) {  // the u16 count row, and its word marked when a count passes 16 bits
-      counts[idx] = x <= 0xFFFFu ? (u16)x : (u16)0xFFFFu;
-      if (x > 0xFFFFu) wide[v] = 1;
+      counts[idx] = xb <= 0xFFFFu ? (u16)xb : (u16)0xFFFFu;
+      if (xb > 0xFFFFu) wide[v] = 1;
     }
-    acc += x;
+    acc += xb;
+  };
+  for (u32 vb = v0; vb < v1; vb += kPhiBatch) {
+    u32 nv[kPhiBatch], x[kPhiBatch], rk[kPhiBatch];
+    AliasSlot sl[kPhiBatch];
+#pragma unroll
+    for (u32 b = 0; b < kPhiBatch; ++b) nv[b] = vb + b < v1 ? n[(u64)(vb + b) * K + k] : 0u;
+#pragma unroll
+    for (u32 b = 0; b < kPhiBatch; ++b) {
+      const u32x4 r = philox_tagged(k, vb + b, iter, kTagPhi, 0, k0, k1);
+      x[b] = bounded(r.x, kPoisValues);
+      rk[b] = r.y;
+      const u32 tb = nv[b] < kPoisTables ? nv[b] : 0u;
+      sl[b] = tables[(u64)tb * kPoisValues + x[b]];
+    }
+    bool ptrs = false;
+#pragma unroll
+    for (u32 b = 0; b < kPhiBatch; ++b) {
+      if (vb + b < v1 && nv[b] < kPoisTables) put(vb + b, u24(rk[b]) < sl[b].prob ? x[b] : sl[b].alias);
+      ptrs = ptrs || nv[b] >= kPoisTables;
+    }
+    if (ptrs) {  // the words whose count is at least kPoisTables, one at a time (their count loaded again)
+#pragma unroll 1
+      for (u32 v = vb; v < v1 && v < vb + kPhiBatch; ++v) {
+        const u32 nb = n[(u64)v * K + k];
+        if (nb >= kPoisTables) put(v, poisson_ptrs((f64)nb + (f64)beta, k, v, iter, kTagPhi, k0, k1));
+      }
+    }
   }
   if (acc) atomicAdd(&topic_sum[k], (unsigned long long)acc);
 }
