@@ -80,7 +80,7 @@ __global__ void k_phi_draw(const u32* __restrict__ n, f32* __restrict__ phi, uns
   // kPhiBatch words at a time, so that their counts and then their table slots are loaded
   // together rather than one dependent pair of loads per word: the draw of every cell is
   // poisson_draw's (the table draw below n = kPoisTables, PTRS above it)
-  constexpr u32 kPhiBatch = 8;
+  constexpr u32 kPhiBatch = 16;
   auto put = [&](u32 v, u32 xb) {
     const u64 idx = (u64)v * K + k;
     phi[idx] = (f32)xb;
