@@ -1697,17 +1697,26 @@ __global__ void k_count(const CountTask* __restrict__ tasks, u32 n_tasks, const 
                         u32* __restrict__ n, u32 K, u32 waves_per_block) {
   extern __shared__ u32 count_smem[];
   const u32 lane = lane_id();
-  const u32 wave = threadIdx.x >> 6;
+  const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: the task's load is scalar
   const u32 task = blockIdx.x * waves_per_block + wave;
   if (task >= n_tasks) return;
+  const CountTask tk = tasks[task];
   // one histogram of K bins per wavefront, K rounded up to a multiple of 4 so that every
   // histogram starts 16-byte aligned; bins are cleared four at a time
   const u32 Kp = (K + 3u) & ~3u;
   u32* H = count_smem + (u64)wave * Kp;
   for (u32 k = 4 * lane; k < Kp; k += 256) *reinterpret_cast<uint4*>(&H[k]) = make_uint4(0u, 0u, 0u, 0u);
   wave_sync();
-  const CountTask tk = tasks[task];
-  for (u32 p = lane; p < tk.len; p += 64) atomicAdd(&H[zs[tk.p0 + p]], 1u);
+  // eight loads of 64 topics in flight before their LDS adds (one load and add at a time leaves
+  // each step waiting on the load)
+  for (u32 p = lane; p < tk.len; p += 8 * 64) {
+    u32 zb[8];
+#pragma unroll
+    for (u32 b = 0; b < 8; ++b) zb[b] = p + 64 * b < tk.len ? (u32)zs[tk.p0 + p + 64 * b] : 0u;
+#pragma unroll
+    for (u32 b = 0; b < 8; ++b)
+      if (p + 64 * b < tk.len) atomicAdd(&H[zb[b]], 1u);
+  }
   wave_sync();
   // bins read one per lane, consecutive, so each atomic instruction covers two cache lines
   u32* row = n + (u64)tk.word * K;
@@ -1721,11 +1730,18 @@ __global__ void k_count(const CountTask* __restrict__ tasks, u32 n_tasks, const 
 // one to n_wk directly.
 __global__ void k_count_direct(const CountTask* __restrict__ tasks, u32 n_tasks, const u16* __restrict__ zs,
                                u32* __restrict__ n, u32 K) {
-  const u32 task = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const u32 task = blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (task >= n_tasks) return;
   const CountTask tk = tasks[task];
   u32* row = n + (u64)tk.word * K;
-  for (u32 p = lane_id(); p < tk.len; p += 64) atomicAdd(&row[zs[tk.p0 + p]], 1u);
+  for (u32 p = lane_id(); p < tk.len; p += 8 * 64) {  // eight loads in flight, as in k_count
+    u32 zb[8];
+#pragma unroll
+    for (u32 b = 0; b < 8; ++b) zb[b] = p + 64 * b < tk.len ? (u32)zs[tk.p0 + p + 64 * b] : 0u;
+#pragma unroll
+    for (u32 b = 0; b < 8; ++b)
+      if (p + 64 * b < tk.len) atomicAdd(&row[zb[b]], 1u);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
