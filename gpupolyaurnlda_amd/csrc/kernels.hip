@@ -1645,19 +1645,33 @@ __global__ __launch_bounds__(1024) void k_tile_reorder(const u16* __restrict__ i
   const u32 m = (u32)(n - t0 < kCountTile ? n - t0 : kCountTile);
   const u32 m4 = m & ~3u;
   const u16* rank = R.rank;
-  // four entries per lane with 8-byte loads of values and ranks
-  for (u32 l = 4 * threadIdx.x; l < m4; l += 4 * blockDim.x) {
-    const uint2 v = *reinterpret_cast<const uint2*>(in + t0 + l);
-    const uint2 r = *reinterpret_cast<const uint2*>(rank + t0 + l);
+  // exclusive prefix counts of run starts per bitmap word (one word per thread: a block scan)
+  static_assert(kWords == 1024, "one bitmap word per thread");
+  const u32 wb = R.bits[(u64)blockIdx.x * kWords + threadIdx.x];
+  // four entries per lane with 8-byte loads of values and ranks; a full tile (launched with 1024
+  // threads) has all eight of each lane's load pairs in flight before its LDS stores
+  auto put4 = [&](uint2 v, uint2 r) {
     tile[r.x & 0xFFFFu] = (u16)(v.x & 0xFFFFu);
     tile[r.x >> 16] = (u16)(v.x >> 16);
     tile[r.y & 0xFFFFu] = (u16)(v.y & 0xFFFFu);
     tile[r.y >> 16] = (u16)(v.y >> 16);
+  };
+  constexpr u32 kFullPer = kCountTile / (4 * 1024);
+  if (m == kCountTile && blockDim.x == 1024) {
+    uint2 v[kFullPer], r[kFullPer];
+#pragma unroll
+    for (u32 i = 0; i < kFullPer; ++i) {
+      const u32 l = 4 * (threadIdx.x + i * 1024);
+      v[i] = *reinterpret_cast<const uint2*>(in + t0 + l);
+      r[i] = *reinterpret_cast<const uint2*>(rank + t0 + l);
+    }
+#pragma unroll
+    for (u32 i = 0; i < kFullPer; ++i) put4(v[i], r[i]);
+  } else {
+    for (u32 l = 4 * threadIdx.x; l < m4; l += 4 * blockDim.x)
+      put4(*reinterpret_cast<const uint2*>(in + t0 + l), *reinterpret_cast<const uint2*>(rank + t0 + l));
+    for (u32 l = m4 + threadIdx.x; l < m; l += blockDim.x) tile[rank[t0 + l]] = in[t0 + l];
   }
-  for (u32 l = m4 + threadIdx.x; l < m; l += blockDim.x) tile[rank[t0 + l]] = in[t0 + l];
-  // exclusive prefix counts of run starts per bitmap word (one word per thread: a block scan)
-  static_assert(kWords == 1024, "one bitmap word per thread");
-  const u32 wb = R.bits[(u64)blockIdx.x * kWords + threadIdx.x];
   wbits[threadIdx.x] = wb;
   u32 c = (u32)__popc(wb);
   const u32 lane = lane_id();
@@ -1672,7 +1686,7 @@ __global__ __launch_bounds__(1024) void k_tile_reorder(const u16* __restrict__ i
   for (u32 q = 0; q < wave; ++q) before += wtot[q];
   wpref[threadIdx.x] = before + c - (u32)__popc(wb);
   __syncthreads();
-  // consecutive lanes write consecutive ranks (a run's entries go to consecutive places); four
+  // consecutive lanes write consecutive ranks (a run's entries go to consecutive places); eight
   // independent places per lane in flight
   const uint2* runs = R.runs + R.run_off[blockIdx.x];
   auto place = [&](u32 j) -> u32 {
@@ -1683,12 +1697,12 @@ __global__ __launch_bounds__(1024) void k_tile_reorder(const u16* __restrict__ i
   };
   const u32 B = blockDim.x;
   u32 j = threadIdx.x;
-  for (; j + 3 * B < m; j += 4 * B) {
-    const u32 d0 = place(j), d1 = place(j + B), d2 = place(j + 2 * B), d3 = place(j + 3 * B);
-    out[d0] = tile[j];
-    out[d1] = tile[j + B];
-    out[d2] = tile[j + 2 * B];
-    out[d3] = tile[j + 3 * B];
+  for (; j + 7 * B < m; j += 8 * B) {
+    u32 d[8];
+#pragma unroll
+    for (u32 i = 0; i < 8; ++i) d[i] = place(j + i * B);
+#pragma unroll
+    for (u32 i = 0; i < 8; ++i) out[d[i]] = tile[j + i * B];
   }
   for (; j < m; j += B) out[place(j)] = tile[j];
 }
