@@ -777,46 +777,43 @@ int launch_sweep_schedule(const SweepParams& P0, Segment* sg, hipStream_t st) {
   P.doc_order = sg->doc_lpt;
   const u64 lds = sweep_lds_bytes(sg);
   if (lds > (u64)max_lds_per_block()) return fail("doc-topic map exceeds LDS");
-  if (S.schedule == kSchedPersistentSplit) {
-    u32 n_long = 0;
-    while (n_long < sg->n_docs && sg->lens_desc[n_long] > S.block_min_len) ++n_long;
-    CK(fill_u32(S.heads, 2, 0, st));
-    if (n_long > 0) {
-      HCK(hipEventRecord(S.fork, st));
-      HCK(hipStreamWaitEvent(S.stream2, S.fork, 0));
-      const u32 per_cu = std::max<u32>(1, std::min<u32>(S.waves_per_cu, (u32)(lds_per_cu() / lds)));
-      const u32 waves = std::min<u32>((u32)S.cu_count * per_cu, n_long);
-      k_sweep_persistent_long<<<waves, 64, lds, S.stream2>>>(P, S.heads + 1, n_long);
-      HCK(hipGetLastError());
-      HCK(hipEventRecord(S.join, S.stream2));
-    }
-    if (n_long < sg->n_docs) {
-      SweepParams Q = P;
-      Q.doc_order = sg->doc_lpt + n_long;
-      const u64 lds_s = doc_state_bytes(sg->lens_desc[n_long], S.K);
-      CK(launch_persistent_short(Q, S.heads, sg->n_docs - n_long, lds_s, S.waves_per_cu, st));
-    }
-    if (n_long > 0) HCK(hipStreamWaitEvent(st, S.join, 0));
-    return 0;
-  }
+  // Long documents and the rest run side by side: the long documents' launch goes on the launch
+  // stream, the rest on the second stream behind an event, so that the long documents' wavefronts
+  // are placed first (a long document is one serial chain; started behind a persistent launch
+  // that holds every slot, it ends that much later).
   u32 n_long = 0;
-  if (S.schedule == kSchedPersistentBlock && S.block_min_len > 0)
+  if (S.schedule == kSchedPersistentSplit)
+    while (n_long < sg->n_docs && sg->lens_desc[n_long] > S.block_min_len) ++n_long;
+  else if (S.schedule == kSchedPersistentBlock && S.block_min_len > 0)
     while (n_long < sg->n_docs && sg->lens_desc[n_long] >= S.block_min_len) ++n_long;
   CK(fill_u32(S.heads, 2, 0, st));
-  if (n_long > 0) {
+  const bool both = n_long > 0 && n_long < sg->n_docs;
+  if (both) {
     HCK(hipEventRecord(S.fork, st));
     HCK(hipStreamWaitEvent(S.stream2, S.fork, 0));
-    u32 blocks = std::min<u32>(n_long, (u32)S.cu_count * 2);
-    k_sweep_block<<<blocks, 256, lds, S.stream2>>>(P, S.heads + 1, n_long);
+  }
+  if (n_long > 0) {
+    if (S.schedule == kSchedPersistentSplit) {
+      const u32 per_cu = std::max<u32>(1, std::min<u32>(S.waves_per_cu, (u32)(lds_per_cu() / lds)));
+      const u32 waves = std::min<u32>((u32)S.cu_count * per_cu, n_long);
+      k_sweep_persistent_long<<<waves, 64, lds, st>>>(P, S.heads + 1, n_long);
+    } else {
+      const u32 blocks = std::min<u32>(n_long, (u32)S.cu_count * 2);
+      k_sweep_block<<<blocks, 256, lds, st>>>(P, S.heads + 1, n_long);
+    }
     HCK(hipGetLastError());
-    HCK(hipEventRecord(S.join, S.stream2));
   }
   if (n_long < sg->n_docs) {
     SweepParams Q = P;
     Q.doc_order = sg->doc_lpt + n_long;
-    CK(launch_persistent_short(Q, S.heads, sg->n_docs - n_long, lds, S.waves_per_cu, st));
+    const u64 lds_s = S.schedule == kSchedPersistentSplit ? doc_state_bytes(sg->lens_desc[n_long], S.K) : lds;
+    hipStream_t rs = both ? S.stream2 : st;
+    CK(launch_persistent_short(Q, S.heads, sg->n_docs - n_long, lds_s, S.waves_per_cu, rs));
+    if (both) {
+      HCK(hipEventRecord(S.join, S.stream2));
+      HCK(hipStreamWaitEvent(st, S.join, 0));
+    }
   }
-  if (n_long > 0) HCK(hipStreamWaitEvent(st, S.join, 0));
   return 0;
 }
 
