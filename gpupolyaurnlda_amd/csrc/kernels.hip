@@ -1721,22 +1721,27 @@ __global__ void k_count(const CountTask* __restrict__ tasks, u32 n_tasks, const 
   u32* H = count_smem + (u64)wave * Kp;
   for (u32 k = 4 * lane; k < Kp; k += 256) *reinterpret_cast<uint4*>(&H[k]) = make_uint4(0u, 0u, 0u, 0u);
   wave_sync();
-  // eight loads of 64 topics in flight before their LDS adds (one load and add at a time leaves
+  // sixteen loads of 64 topics in flight before their LDS adds (one load and add at a time leaves
   // each step waiting on the load)
-  for (u32 p = lane; p < tk.len; p += 8 * 64) {
-    u32 zb[8];
+  for (u32 p = lane; p < tk.len; p += 16 * 64) {
+    u32 zb[16];
 #pragma unroll
-    for (u32 b = 0; b < 8; ++b) zb[b] = p + 64 * b < tk.len ? (u32)zs[tk.p0 + p + 64 * b] : 0u;
+    for (u32 b = 0; b < 16; ++b) zb[b] = p + 64 * b < tk.len ? (u32)zs[tk.p0 + p + 64 * b] : 0u;
 #pragma unroll
-    for (u32 b = 0; b < 8; ++b)
+    for (u32 b = 0; b < 16; ++b)
       if (p + 64 * b < tk.len) atomicAdd(&H[zb[b]], 1u);
   }
   wave_sync();
-  // bins read one per lane, consecutive, so each atomic instruction covers two cache lines
+  // bins read one per lane, consecutive, so each atomic instruction covers two cache lines; four
+  // bins per lane read before their atomics
   u32* row = n + (u64)tk.word * K;
-  for (u32 k = lane; k < K; k += 64) {
-    const u32 c = H[k];
-    if (c) atomicAdd(&row[k], c);
+  for (u32 k = lane; k < K; k += 4 * 64) {
+    u32 c[4];
+#pragma unroll
+    for (u32 j = 0; j < 4; ++j) c[j] = k + 64 * j < K ? H[k + 64 * j] : 0u;
+#pragma unroll
+    for (u32 j = 0; j < 4; ++j)
+      if (c[j]) atomicAdd(&row[k + 64 * j], c[j]);
   }
 }
 
