@@ -394,8 +394,10 @@ __global__ __launch_bounds__(256) void k_alias_block(f32* __restrict__ rows, u32
   extern __shared__ unsigned char alias_smem[];
   __shared__ unsigned long long red[3][kAliasBlockThreads / 64];
   __shared__ f32 shT;
-  const u32 row = blockIdx.x;
-  if (row >= n_rows) return;
+  // rows last to first: the Poisson kernel wrote them first to last, so the rows still in the MALL
+  // are read before later reads displace them
+  if (blockIdx.x >= n_rows) return;
+  const u32 row = n_rows - 1u - blockIdx.x;
   const u32 t = threadIdx.x;
   const u32 lane = lane_id();
   const u32 wave = t >> 6;
